@@ -1,0 +1,197 @@
+/*
+ * pm_gather.h -- C ABI of the MI355X (gfx950) photon-gather library.
+ *
+ * This is the drop-in boundary for the kNN photon gather of a Jensen-style
+ * photon mapper.  Every entry point is plain C: opaque handles, plain
+ * pointers and sizes, an int status.  Streams are hipStream_t passed as
+ * void* (NULL = the null stream).  Reference citations are relative to the
+ * yumcyaWiz/photon_mapping checkout.
+ *
+ * Semantics kept from the reference:
+ *   - neighbour sets are the k smallest (FP32 d^2, photon index) keys, with
+ *     d^2 = ((0 + dx*dx) + dy*dy) + dz*dz evaluated without FMA
+ *     (include/photon_map.h:33-41, 107-147);
+ *   - neighbour lists come out farthest first, i.e. in descending
+ *     (d^2, index) order, and r^2 is the k-th smallest d^2
+ *     (include/photon_map.h:166-183);
+ *   - k > N returns all N photons; an empty map returns an empty set with
+ *     r^2 = 0 (include/photon_map.h:112, 173-174);
+ *   - photon indices are positions in the array handed to the map
+ *     (include/photon_map.h:195-200).
+ */
+#ifndef PM_GATHER_H
+#define PM_GATHER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define PM_OK 0
+#define PM_ERR_INVALID_ARG 1 /* bad size, null pointer, NaN input, k < 1   */
+#define PM_ERR_HIP 2         /* a HIP runtime call failed                  */
+#define PM_ERR_OOM 3         /* device allocation failed                   */
+#define PM_ERR_NOT_BUILT 4   /* query on a map whose build() has not run   */
+#define PM_ERR_UNSUPPORTED 5 /* request outside a documented limit         */
+
+/* ---- memory spaces of pointer arguments --------------------------------- */
+#define PM_MEM_DEVICE 0 /* pointers are device memory; call is asynchronous  */
+#define PM_MEM_HOST 1   /* pointers are host memory; call stages and syncs   */
+
+/* ---- material types ------------------------------------------------------ */
+/* include/material.h:73-245; only Lambert is non-zero inside the estimate.   */
+#define PM_MATERIAL_LAMBERT 0
+#define PM_MATERIAL_MIRROR 1
+#define PM_MATERIAL_GLASS 2
+
+typedef struct pm_context pm_context;
+typedef struct pm_photonmap pm_photonmap;
+
+/* One gather query (40 bytes): the surface point of a diffuse hit.
+ * pos = SurfaceInfo::position, ns = SurfaceInfo::shadingNormal (not
+ * necessarily unit length, include/triangle.h:64-71), wo = the outgoing
+ * direction (-ray.direction), material = index into the material table.
+ * Replaces the (wo, IntersectInfo) pair of
+ * PhotonMapping::computeRadianceWithPhotonMap, include/integrator.h:133-134. */
+typedef struct pm_query {
+  float pos[3];
+  float ns[3];
+  float wo[3];
+  int32_t material;
+} pm_query;
+
+/* One material (16 bytes): Lambert albedo rho (include/material.h:73-105). */
+typedef struct pm_material {
+  float rho[3];
+  int32_t type;
+} pm_material;
+
+/* Per-call statistics of the last gather/knn/adjoint on a context.  Kernel
+ * times are HIP-event times on the call's stream (filled only when the
+ * PM_OPT_TIMING option is on, which synchronises the stream). */
+typedef struct pm_stats {
+  double sort_ms;      /* query Morton keys + radix sort + group anchors  */
+  double gather_ms;    /* the persistent gather kernel                    */
+  double post_ms;      /* knn row sort / adjoint flush                    */
+  double build_ms;     /* last photon-map build (incl. anchors)           */
+  int64_t groups;      /* query groups processed by the last launch       */
+  int64_t fallbacks;   /* groups that re-ran with an unbounded radius     */
+  int64_t refinements; /* histogram refinement passes in the last launch  */
+} pm_stats;
+
+/* Options for pm_context_set_option. */
+#define PM_OPT_TIMING 1         /* 0/1: record HIP events (syncs stream)    */
+#define PM_OPT_WAVES_PER_CU 2   /* persistent waves per CU (default 8)      */
+#define PM_OPT_ANCHOR_STRIDE 3  /* photons per radius anchor (default 8)    */
+#define PM_OPT_LEAF_SIZE 4      /* default kd-tree leaf size (default 32)   */
+#define PM_OPT_XCD_AWARE 5      /* 0/1: per-XCD work queues (default 1)     */
+#define PM_OPT_COUNT_STATS 6    /* 0/1: count fallbacks/refinements         */
+
+/* ---- errors / version ----------------------------------------------------- */
+/* Thread-local text of the last failing call ("" when none). */
+const char* pm_last_error(void);
+int pm_version(void);
+
+/* ---- context --------------------------------------------------------------- */
+int pm_context_create(int device, pm_context** out_ctx);
+int pm_context_destroy(pm_context* ctx);
+int pm_context_set_option(pm_context* ctx, int option, int64_t value);
+int pm_context_get_stats(pm_context* ctx, pm_stats* out);
+int pm_context_synchronize(pm_context* ctx, void* stream);
+
+/* ---- photon map ---------------------------------------------------------------
+ * Replaces PhotonMap::setPhotons + the KdTree it owns
+ * (include/photon_map.h:186-206).  The map copies the photons; positions
+ * must be finite.  pos_soa / wi_soa / flux_soa are each 3*n floats laid out
+ * as [x0..x_{n-1}, y0..y_{n-1}, z0..z_{n-1}] (flux: r, g, b).
+ * Photon.throughput is the flux, Photon.wi the incident direction
+ * (include/photon_map.h:10-22). */
+int pm_photonmap_create(pm_context* ctx, int64_t n, const float* pos_soa,
+                        const float* wi_soa, const float* flux_soa, int mem,
+                        pm_photonmap** out_map);
+/* Same, from the reference's 36-byte Photon records
+ * {throughput[3], position[3], wi[3]} (include/photon_map.h:10-13). */
+int pm_photonmap_create_aos(pm_context* ctx, int64_t n, const float* photons,
+                            int mem, pm_photonmap** out_map);
+int pm_photonmap_destroy(pm_photonmap* map);
+/* Replaces PhotonMap::build / KdTree::buildTree (include/photon_map.h:157-164,
+ * 202-206): GPU Morton sort, balanced tree over leaf buckets of leaf_size
+ * photons (0 = context default; 4..256). */
+int pm_photonmap_build(pm_photonmap* map, int leaf_size, void* stream);
+/* Replaces the flux of every photon (3*n SoA, original photon order)
+ * without rebuilding the tree. */
+int pm_photonmap_update_flux(pm_photonmap* map, const float* flux_soa, int mem,
+                             void* stream);
+/* PhotonMap::getNPhotons (include/photon_map.h:194). */
+int64_t pm_photonmap_size(const pm_photonmap* map);
+/* PhotonMap::getIthPhoton (include/photon_map.h:195): writes the 9 floats
+ * {throughput, position, wi} of photon i to host memory. */
+int pm_photonmap_get_photon(pm_photonmap* map, int64_t i, float* out9);
+
+/* ---- kNN ---------------------------------------------------------------------
+ * Batched PhotonMap::queryKNearestPhotons (include/photon_map.h:208-211).
+ * q_soa: 3*m floats [x.., y.., z..].  out_idx: m rows of k int32, each row
+ * farthest first; slots past min(k, N) hold -1.  out_r2: m floats.
+ * k <= 1024. */
+int pm_knn(pm_photonmap* map, int64_t m, const float* q_soa, int k,
+           int32_t* out_idx, float* out_r2, int mem, void* stream);
+
+/* ---- radiance gather ----------------------------------------------------------
+ * Batched PhotonMapping::computeRadianceWithPhotonMap
+ * (include/integrator.h:133-152) and computeCausticsWithPhotonMap (:155-175,
+ * same formula on another map): for each query
+ *   L = [sum_{i in kNN} f(wo, wi_i) * flux_i] / (n_emit * PI * r^2)
+ * with f = rho/PI when dot(wo, ns) >= 0 and dot(wi_i, ns) >= 0, else 0.
+ * out_rgb: m*3 floats; out_r2: m floats (may be NULL); out_key (may be NULL):
+ * m uint64 neighbour-set keys (bits(r^2) << 32 | index of the k-th
+ * neighbour), which encode the neighbour set exactly and are what
+ * pm_gather_adjoint reuses. */
+int pm_gather_radiance(pm_photonmap* map, int64_t m, const pm_query* q,
+                       const pm_material* mats, int n_mats, int k,
+                       double n_emit, float* out_rgb, float* out_r2,
+                       uint64_t* out_key, int mem, void* stream);
+
+/* ---- adjoint (no reference counterpart) ----------------------------------------
+ * Reverse-mode derivative of pm_gather_radiance for dl_drgb (m*3):
+ *   dl_dflux[c*N + i]  += sum_q s_qi (rho_c/PI) dl_drgb[q,c] / (n_emit PI r_q^2)
+ *   dl_dalbedo[3*m + c] += sum_{q: mat=m} dl_drgb[q,c] sum_i s_qi flux_ic
+ *                                          / (PI n_emit PI r_q^2)
+ * key == NULL: the neighbour sets are recomputed; otherwise the keys from
+ * the forward pass are reused.  dl_dflux is 3*N SoA in original photon order,
+ * dl_dalbedo is n_mats*3; both are accumulated into (caller zeroes them).
+ * fg_mat (may be NULL): per-query index of the primary-hit material of a
+ * final-gathering bounce; when given, the forward value is rho_fg * L, so
+ * the incoming gradient is scaled by rho_fg and dl_dalbedo[fg_mat] receives
+ * dl_drgb * L (include/integrator.h:227-243). */
+int pm_gather_adjoint(pm_photonmap* map, int64_t m, const pm_query* q,
+                      const pm_material* mats, int n_mats, int k,
+                      double n_emit, const float* dl_drgb, const uint64_t* key,
+                      const int32_t* fg_mat, float* dl_dflux,
+                      float* dl_dalbedo, int mem, void* stream);
+
+/* ---- batched persistent gather (no reference counterpart) -----------------------
+ * Many independent gathers (photon map, k, n_emit, query list) in one
+ * persistent launch fed by an atomic work queue.  All pointers are device
+ * memory.  Results equal pm_gather_radiance on each instance bit for bit. */
+typedef struct pm_instance {
+  pm_photonmap* map;
+  int64_t m;
+  const pm_query* q;
+  const pm_material* mats;
+  int32_t n_mats;
+  int32_t k;
+  double n_emit;
+  float* out_rgb;
+  float* out_r2;    /* may be NULL */
+  uint64_t* out_key; /* may be NULL */
+} pm_instance;
+int pm_gather_batched(pm_context* ctx, int n_inst, const pm_instance* inst,
+                      void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PM_GATHER_H */

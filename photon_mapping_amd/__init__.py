@@ -1,0 +1,12 @@
+"""MI355X-native kNN photon gather (gfx950 HIP kernels behind a C ABI).
+
+The compute path is libpm_gather.so; this package holds its Python mirror
+of the reference's photon-map interface, the procedural benchmark scenes and
+the torch autograd wrapper of the gather.
+"""
+
+from ._lib import PmError  # noqa: F401
+from .photon_map import (Context, Photon, PhotonMap, make_materials,  # noqa: F401
+                         make_queries)
+
+__all__ = ["Context", "Photon", "PhotonMap", "PmError", "make_materials", "make_queries"]

@@ -1,0 +1,147 @@
+"""ctypes bindings of libpm_gather.so (the C ABI in include/pm_gather.h).
+
+The library is loaded from the package directory only.  There is no CPU
+fallback: if the HIP library is missing or does not load, every entry point
+raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libpm_gather.so"
+
+PM_OK = 0
+PM_ERR_INVALID_ARG = 1
+PM_ERR_HIP = 2
+PM_ERR_OOM = 3
+PM_ERR_NOT_BUILT = 4
+PM_ERR_UNSUPPORTED = 5
+
+PM_MEM_DEVICE = 0
+PM_MEM_HOST = 1
+
+PM_MATERIAL_LAMBERT = 0
+PM_MATERIAL_MIRROR = 1
+PM_MATERIAL_GLASS = 2
+
+PM_OPT_TIMING = 1
+PM_OPT_WAVES_PER_CU = 2
+PM_OPT_ANCHOR_STRIDE = 3
+PM_OPT_LEAF_SIZE = 4
+PM_OPT_XCD_AWARE = 5
+PM_OPT_COUNT_STATS = 6
+
+
+class PmError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, code: int, func: str, msg: str):
+        super().__init__(f"{func} failed with status {code}: {msg}")
+        self.code = code
+
+
+class pm_query(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("ns", C.c_float * 3), ("wo", C.c_float * 3),
+                ("material", C.c_int32)]
+
+
+class pm_material(C.Structure):
+    _fields_ = [("rho", C.c_float * 3), ("type", C.c_int32)]
+
+
+class pm_stats(C.Structure):
+    _fields_ = [("sort_ms", C.c_double), ("gather_ms", C.c_double), ("post_ms", C.c_double),
+                ("build_ms", C.c_double), ("groups", C.c_int64), ("fallbacks", C.c_int64),
+                ("refinements", C.c_int64)]
+
+
+class pm_instance(C.Structure):
+    _fields_ = [("map", C.c_void_p), ("m", C.c_int64), ("q", C.c_void_p),
+                ("mats", C.c_void_p), ("n_mats", C.c_int32), ("k", C.c_int32),
+                ("n_emit", C.c_double), ("out_rgb", C.c_void_p), ("out_r2", C.c_void_p),
+                ("out_key", C.c_void_p)]
+
+
+assert C.sizeof(pm_query) == 40
+assert C.sizeof(pm_material) == 16
+
+VP = C.c_void_p
+I32 = C.c_int
+I64 = C.c_int64
+DBL = C.c_double
+
+# name -> (restype, argtypes); restype None means "returns status"
+SIGNATURES = {
+    "pm_last_error": (C.c_char_p, []),
+    "pm_version": (I32, []),
+    "pm_context_create": (None, [I32, C.POINTER(VP)]),
+    "pm_context_destroy": (None, [VP]),
+    "pm_context_set_option": (None, [VP, I32, I64]),
+    "pm_context_get_stats": (None, [VP, C.POINTER(pm_stats)]),
+    "pm_context_synchronize": (None, [VP, VP]),
+    "pm_photonmap_create": (None, [VP, I64, VP, VP, VP, I32, C.POINTER(VP)]),
+    "pm_photonmap_create_aos": (None, [VP, I64, VP, I32, C.POINTER(VP)]),
+    "pm_photonmap_destroy": (None, [VP]),
+    "pm_photonmap_build": (None, [VP, I32, VP]),
+    "pm_photonmap_update_flux": (None, [VP, VP, I32, VP]),
+    "pm_photonmap_size": (I64, [VP]),
+    "pm_photonmap_get_photon": (None, [VP, I64, VP]),
+    "pm_knn": (None, [VP, I64, VP, I32, VP, VP, I32, VP]),
+    "pm_gather_radiance": (None, [VP, I64, VP, VP, I32, I32, DBL, VP, VP, VP, I32, VP]),
+    "pm_gather_adjoint": (None, [VP, I64, VP, VP, I32, I32, DBL, VP, VP, VP, VP, VP, I32,
+                                 VP]),
+    "pm_gather_batched": (None, [VP, I32, VP, VP]),
+    "pmx_fg_queries": (None, [VP, VP, I32, VP, I32, I32, I32, I32, C.c_uint64, VP, VP]),
+    "pmx_surrogate_photons": (None, [VP, VP, I32, VP, I32, VP, I64, C.c_uint64, VP, VP,
+                                     VP]),
+}
+
+# Symbols declared in include/*.h (checked by the CPU tests).
+EXPORTED = sorted(SIGNATURES)
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads libpm_gather.so (once).  Raises if it is missing."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `python photon_mapping_amd/build.py`")
+        lib = C.CDLL(str(LIB_PATH), mode=C.RTLD_GLOBAL)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = C.c_int if res is None else res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def call(name: str, *args) -> int:
+    """Calls a status-returning entry point; raises PmError on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != PM_OK:
+        msg = lib.pm_last_error().decode(errors="replace")
+        raise PmError(rc, name, msg)
+    return rc
+
+
+def ptr(x) -> C.c_void_p | None:
+    """Raw pointer of a torch tensor / numpy array / None."""
+    if x is None:
+        return None
+    if hasattr(x, "data_ptr"):
+        return C.c_void_p(x.data_ptr())
+    if hasattr(x, "ctypes"):
+        return C.c_void_p(x.ctypes.data)
+    raise TypeError(f"cannot take a pointer of {type(x)!r}")

@@ -1,0 +1,554 @@
+// Photon-map construction and query preparation kernels.
+//
+// Build (replaces KdTree::buildTree, include/photon_map.h:61-105,157-164):
+//   1. bounding box of the photon positions (and a finiteness check);
+//   2. 30-bit Morton code per photon, 10 bits per axis;
+//   3. stable radix sort of (code, original index);
+//   4. reorder into slot order: pos4 = (x, y, z, index bits),
+//      wif4 = (wi, flux.r), flux2 = (flux.g, flux.b), plus index -> slot;
+//   5. a balanced binary tree over leaf buckets of leaf_size consecutive
+//      slots: leaf boxes, then each level's boxes as unions of its children
+//      (median split by count over the Morton order);
+//   6. radius anchors every anchor_stride slots (centroid + Morton code).
+// The anchors' k-NN radius bounds are computed lazily per k by running the
+// gather kernel itself in anchor mode.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "pm_internal.hpp"
+
+namespace pm {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+  v &= 1023u;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t quantize(float v, float lo, float scale) {
+  const float t = (v - lo) * scale;
+  if (!(t > 0.0f)) return 0u;
+  if (t >= 1023.0f) return 1023u;
+  return static_cast<uint32_t>(t);
+}
+
+__device__ __forceinline__ uint32_t morton3(float x, float y, float z, const float* lo,
+                                            const float* scale) {
+  return (expand10(quantize(x, lo[0], scale[0])) << 2) |
+         (expand10(quantize(y, lo[1], scale[1])) << 1) |
+         expand10(quantize(z, lo[2], scale[2]));
+}
+
+// Per-block min/max of positions plus a count of non-finite coordinates.
+__global__ __launch_bounds__(kThreads) void bbox_partial_kernel(
+    const float* __restrict__ pos, int64_t n, float* __restrict__ partial,
+    uint32_t* __restrict__ bad) {
+  __shared__ float smin[3][kThreads];
+  __shared__ float smax[3][kThreads];
+  const int t = threadIdx.x;
+  float mn[3] = {INFINITY, INFINITY, INFINITY};
+  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  uint32_t nbad = 0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + t; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kThreads) {
+    for (int c = 0; c < 3; ++c) {
+      const float v = pos[c * n + i];
+      if (!isfinite(v)) { ++nbad; continue; }
+      mn[c] = fminf(mn[c], v);
+      mx[c] = fmaxf(mx[c], v);
+    }
+  }
+  if (nbad) atomicAdd(bad, nbad);
+  for (int c = 0; c < 3; ++c) { smin[c][t] = mn[c]; smax[c][t] = mx[c]; }
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      for (int c = 0; c < 3; ++c) {
+        smin[c][t] = fminf(smin[c][t], smin[c][t + o]);
+        smax[c][t] = fmaxf(smax[c][t], smax[c][t + o]);
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int c = 0; c < 3; ++c) {
+      partial[blockIdx.x * 6 + c] = smin[c][0];
+      partial[blockIdx.x * 6 + 3 + c] = smax[c][0];
+    }
+  }
+}
+
+__global__ void bbox_final_kernel(const float* __restrict__ partial, int nb,
+                                  float* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  float r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int b = 0; b < nb; ++b) {
+    for (int c = 0; c < 3; ++c) {
+      r[c] = fminf(r[c], partial[b * 6 + c]);
+      r[3 + c] = fmaxf(r[3 + c], partial[b * 6 + 3 + c]);
+    }
+  }
+  for (int c = 0; c < 6; ++c) out[c] = r[c];
+}
+
+struct Quant {
+  float lo[3];
+  float scale[3];
+};
+
+__global__ __launch_bounds__(kThreads) void photon_codes_kernel(
+    const float* __restrict__ pos, int64_t n, Quant qz, uint32_t* __restrict__ codes,
+    uint32_t* __restrict__ vals) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  codes[i] = morton3(pos[i], pos[n + i], pos[2 * n + i], qz.lo, qz.scale);
+  vals[i] = static_cast<uint32_t>(i);
+}
+
+__global__ __launch_bounds__(kThreads) void reorder_kernel(
+    const float* __restrict__ pos, const float* __restrict__ wi,
+    const float* __restrict__ flux, int64_t n, const uint32_t* __restrict__ order,
+    float4* __restrict__ pos4, float4* __restrict__ wif4, float2* __restrict__ flux2,
+    int32_t* __restrict__ inv_perm) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t i = order[s];
+  pos4[s] = make_float4(pos[i], pos[n + i], pos[2 * n + i], __uint_as_float(i));
+  wif4[s] = make_float4(wi[i], wi[n + i], wi[2 * n + i], flux[i]);
+  flux2[s] = make_float2(flux[n + i], flux[2 * n + i]);
+  inv_perm[i] = static_cast<int32_t>(s);
+}
+
+__global__ __launch_bounds__(kThreads) void update_flux_kernel(
+    const float* __restrict__ flux, int64_t n, const int32_t* __restrict__ inv_perm,
+    float4* __restrict__ wif4, float2* __restrict__ flux2) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int32_t s = inv_perm[i];
+  wif4[s].w = flux[i];
+  flux2[s] = make_float2(flux[n + i], flux[2 * n + i]);
+}
+
+__global__ __launch_bounds__(kThreads) void leaf_box_kernel(
+    const float4* __restrict__ pos4, int64_t n, int leaf_size, int n_leaves,
+    int first_leaf, float4* __restrict__ boxes) {
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= first_leaf) return;
+  float4 mn = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+  float4 mx = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+  if (j < n_leaves) {
+    const int64_t s0 = static_cast<int64_t>(j) * leaf_size;
+    const int64_t s1 = min(s0 + leaf_size, n);
+    for (int64_t s = s0; s < s1; ++s) {
+      const float4 p = pos4[s];
+      mn.x = fminf(mn.x, p.x); mn.y = fminf(mn.y, p.y); mn.z = fminf(mn.z, p.z);
+      mx.x = fmaxf(mx.x, p.x); mx.y = fmaxf(mx.y, p.y); mx.z = fmaxf(mx.z, p.z);
+    }
+  }
+  boxes[2 * (first_leaf + j)] = mn;
+  boxes[2 * (first_leaf + j) + 1] = mx;
+}
+
+__global__ __launch_bounds__(kThreads) void level_box_kernel(int level_start,
+                                                             float4* __restrict__ boxes) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= level_start) return;
+  const int node = level_start + i;
+  const float4 a0 = boxes[2 * (2 * node)], a1 = boxes[2 * (2 * node) + 1];
+  const float4 b0 = boxes[2 * (2 * node + 1)], b1 = boxes[2 * (2 * node + 1) + 1];
+  boxes[2 * node] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.f);
+  boxes[2 * node + 1] =
+      make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.f);
+}
+
+__global__ __launch_bounds__(kThreads) void anchor_init_kernel(
+    const float4* __restrict__ pos4, const uint32_t* __restrict__ codes, int64_t n,
+    int stride, int n_anchors, float4* __restrict__ anchors,
+    uint32_t* __restrict__ anchor_code) {
+  const int a = blockIdx.x * kThreads + threadIdx.x;
+  if (a >= n_anchors) return;
+  const int64_t s0 = static_cast<int64_t>(a) * stride;
+  const int64_t s1 = min(s0 + stride, n);
+  float cx = 0.f, cy = 0.f, cz = 0.f;
+  for (int64_t s = s0; s < s1; ++s) {
+    const float4 p = pos4[s];
+    cx += p.x; cy += p.y; cz += p.z;
+  }
+  const float inv = 1.0f / static_cast<float>(s1 - s0);
+  anchors[a] = make_float4(cx * inv, cy * inv, cz * inv, INFINITY);
+  anchor_code[a] = codes[s0];
+}
+
+// ---- query preparation ---------------------------------------------------------
+
+__device__ __forceinline__ int find_inst(const int64_t* __restrict__ prefix, int n_inst,
+                                         int64_t i) {
+  int lo = 0, hi = n_inst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void query_keys_kernel(
+    const GatherDesc* __restrict__ descs, const int64_t* __restrict__ q_prefix,
+    int n_inst, int64_t total, K* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= total) return;
+  const int inst = n_inst > 1 ? find_inst(q_prefix, n_inst, i) : 0;
+  const GatherDesc& D = descs[inst];
+  const int64_t li = i - q_prefix[inst];
+  float x, y, z;
+  if (D.q) {
+    const pm_query& q = D.q[li];
+    x = q.pos[0]; y = q.pos[1]; z = q.pos[2];
+  } else {
+    x = D.q_soa[li]; y = D.q_soa[D.m + li]; z = D.q_soa[2 * D.m + li];
+  }
+  const uint32_t code = morton3(x, y, z, D.qlo, D.qscale);
+  keys[i] = static_cast<K>((static_cast<uint64_t>(inst) << 32) | code);
+  vals[i] = static_cast<uint32_t>(i);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void group_anchor_kernel(
+    const GatherDesc* __restrict__ descs, const int64_t* __restrict__ q_prefix,
+    const int64_t* __restrict__ g_prefix, int n_inst, int64_t total_groups,
+    const K* __restrict__ sorted_keys, uint32_t* __restrict__ group_anchor) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (g >= total_groups) return;
+  const int inst = n_inst > 1 ? find_inst(g_prefix, n_inst, g) : 0;
+  const GatherDesc& D = descs[inst];
+  const int64_t lg = g - g_prefix[inst];
+  const int64_t mid = min(lg * kWave + kWave / 2, D.m - 1);
+  const uint32_t code = static_cast<uint32_t>(sorted_keys[q_prefix[inst] + mid] & 0xffffffffu);
+  // upper bound over the anchor codes, then one step back
+  int lo = 0, hi = D.n_anchors;
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (D.anchor_code[m] <= code) lo = m + 1;
+    else hi = m;
+  }
+  group_anchor[g] = static_cast<uint32_t>(lo > 0 ? lo - 1 : 0);
+}
+
+// ---- kNN row sort: slots -> original indices, farthest first ------------------------
+__global__ __launch_bounds__(kWave) void knn_rowsort_kernel(const GatherDesc* __restrict__ dp,
+                                                            int32_t* __restrict__ out_idx) {
+  __shared__ uint64_t key[kMaxKnnK];
+  __shared__ int32_t slot[kMaxKnnK];
+  const GatherDesc& D = *dp;
+  const int64_t q = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int k = D.k;
+  int p2 = 64;
+  while (p2 < k) p2 <<= 1;
+  const float qx = D.q_soa[q], qy = D.q_soa[D.m + q], qz = D.q_soa[2 * D.m + q];
+  int32_t* row = out_idx + q * k;
+  for (int j = lane; j < p2; j += kWave) {
+    uint64_t kk = 0;  // padding sorts last (descending)
+    int32_t sl = -1;
+    if (j < k) {
+      sl = row[j];
+      if (sl >= 0) {
+        const float4 p = D.pos4[sl];
+        const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+        float acc = dx * dx;
+        acc = acc + dy * dy;
+        acc = acc + dz * dz;
+        kk = ((static_cast<uint64_t>(__float_as_uint(acc)) << 32) | __float_as_uint(p.w)) + 1ull;
+      }
+    }
+    key[j] = kk;
+    slot[j] = sl;
+  }
+  __syncthreads();
+  // bitonic sort, descending
+  for (int size = 2; size <= p2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = lane; t < p2 / 2; t += kWave) {
+        const int i = 2 * t - (t & (stride - 1));
+        const int j = i + stride;
+        const bool desc = ((i & size) == 0);
+        const uint64_t a = key[i], b = key[j];
+        if ((a < b) == desc) {
+          key[i] = b; key[j] = a;
+          const int32_t s = slot[i]; slot[i] = slot[j]; slot[j] = s;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int j = lane; j < k; j += kWave) {
+    const int32_t sl = slot[j];
+    row[j] = (key[j] != 0 && sl >= 0) ? static_cast<int32_t>(__float_as_uint(D.pos4[sl].w)) : -1;
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void fill_f32_kernel(float* p, float v, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
+  if (n <= 0) return PM_OK;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, p,
+                     v, n);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+int launch_fill_f32(float* p, float v, int64_t n, hipStream_t s) {
+  if (n <= 0) return PM_OK;
+  hipLaunchKernelGGL(fill_f32_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, p,
+                     v, n);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+void fill_desc_map_part(const PhotonMapImpl* map, GatherDesc* d) {
+  d->pos4 = map->pos4.as<float4>();
+  d->wif4 = map->wif4.as<float4>();
+  d->flux2 = map->flux2.as<float2>();
+  d->boxes = map->boxes.as<float4>();
+  d->anchors = map->anchors.as<float4>();
+  d->anchor_code = map->anchor_code.as<uint32_t>();
+  d->inv_perm = map->inv_perm.as<int32_t>();
+  for (int c = 0; c < 3; ++c) {
+    d->qlo[c] = map->qlo[c];
+    d->qscale[c] = map->qscale[c];
+  }
+  d->n = static_cast<int32_t>(map->n);
+  d->leaf_size = map->leaf_size;
+  d->n_leaves = map->n_leaves;
+  d->first_leaf = map->first_leaf;
+  d->n_anchors = map->n_anchors;
+  d->anchor_stride = map->anchor_stride;
+}
+
+int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
+  Context* ctx = map->ctx;
+  const int64_t n = map->n;
+  map->built = false;
+  map->anchor_k = -1;
+  if (n == 0) {
+    map->n_leaves = 0;
+    map->first_leaf = 1;
+    map->n_anchors = 0;
+    map->built = true;
+    return PM_OK;
+  }
+  const float* pos = map->pos_soa.as<float>();
+
+  // 1. bounding box + finiteness
+  const int nb = static_cast<int>(std::min<int64_t>(ceil_div(n, kThreads), 1024));
+  if (int rc = ctx->ws_tmp.reserve(static_cast<size_t>(nb) * 6 * sizeof(float) + 64)) return rc;
+  float* partial = ctx->ws_tmp.as<float>();
+  float* box_out = partial + nb * 6;  // 6 floats
+  uint32_t* bad = reinterpret_cast<uint32_t*>(box_out + 6);
+  if (int rc = launch_fill_u32(bad, 0u, 1, s)) return rc;
+  hipLaunchKernelGGL(bbox_partial_kernel, dim3(nb), dim3(kThreads), 0, s, pos, n, partial,
+                     bad);
+  hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(64), 0, s, partial, nb, box_out);
+  PM_HIP(hipGetLastError());
+  float hb[7];
+  PM_HIP(hipMemcpyAsync(hb, box_out, 7 * sizeof(float), hipMemcpyDeviceToHost, s));
+  PM_HIP(hipStreamSynchronize(s));
+  uint32_t nbad;
+  std::memcpy(&nbad, &hb[6], sizeof(nbad));
+  if (nbad != 0) return fail(PM_ERR_INVALID_ARG, "photon positions must be finite");
+  for (int c = 0; c < 3; ++c) {
+    const float ext = hb[3 + c] - hb[c];
+    map->qlo[c] = hb[c];
+    map->qscale[c] = ext > 0.0f ? 1024.0f / ext : 1.0f;
+  }
+
+  // 2-3. Morton codes, stable sort
+  if (int rc = ctx->ws_keys.reserve(n * sizeof(uint32_t))) return rc;
+  if (int rc = ctx->ws_vals.reserve(n * sizeof(uint32_t))) return rc;
+  if (int rc = ctx->ws_keys2.reserve(n * sizeof(uint32_t))) return rc;
+  if (int rc = ctx->ws_vals2.reserve(n * sizeof(uint32_t))) return rc;
+  Quant qz;
+  for (int c = 0; c < 3; ++c) { qz.lo[c] = map->qlo[c]; qz.scale[c] = map->qscale[c]; }
+  uint32_t* k0 = ctx->ws_keys.as<uint32_t>();
+  uint32_t* v0 = ctx->ws_vals.as<uint32_t>();
+  uint32_t* k1 = ctx->ws_keys2.as<uint32_t>();
+  uint32_t* v1 = ctx->ws_vals2.as<uint32_t>();
+  hipLaunchKernelGGL(photon_codes_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s,
+                     pos, n, qz, k0, v0);
+  PM_HIP(hipGetLastError());
+  int which = 0;
+  if (int rc = radix_sort_u32(ctx, k0, v0, k1, v1, n, 3 * kMortonBits, s, &which)) return rc;
+  const uint32_t* sorted_codes = which ? k1 : k0;
+  const uint32_t* order = which ? v1 : v0;
+
+  // 4. reorder into slot order
+  if (int rc = map->pos4.reserve(n * sizeof(float4))) return rc;
+  if (int rc = map->wif4.reserve(n * sizeof(float4))) return rc;
+  if (int rc = map->flux2.reserve(n * sizeof(float2))) return rc;
+  if (int rc = map->inv_perm.reserve(n * sizeof(int32_t))) return rc;
+  if (int rc = map->codes.reserve(n * sizeof(uint32_t))) return rc;
+  hipLaunchKernelGGL(reorder_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, pos,
+                     map->wi_soa.as<float>(), map->flux_soa.as<float>(), n, order,
+                     map->pos4.as<float4>(), map->wif4.as<float4>(),
+                     map->flux2.as<float2>(), map->inv_perm.as<int32_t>());
+  PM_HIP(hipMemcpyAsync(map->codes.ptr, sorted_codes, n * sizeof(uint32_t),
+                        hipMemcpyDeviceToDevice, s));
+
+  // 5. tree boxes
+  map->n_leaves = static_cast<int>(ceil_div(n, map->leaf_size));
+  int fl = 1;
+  while (fl < map->n_leaves) fl <<= 1;
+  map->first_leaf = fl;
+  if (int rc = map->boxes.reserve(static_cast<size_t>(2 * fl) * 2 * sizeof(float4))) return rc;
+  float4* boxes = map->boxes.as<float4>();
+  hipLaunchKernelGGL(leaf_box_kernel, dim3(ceil_div(fl, kThreads)), dim3(kThreads), 0, s,
+                     map->pos4.as<float4>(), n, map->leaf_size, map->n_leaves, fl, boxes);
+  for (int level = fl >> 1; level >= 1; level >>= 1) {
+    hipLaunchKernelGGL(level_box_kernel, dim3(ceil_div(level, kThreads)), dim3(kThreads), 0,
+                       s, level, boxes);
+  }
+  PM_HIP(hipGetLastError());
+
+  // 6. anchors
+  map->anchor_stride = ctx->anchor_stride;
+  map->n_anchors = static_cast<int>(ceil_div(n, map->anchor_stride));
+  if (int rc = map->anchors.reserve(static_cast<size_t>(map->n_anchors) * sizeof(float4)))
+    return rc;
+  if (int rc = map->anchor_code.reserve(static_cast<size_t>(map->n_anchors) * sizeof(uint32_t)))
+    return rc;
+  hipLaunchKernelGGL(anchor_init_kernel, dim3(ceil_div(map->n_anchors, kThreads)),
+                     dim3(kThreads), 0, s, map->pos4.as<float4>(),
+                     map->codes.as<uint32_t>(), n, map->anchor_stride, map->n_anchors,
+                     map->anchors.as<float4>(), map->anchor_code.as<uint32_t>());
+  PM_HIP(hipGetLastError());
+  map->built = true;
+  return PM_OK;
+}
+
+int update_flux(PhotonMapImpl* map, const float* flux_dev, hipStream_t s) {
+  if (map->n == 0 || !map->built) return PM_OK;
+  hipLaunchKernelGGL(update_flux_kernel, dim3(ceil_div(map->n, kThreads)), dim3(kThreads), 0,
+                     s, flux_dev, map->n, map->inv_perm.as<int32_t>(),
+                     map->wif4.as<float4>(), map->flux2.as<float2>());
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s) {
+  if (map->n == 0 || k_eff >= map->n) return PM_OK;  // every photon is a neighbour
+  if (map->anchor_k == k_eff) return PM_OK;
+  Context* ctx = map->ctx;
+  GatherDesc d{};
+  fill_desc_map_part(map, &d);
+  d.m = map->n_anchors;
+  d.q_base = 0;
+  d.group_base = 0;
+  d.n_groups = ceil_div(map->n_anchors, kWave);
+  d.k = k_eff;
+  d.k_eff = k_eff;
+  d.all_members = 0;
+  d.anchor_out = reinterpret_cast<float*>(map->anchors.ptr);
+  if (int rc = ctx->ws_counters.reserve(64 * sizeof(uint32_t))) return rc;
+  int64_t gp[2] = {0, d.n_groups};
+  if (int rc = stage_begin(ctx, sizeof(GatherDesc) + sizeof(gp) + 256, s)) return rc;
+  GatherLaunch L{};
+  L.descs = static_cast<const GatherDesc*>(stage_put(ctx, &d, sizeof(d)));
+  L.group_prefix = static_cast<const int64_t*>(stage_put(ctx, gp, sizeof(gp)));
+  if (int rc = stage_commit(ctx, s)) return rc;
+  L.perm = nullptr;
+  L.q_prefix = nullptr;
+  L.counters = ctx->ws_counters.as<uint32_t>();
+  L.total_groups = d.n_groups;
+  L.n_inst = 1;
+  L.xcd_aware = 0;
+  L.count_stats = 0;
+  L.mode = kModeAnchor;
+  if (int rc = launch_gather(ctx, L, s)) return rc;
+  map->anchor_k = k_eff;
+  return PM_OK;
+}
+
+int prepare_queries(Context* ctx, int n_inst, const GatherDesc* dev_descs,
+                    const int64_t* dev_q_prefix, const int64_t* dev_g_prefix,
+                    int64_t total_q, int64_t total_groups, hipStream_t s,
+                    const uint32_t** perm_out) {
+  if (total_q == 0) {
+    *perm_out = nullptr;
+    return PM_OK;
+  }
+  const bool wide = n_inst > 1;
+  const size_t kbytes = wide ? sizeof(uint64_t) : sizeof(uint32_t);
+  if (int rc = ctx->ws_keys.reserve(total_q * kbytes)) return rc;
+  if (int rc = ctx->ws_keys2.reserve(total_q * kbytes)) return rc;
+  if (int rc = ctx->ws_vals.reserve(total_q * sizeof(uint32_t))) return rc;
+  if (int rc = ctx->ws_vals2.reserve(total_q * sizeof(uint32_t))) return rc;
+  if (int rc = ctx->ws_group_anchor.reserve(std::max<int64_t>(total_groups, 1) * sizeof(uint32_t)))
+    return rc;
+  uint32_t* v0 = ctx->ws_vals.as<uint32_t>();
+  uint32_t* v1 = ctx->ws_vals2.as<uint32_t>();
+  int which = 0;
+  const dim3 grid(ceil_div(total_q, kThreads));
+  const dim3 ggrid(ceil_div(std::max<int64_t>(total_groups, 1), kThreads));
+  if (!wide) {
+    uint32_t* k0 = ctx->ws_keys.as<uint32_t>();
+    uint32_t* k1 = ctx->ws_keys2.as<uint32_t>();
+    hipLaunchKernelGGL(query_keys_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
+                       dev_q_prefix, n_inst, total_q, k0, v0);
+    PM_HIP(hipGetLastError());
+    if (int rc = radix_sort_u32(ctx, k0, v0, k1, v1, total_q, 3 * kMortonBits, s, &which))
+      return rc;
+    hipLaunchKernelGGL(group_anchor_kernel<uint32_t>, ggrid, dim3(kThreads), 0, s, dev_descs,
+                       dev_q_prefix, dev_g_prefix, n_inst, total_groups, which ? k1 : k0,
+                       ctx->ws_group_anchor.as<uint32_t>());
+  } else {
+    uint64_t* k0 = ctx->ws_keys.as<uint64_t>();
+    uint64_t* k1 = ctx->ws_keys2.as<uint64_t>();
+    int ib = 1;
+    while ((1 << ib) < n_inst) ++ib;
+    hipLaunchKernelGGL(query_keys_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
+                       dev_q_prefix, n_inst, total_q, k0, v0);
+    PM_HIP(hipGetLastError());
+    if (int rc = radix_sort_u64(ctx, k0, v0, k1, v1, total_q, 32 + ib, s, &which)) return rc;
+    hipLaunchKernelGGL(group_anchor_kernel<uint64_t>, ggrid, dim3(kThreads), 0, s, dev_descs,
+                       dev_q_prefix, dev_g_prefix, n_inst, total_groups, which ? k1 : k0,
+                       ctx->ws_group_anchor.as<uint32_t>());
+  }
+  PM_HIP(hipGetLastError());
+  *perm_out = which ? v1 : v0;
+  return PM_OK;
+}
+
+int launch_knn_rowsort(const GatherDesc* dev_desc, int64_t m, int32_t* out_idx,
+                       hipStream_t s) {
+  if (m <= 0) return PM_OK;
+  hipLaunchKernelGGL(knn_rowsort_kernel, dim3(m), dim3(kWave), 0, s, dev_desc, out_idx);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+}  // namespace pm

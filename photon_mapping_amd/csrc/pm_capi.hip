@@ -1,0 +1,633 @@
+// C ABI of libpm_gather (declared in include/pm_gather.h).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pm_internal.hpp"
+
+namespace pm {
+
+namespace {
+thread_local std::string g_last_error;
+constexpr int kVersion = 1;
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
+  return e == hipErrorOutOfMemory ? PM_ERR_OOM : PM_ERR_HIP;
+}
+
+DevBuf::~DevBuf() { release(); }
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+int DevBuf::reserve(size_t need) {
+  if (need == 0) need = 16;
+  if (need <= bytes) return PM_OK;
+  release();
+  void* p = nullptr;
+  const hipError_t e = hipMalloc(&p, need);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PM_ERR_OOM, "hipMalloc of " + std::to_string(need) + " bytes failed: " +
+                                hipGetErrorString(e));
+  }
+  ptr = p;
+  bytes = need;
+  return PM_OK;
+}
+
+int stage_begin(Context* ctx, size_t max_bytes, hipStream_t s) {
+  // The previous upload must have left the pinned arena before it is reused.
+  if (ctx->stage_done) PM_HIP(hipEventSynchronize(ctx->stage_done));
+  if (max_bytes > ctx->stage_cap) {
+    // Growing also replaces the device arena: wait for users of the old one.
+    PM_HIP(hipStreamSynchronize(s));
+    if (ctx->stage_host) (void)hipHostFree(ctx->stage_host);
+    ctx->stage_host = nullptr;
+    size_t cap = std::max<size_t>(max_bytes, 64 * 1024);
+    PM_HIP(hipHostMalloc(&ctx->stage_host, cap, hipHostMallocDefault));
+    ctx->stage_dev.release();
+    if (int rc = ctx->stage_dev.reserve(cap)) return rc;
+    ctx->stage_cap = cap;
+  }
+  ctx->stage_off = 0;
+  return PM_OK;
+}
+
+void* stage_put(Context* ctx, const void* src, size_t bytes) {
+  const size_t off = (ctx->stage_off + 255) & ~static_cast<size_t>(255);
+  std::memcpy(static_cast<char*>(ctx->stage_host) + off, src, bytes);
+  ctx->stage_off = off + bytes;
+  return static_cast<char*>(ctx->stage_dev.ptr) + off;
+}
+
+int stage_commit(Context* ctx, hipStream_t s) {
+  if (ctx->stage_off == 0) return PM_OK;
+  PM_HIP(hipMemcpyAsync(ctx->stage_dev.ptr, ctx->stage_host, ctx->stage_off,
+                        hipMemcpyHostToDevice, s));
+  PM_HIP(hipEventRecord(ctx->stage_done, s));
+  return PM_OK;
+}
+
+namespace {
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+int set_device(Context* ctx) {
+  PM_HIP(hipSetDevice(ctx->device));
+  return PM_OK;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+int copy_in(void* dst, const void* src, size_t bytes, int mem, hipStream_t s) {
+  if (bytes == 0) return PM_OK;
+  PM_HIP(hipMemcpyAsync(dst, src, bytes,
+                        mem == PM_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                        s));
+  return PM_OK;
+}
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1.0;
+  }
+  return static_cast<double>(ms);
+}
+
+struct CallTimer {
+  Context* ctx;
+  hipStream_t s;
+  bool on;
+  CallTimer(Context* c, hipStream_t st) : ctx(c), s(st), on(c->timing != 0) {}
+  void mark(int i) {
+    if (on) (void)hipEventRecord(ctx->ev[i], s);
+  }
+};
+
+// Shared driver of pm_knn / pm_gather_radiance / pm_gather_adjoint on one map.
+int run_single(PhotonMapImpl* map, GatherDesc d, int mode, hipStream_t s,
+               int32_t* knn_out_idx) {
+  Context* ctx = map->ctx;
+  const int64_t m = d.m;
+  d.q_base = 0;
+  d.group_base = 0;
+  d.n_groups = ceil_div(m, kWave);
+  if (int rc = ctx->ws_counters.reserve(64 * sizeof(uint32_t))) return rc;
+  if (int rc = ctx->ws_group_anchor.reserve(std::max<int64_t>(d.n_groups, 1) * sizeof(uint32_t)))
+    return rc;
+  d.group_anchor = ctx->ws_group_anchor.as<uint32_t>();
+  const int64_t qp[2] = {0, m};
+  const int64_t gp[2] = {0, d.n_groups};
+  if (int rc = stage_begin(ctx, sizeof(GatherDesc) + sizeof(qp) + sizeof(gp) + 1024, s))
+    return rc;
+  const GatherDesc* dd = static_cast<const GatherDesc*>(stage_put(ctx, &d, sizeof(d)));
+  const int64_t* dqp = static_cast<const int64_t*>(stage_put(ctx, qp, sizeof(qp)));
+  const int64_t* dgp = static_cast<const int64_t*>(stage_put(ctx, gp, sizeof(gp)));
+  if (int rc = stage_commit(ctx, s)) return rc;
+
+  CallTimer tm(ctx, s);
+  tm.mark(0);
+  const uint32_t* perm = nullptr;
+  if (int rc = prepare_queries(ctx, 1, dd, dqp, dgp, m, d.n_groups, s, &perm)) return rc;
+  tm.mark(1);
+  GatherLaunch L{};
+  L.descs = dd;
+  L.group_prefix = dgp;
+  L.perm = perm;
+  L.q_prefix = dqp;
+  L.counters = ctx->ws_counters.as<uint32_t>();
+  L.total_groups = d.n_groups;
+  L.n_inst = 1;
+  L.xcd_aware = ctx->xcd_aware;
+  L.count_stats = ctx->count_stats;
+  L.mode = mode;
+  if (int rc = launch_gather(ctx, L, s)) return rc;
+  tm.mark(2);
+  if (mode == kModeKnn) {
+    if (int rc = launch_knn_rowsort(dd, m, knn_out_idx, s)) return rc;
+  }
+  tm.mark(3);
+  ctx->stats.groups = d.n_groups;
+  if (tm.on || ctx->count_stats) {
+    PM_HIP(hipStreamSynchronize(s));
+    if (tm.on) {
+      ctx->stats.sort_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
+      ctx->stats.gather_ms = elapsed_ms(ctx->ev[1], ctx->ev[2]);
+      ctx->stats.post_ms = elapsed_ms(ctx->ev[2], ctx->ev[3]);
+    }
+    if (ctx->count_stats) {
+      uint32_t c[2] = {0, 0};
+      PM_HIP(hipMemcpy(c, ctx->ws_counters.as<uint32_t>() + 2 * kXcds, sizeof(c),
+                       hipMemcpyDeviceToHost));
+      ctx->stats.fallbacks = c[0];
+      ctx->stats.refinements = c[1];
+    }
+  }
+  return PM_OK;
+}
+
+int check_map(pm_photonmap* h) {
+  if (!h) return fail(PM_ERR_INVALID_ARG, "null photon map");
+  auto* map = reinterpret_cast<PhotonMapImpl*>(h);
+  if (!map->built) return fail(PM_ERR_NOT_BUILT, "photon map has not been built");
+  return set_device(map->ctx);
+}
+
+}  // namespace
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+const char* pm_last_error(void) { return g_last_error.c_str(); }
+
+int pm_version(void) { return kVersion; }
+
+int pm_context_create(int device, pm_context** out_ctx) {
+  if (!out_ctx) return fail(PM_ERR_INVALID_ARG, "null out_ctx");
+  *out_ctx = nullptr;
+  int count = 0;
+  PM_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count)
+    return fail(PM_ERR_INVALID_ARG, "device ordinal out of range");
+  PM_HIP(hipSetDevice(device));
+  auto* ctx = new (std::nothrow) Context();
+  if (!ctx) return fail(PM_ERR_OOM, "host allocation failed");
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->n_cu = prop.multiProcessorCount;
+  for (auto& e : ctx->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete ctx;
+      return fail(PM_ERR_HIP, "hipEventCreate failed");
+    }
+  }
+  if (hipEventCreateWithFlags(&ctx->stage_done, hipEventDisableTiming) != hipSuccess) {
+    delete ctx;
+    return fail(PM_ERR_HIP, "hipEventCreate failed");
+  }
+  *out_ctx = reinterpret_cast<pm_context*>(ctx);
+  return PM_OK;
+}
+
+int pm_context_destroy(pm_context* h) {
+  if (!h) return PM_OK;
+  auto* ctx = reinterpret_cast<Context*>(h);
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stage_done) (void)hipEventDestroy(ctx->stage_done);
+  if (ctx->stage_host) (void)hipHostFree(ctx->stage_host);
+  delete ctx;
+  return PM_OK;
+}
+
+int pm_context_set_option(pm_context* h, int option, int64_t value) {
+  if (!h) return fail(PM_ERR_INVALID_ARG, "null context");
+  auto* ctx = reinterpret_cast<Context*>(h);
+  switch (option) {
+    case PM_OPT_TIMING: ctx->timing = value != 0; break;
+    case PM_OPT_WAVES_PER_CU:
+      if (value < 1 || value > 32) return fail(PM_ERR_INVALID_ARG, "waves per CU in 1..32");
+      ctx->waves_per_cu = static_cast<int>(value);
+      break;
+    case PM_OPT_ANCHOR_STRIDE:
+      if (value < 1 || value > 4096) return fail(PM_ERR_INVALID_ARG, "anchor stride in 1..4096");
+      ctx->anchor_stride = static_cast<int>(value);
+      break;
+    case PM_OPT_LEAF_SIZE:
+      if (value < 4 || value > 256) return fail(PM_ERR_INVALID_ARG, "leaf size in 4..256");
+      ctx->leaf_size = static_cast<int>(value);
+      break;
+    case PM_OPT_XCD_AWARE: ctx->xcd_aware = value != 0; break;
+    case PM_OPT_COUNT_STATS: ctx->count_stats = value != 0; break;
+    default: return fail(PM_ERR_INVALID_ARG, "unknown option");
+  }
+  return PM_OK;
+}
+
+int pm_context_get_stats(pm_context* h, pm_stats* out) {
+  if (!h || !out) return fail(PM_ERR_INVALID_ARG, "null argument");
+  *out = reinterpret_cast<Context*>(h)->stats;
+  return PM_OK;
+}
+
+int pm_context_synchronize(pm_context* h, void* stream) {
+  if (!h) return fail(PM_ERR_INVALID_ARG, "null context");
+  auto* ctx = reinterpret_cast<Context*>(h);
+  if (int rc = set_device(ctx)) return rc;
+  PM_HIP(hipStreamSynchronize(as_stream(stream)));
+  return PM_OK;
+}
+
+int pm_photonmap_create(pm_context* h, int64_t n, const float* pos_soa, const float* wi_soa,
+                        const float* flux_soa, int mem, pm_photonmap** out_map) {
+  if (!h || !out_map) return fail(PM_ERR_INVALID_ARG, "null argument");
+  *out_map = nullptr;
+  if (n < 0 || n > (int64_t{1} << 31) - 1)
+    return fail(PM_ERR_INVALID_ARG, "photon count must be in [0, 2^31)");
+  if (n > 0 && (!pos_soa || !wi_soa || !flux_soa))
+    return fail(PM_ERR_INVALID_ARG, "null photon arrays");
+  if (mem != PM_MEM_HOST && mem != PM_MEM_DEVICE) return fail(PM_ERR_INVALID_ARG, "bad mem");
+  auto* ctx = reinterpret_cast<Context*>(h);
+  if (int rc = set_device(ctx)) return rc;
+  if (mem == PM_MEM_HOST) {
+    for (int64_t i = 0; i < 3 * n; ++i)
+      if (!std::isfinite(pos_soa[i]))
+        return fail(PM_ERR_INVALID_ARG, "photon positions must be finite");
+  }
+  auto* map = new (std::nothrow) PhotonMapImpl();
+  if (!map) return fail(PM_ERR_OOM, "host allocation failed");
+  map->ctx = ctx;
+  map->n = n;
+  map->leaf_size = ctx->leaf_size;
+  const size_t bytes = static_cast<size_t>(3 * n) * sizeof(float);
+  int rc = map->pos_soa.reserve(bytes);
+  if (!rc) rc = map->wi_soa.reserve(bytes);
+  if (!rc) rc = map->flux_soa.reserve(bytes);
+  if (!rc) rc = copy_in(map->pos_soa.ptr, pos_soa, bytes, mem, nullptr);
+  if (!rc) rc = copy_in(map->wi_soa.ptr, wi_soa, bytes, mem, nullptr);
+  if (!rc) rc = copy_in(map->flux_soa.ptr, flux_soa, bytes, mem, nullptr);
+  if (!rc) {
+    const hipError_t e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
+  }
+  if (rc) {
+    delete map;
+    return rc;
+  }
+  *out_map = reinterpret_cast<pm_photonmap*>(map);
+  return PM_OK;
+}
+
+int pm_photonmap_create_aos(pm_context* h, int64_t n, const float* photons, int mem,
+                            pm_photonmap** out_map) {
+  if (!h || !out_map) return fail(PM_ERR_INVALID_ARG, "null argument");
+  if (n < 0) return fail(PM_ERR_INVALID_ARG, "negative photon count");
+  if (n > 0 && !photons) return fail(PM_ERR_INVALID_ARG, "null photons");
+  std::vector<float> aos;
+  const float* src = photons;
+  if (mem == PM_MEM_DEVICE && n > 0) {
+    aos.resize(static_cast<size_t>(9 * n));
+    PM_HIP(hipMemcpy(aos.data(), photons, aos.size() * sizeof(float), hipMemcpyDeviceToHost));
+    src = aos.data();
+  } else if (mem != PM_MEM_HOST && mem != PM_MEM_DEVICE) {
+    return fail(PM_ERR_INVALID_ARG, "bad mem");
+  }
+  std::vector<float> pos(3 * n), wi(3 * n), flux(3 * n);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int c = 0; c < 3; ++c) {
+      flux[c * n + i] = src[9 * i + c];
+      pos[c * n + i] = src[9 * i + 3 + c];
+      wi[c * n + i] = src[9 * i + 6 + c];
+    }
+  }
+  return pm_photonmap_create(h, n, pos.data(), wi.data(), flux.data(), PM_MEM_HOST, out_map);
+}
+
+int pm_photonmap_destroy(pm_photonmap* h) {
+  if (!h) return PM_OK;
+  auto* map = reinterpret_cast<PhotonMapImpl*>(h);
+  (void)hipSetDevice(map->ctx->device);
+  (void)hipDeviceSynchronize();
+  delete map;
+  return PM_OK;
+}
+
+int pm_photonmap_build(pm_photonmap* h, int leaf_size, void* stream) {
+  if (!h) return fail(PM_ERR_INVALID_ARG, "null photon map");
+  auto* map = reinterpret_cast<PhotonMapImpl*>(h);
+  Context* ctx = map->ctx;
+  if (int rc = set_device(ctx)) return rc;
+  if (leaf_size == 0) leaf_size = ctx->leaf_size;
+  if (leaf_size < 4 || leaf_size > 256)
+    return fail(PM_ERR_INVALID_ARG, "leaf size must be in 4..256");
+  map->leaf_size = leaf_size;
+  hipStream_t s = as_stream(stream);
+  if (ctx->timing) PM_HIP(hipEventRecord(ctx->ev[0], s));
+  if (int rc = build_photon_map(map, s)) return rc;
+  if (ctx->timing) {
+    PM_HIP(hipEventRecord(ctx->ev[1], s));
+    PM_HIP(hipEventSynchronize(ctx->ev[1]));
+    ctx->stats.build_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
+  }
+  return PM_OK;
+}
+
+int pm_photonmap_update_flux(pm_photonmap* h, const float* flux_soa, int mem, void* stream) {
+  if (!h) return fail(PM_ERR_INVALID_ARG, "null photon map");
+  auto* map = reinterpret_cast<PhotonMapImpl*>(h);
+  if (int rc = set_device(map->ctx)) return rc;
+  if (map->n == 0) return PM_OK;
+  if (!flux_soa) return fail(PM_ERR_INVALID_ARG, "null flux");
+  hipStream_t s = as_stream(stream);
+  if (int rc = copy_in(map->flux_soa.ptr, flux_soa, 3 * map->n * sizeof(float), mem, s))
+    return rc;
+  if (int rc = update_flux(map, map->flux_soa.as<float>(), s)) return rc;
+  if (mem == PM_MEM_HOST) PM_HIP(hipStreamSynchronize(s));
+  return PM_OK;
+}
+
+int64_t pm_photonmap_size(const pm_photonmap* h) {
+  if (!h) return -1;
+  return reinterpret_cast<const PhotonMapImpl*>(h)->n;
+}
+
+int pm_photonmap_get_photon(pm_photonmap* h, int64_t i, float* out9) {
+  if (!h || !out9) return fail(PM_ERR_INVALID_ARG, "null argument");
+  auto* map = reinterpret_cast<PhotonMapImpl*>(h);
+  if (i < 0 || i >= map->n) return fail(PM_ERR_INVALID_ARG, "photon index out of range");
+  if (int rc = set_device(map->ctx)) return rc;
+  const int64_t n = map->n;
+  for (int c = 0; c < 3; ++c) {
+    PM_HIP(hipMemcpy(out9 + c, map->flux_soa.as<float>() + c * n + i, sizeof(float),
+                     hipMemcpyDeviceToHost));
+    PM_HIP(hipMemcpy(out9 + 3 + c, map->pos_soa.as<float>() + c * n + i, sizeof(float),
+                     hipMemcpyDeviceToHost));
+    PM_HIP(hipMemcpy(out9 + 6 + c, map->wi_soa.as<float>() + c * n + i, sizeof(float),
+                     hipMemcpyDeviceToHost));
+  }
+  return PM_OK;
+}
+
+int pm_knn(pm_photonmap* h, int64_t m, const float* q_soa, int k, int32_t* out_idx,
+           float* out_r2, int mem, void* stream) {
+  if (int rc = check_map(h)) return rc;
+  auto* map = reinterpret_cast<PhotonMapImpl*>(h);
+  Context* ctx = map->ctx;
+  if (k < 1) return fail(PM_ERR_INVALID_ARG, "k must be >= 1");
+  if (k > kMaxKnnK) return fail(PM_ERR_UNSUPPORTED, "pm_knn supports k <= 1024");
+  if (m < 0) return fail(PM_ERR_INVALID_ARG, "negative query count");
+  if (m == 0) return PM_OK;
+  if (!q_soa || !out_idx) return fail(PM_ERR_INVALID_ARG, "null query or output");
+  if (mem != PM_MEM_HOST && mem != PM_MEM_DEVICE) return fail(PM_ERR_INVALID_ARG, "bad mem");
+  if (m > (int64_t{1} << 31) - 1) return fail(PM_ERR_UNSUPPORTED, "m must be < 2^31");
+  hipStream_t s = as_stream(stream);
+  const size_t qbytes = static_cast<size_t>(3 * m) * sizeof(float);
+  const size_t ibytes = static_cast<size_t>(m) * k * sizeof(int32_t);
+  const float* dq = q_soa;
+  int32_t* didx = out_idx;
+  float* dr2 = out_r2;
+  if (mem == PM_MEM_HOST) {
+    if (int rc = ctx->ws_stage_in.reserve(qbytes)) return rc;
+    if (int rc = ctx->ws_stage_out.reserve(ibytes)) return rc;
+    if (int rc = ctx->ws_stage_aux.reserve(static_cast<size_t>(m) * sizeof(float))) return rc;
+    if (int rc = copy_in(ctx->ws_stage_in.ptr, q_soa, qbytes, PM_MEM_HOST, s)) return rc;
+    dq = ctx->ws_stage_in.as<float>();
+    didx = ctx->ws_stage_out.as<int32_t>();
+    dr2 = ctx->ws_stage_aux.as<float>();
+  }
+  const int64_t n = map->n;
+  if (n == 0) {
+    if (int rc = launch_fill_u32(reinterpret_cast<uint32_t*>(didx), 0xffffffffu, m * k, s))
+      return rc;
+    if (dr2)
+      if (int rc = launch_fill_f32(dr2, 0.0f, m, s)) return rc;
+  } else {
+    const int k_eff = static_cast<int>(std::min<int64_t>(k, n));
+    if (int rc = ensure_anchors(map, k_eff, s)) return rc;
+    GatherDesc d{};
+    fill_desc_map_part(map, &d);
+    d.q_soa = dq;
+    d.out_slots = didx;
+    d.out_r2 = dr2;
+    d.m = m;
+    d.k = k;
+    d.k_eff = k_eff;
+    d.all_members = k_eff >= n ? 1 : 0;
+    if (int rc = run_single(map, d, kModeKnn, s, didx)) return rc;
+  }
+  if (mem == PM_MEM_HOST) {
+    PM_HIP(hipMemcpyAsync(out_idx, didx, ibytes, hipMemcpyDeviceToHost, s));
+    if (out_r2)
+      PM_HIP(hipMemcpyAsync(out_r2, dr2, m * sizeof(float), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+  }
+  return PM_OK;
+}
+
+int pm_gather_radiance(pm_photonmap* h, int64_t m, const pm_query* q, const pm_material* mats,
+                       int n_mats, int k, double n_emit, float* out_rgb, float* out_r2,
+                       uint64_t* out_key, int mem, void* stream) {
+  if (int rc = check_map(h)) return rc;
+  auto* map = reinterpret_cast<PhotonMapImpl*>(h);
+  Context* ctx = map->ctx;
+  if (k < 1) return fail(PM_ERR_INVALID_ARG, "k must be >= 1");
+  if (m < 0) return fail(PM_ERR_INVALID_ARG, "negative query count");
+  if (m == 0) return PM_OK;
+  if (!q || !out_rgb) return fail(PM_ERR_INVALID_ARG, "null query or output");
+  if (n_mats < 0 || (n_mats > 0 && !mats)) return fail(PM_ERR_INVALID_ARG, "bad materials");
+  if (mem != PM_MEM_HOST && mem != PM_MEM_DEVICE) return fail(PM_ERR_INVALID_ARG, "bad mem");
+  if (m > (int64_t{1} << 31) - 1) return fail(PM_ERR_UNSUPPORTED, "m must be < 2^31");
+  hipStream_t s = as_stream(stream);
+  const pm_query* dq = q;
+  const pm_material* dm = mats;
+  float* drgb = out_rgb;
+  float* dr2 = out_r2;
+  uint64_t* dkey = out_key;
+  if (mem == PM_MEM_HOST) {
+    const size_t qb = static_cast<size_t>(m) * sizeof(pm_query);
+    const size_t ob = static_cast<size_t>(m) * (3 * sizeof(float) + sizeof(float) + sizeof(uint64_t));
+    if (int rc = ctx->ws_stage_in.reserve(qb)) return rc;
+    if (int rc = ctx->ws_stage_aux.reserve(std::max<size_t>(n_mats, 1) * sizeof(pm_material)))
+      return rc;
+    if (int rc = ctx->ws_stage_out.reserve(ob)) return rc;
+    if (int rc = copy_in(ctx->ws_stage_in.ptr, q, qb, PM_MEM_HOST, s)) return rc;
+    if (n_mats > 0)
+      if (int rc = copy_in(ctx->ws_stage_aux.ptr, mats, n_mats * sizeof(pm_material),
+                           PM_MEM_HOST, s))
+        return rc;
+    dq = ctx->ws_stage_in.as<pm_query>();
+    dm = ctx->ws_stage_aux.as<pm_material>();
+    drgb = ctx->ws_stage_out.as<float>();
+    dr2 = out_r2 ? drgb + 3 * m : nullptr;
+    dkey = out_key ? reinterpret_cast<uint64_t*>(drgb + 4 * m) : nullptr;
+  }
+  const int64_t n = map->n;
+  if (n == 0) {
+    if (int rc = launch_fill_f32(drgb, 0.0f, 3 * m, s)) return rc;
+    if (dr2)
+      if (int rc = launch_fill_f32(dr2, 0.0f, m, s)) return rc;
+    if (dkey)
+      if (int rc = launch_fill_u32(reinterpret_cast<uint32_t*>(dkey), 0u, 2 * m, s)) return rc;
+  } else {
+    const int k_eff = static_cast<int>(std::min<int64_t>(k, n));
+    if (int rc = ensure_anchors(map, k_eff, s)) return rc;
+    GatherDesc d{};
+    fill_desc_map_part(map, &d);
+    d.q = dq;
+    d.mats = dm;
+    d.n_mats = n_mats;
+    d.out_rgb = drgb;
+    d.out_r2 = dr2;
+    d.out_key = dkey;
+    d.m = m;
+    d.k = k;
+    d.k_eff = k_eff;
+    d.n_emit = static_cast<float>(n_emit);
+    d.all_members = k_eff >= n ? 1 : 0;
+    if (int rc = run_single(map, d, kModeRadiance, s, nullptr)) return rc;
+  }
+  if (mem == PM_MEM_HOST) {
+    PM_HIP(hipMemcpyAsync(out_rgb, drgb, 3 * m * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (out_r2)
+      PM_HIP(hipMemcpyAsync(out_r2, dr2, m * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (out_key)
+      PM_HIP(hipMemcpyAsync(out_key, dkey, m * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+  }
+  return PM_OK;
+}
+
+int pm_gather_adjoint(pm_photonmap* h, int64_t m, const pm_query* q, const pm_material* mats,
+                      int n_mats, int k, double n_emit, const float* dl_drgb,
+                      const uint64_t* key, const int32_t* fg_mat, float* dl_dflux,
+                      float* dl_dalbedo, int mem, void* stream) {
+  if (int rc = check_map(h)) return rc;
+  auto* map = reinterpret_cast<PhotonMapImpl*>(h);
+  Context* ctx = map->ctx;
+  if (k < 1) return fail(PM_ERR_INVALID_ARG, "k must be >= 1");
+  if (m < 0) return fail(PM_ERR_INVALID_ARG, "negative query count");
+  if (m == 0 || map->n == 0) return PM_OK;
+  if (!q || !dl_drgb || !dl_dflux) return fail(PM_ERR_INVALID_ARG, "null query or gradient");
+  if (n_mats < 0 || (n_mats > 0 && (!mats || !dl_dalbedo)))
+    return fail(PM_ERR_INVALID_ARG, "bad materials");
+  if (mem != PM_MEM_HOST && mem != PM_MEM_DEVICE) return fail(PM_ERR_INVALID_ARG, "bad mem");
+  if (m > (int64_t{1} << 31) - 1) return fail(PM_ERR_UNSUPPORTED, "m must be < 2^31");
+  hipStream_t s = as_stream(stream);
+  const int64_t n = map->n;
+  const pm_query* dq = q;
+  const pm_material* dm = mats;
+  const float* dg = dl_drgb;
+  const uint64_t* dkey = key;
+  const int32_t* dfg = fg_mat;
+  float* dflux = dl_dflux;
+  float* drho = dl_dalbedo;
+  if (mem == PM_MEM_HOST) {
+    // one staging block: queries | keys | grads | fg | materials | dflux | drho
+    const size_t qb = m * sizeof(pm_query), kb = key ? m * sizeof(uint64_t) : 0,
+                 gb = 3 * m * sizeof(float), fb = fg_mat ? m * sizeof(int32_t) : 0,
+                 mb = n_mats * sizeof(pm_material), xb = 3 * n * sizeof(float),
+                 rb = 3 * n_mats * sizeof(float);
+    auto al = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
+    if (int rc = ctx->ws_stage_in.reserve(al(qb) + al(kb) + al(gb) + al(fb) + al(mb) + 256))
+      return rc;
+    if (int rc = ctx->ws_stage_out.reserve(al(xb) + al(rb) + 256)) return rc;
+    char* in = static_cast<char*>(ctx->ws_stage_in.ptr);
+    char* out = static_cast<char*>(ctx->ws_stage_out.ptr);
+    size_t o = 0;
+    auto put = [&](const void* src, size_t bytes) -> void* {
+      void* dst = in + o;
+      o += al(bytes);
+      if (bytes) (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+      return dst;
+    };
+    dq = static_cast<const pm_query*>(put(q, qb));
+    dkey = key ? static_cast<const uint64_t*>(put(key, kb)) : nullptr;
+    dg = static_cast<const float*>(put(dl_drgb, gb));
+    dfg = fg_mat ? static_cast<const int32_t*>(put(fg_mat, fb)) : nullptr;
+    dm = n_mats ? static_cast<const pm_material*>(put(mats, mb)) : nullptr;
+    dflux = reinterpret_cast<float*>(out);
+    drho = n_mats ? reinterpret_cast<float*>(out + al(xb)) : nullptr;
+    PM_HIP(hipMemcpyAsync(dflux, dl_dflux, xb, hipMemcpyHostToDevice, s));
+    if (n_mats) PM_HIP(hipMemcpyAsync(drho, dl_dalbedo, rb, hipMemcpyHostToDevice, s));
+    PM_HIP(hipGetLastError());
+  }
+  const int k_eff = static_cast<int>(std::min<int64_t>(k, n));
+  if (int rc = ensure_anchors(map, k_eff, s)) return rc;
+  GatherDesc d{};
+  fill_desc_map_part(map, &d);
+  d.q = dq;
+  d.mats = dm;
+  d.n_mats = n_mats;
+  d.dl_drgb = dg;
+  d.key_in = dkey;
+  d.fg_mat = dfg;
+  d.dl_dflux = dflux;
+  d.dl_dalbedo = drho;
+  d.m = m;
+  d.k = k;
+  d.k_eff = k_eff;
+  d.n_emit = static_cast<float>(n_emit);
+  d.all_members = k_eff >= n ? 1 : 0;
+  if (int rc = run_single(map, d, kModeAdjoint, s, nullptr)) return rc;
+  if (mem == PM_MEM_HOST) {
+    PM_HIP(hipMemcpyAsync(dl_dflux, dflux, 3 * n * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (n_mats)
+      PM_HIP(hipMemcpyAsync(dl_dalbedo, drho, 3 * n_mats * sizeof(float),
+                            hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+  }
+  return PM_OK;
+}
+
+int pm_gather_batched(pm_context*, int, const pm_instance*, void*) {
+  return fail(PM_ERR_UNSUPPORTED, "pm_gather_batched is not available in this build");
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+// Internal declarations shared by the HIP translation units of libpm_gather.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pm_gather.h"
+
+namespace pm {
+
+// ---- tuning constants ------------------------------------------------------
+constexpr int kWave = 64;           // CDNA wavefront width
+constexpr int kBins = 32;           // selection histogram bins per query
+constexpr int kStashCap = 16;       // per-query candidates kept past the cut
+constexpr int kMortonBits = 10;     // per axis
+constexpr int kAnchorWindow = 4;    // anchors tried on each side of a group
+constexpr int kMaxKnnK = 1024;      // row-sort limit of pm_knn
+constexpr int kXcds = 8;            // MI355X accelerator complex dies
+
+// ---- error plumbing ----------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define PM_HIP(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return pm::hip_fail(e_, #call); \
+  } while (0)
+
+// ---- device buffer -----------------------------------------------------------
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  ~DevBuf();
+  int reserve(size_t need);  // grows (never shrinks); contents not kept
+  void release();
+  template <typename T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+// ---- gather instance descriptor (device-resident, read with scalar loads) ----
+// One per photon-map x query-batch pairing.  All pointers are device memory.
+struct alignas(16) GatherDesc {
+  // photon map, sorted by Morton code ("slot" order)
+  const float4* pos4;      // slot -> (x, y, z, original index as bits)
+  const float4* wif4;      // slot -> (wi.x, wi.y, wi.z, flux.r)
+  const float2* flux2;     // slot -> (flux.g, flux.b)
+  const float4* boxes;     // heap node -> [2n] = min, [2n+1] = max
+  const float4* anchors;   // anchor -> (centroid xyz, k-radius bound)
+  const uint32_t* anchor_code;  // anchor -> Morton code of its first slot
+  const int32_t* inv_perm; // original index -> slot
+  // queries
+  const pm_query* q;       // RADIANCE / ADJOINT: local query records
+  const float* q_soa;      // KNN: 3*m positions
+  const pm_material* mats;
+  const uint32_t* group_anchor;  // local group -> nearest anchor index
+  // outputs
+  float* out_rgb;
+  float* out_r2;
+  uint64_t* out_key;
+  int32_t* out_slots;      // KNN: m rows of k slots (unsorted)
+  float* anchor_out;       // ANCHOR: written radius bound per anchor
+  // adjoint
+  const float* dl_drgb;
+  const uint64_t* key_in;
+  const int32_t* fg_mat;
+  float* dl_dflux;
+  float* dl_dalbedo;
+  // sizes / parameters
+  int64_t m;               // queries of this instance
+  int64_t q_base;          // first sorted position of this instance
+  int64_t group_base;      // first global group of this instance
+  int64_t n_groups;
+  float qlo[3];            // Morton quantisation origin
+  float qscale[3];         // Morton quantisation scale (cells per unit)
+  int32_t n;               // photons
+  int32_t leaf_size;
+  int32_t n_leaves;
+  int32_t first_leaf;      // heap index of leaf 0 (power of two)
+  int32_t n_anchors;
+  int32_t anchor_stride;
+  int32_t n_mats;
+  int32_t k;               // requested k
+  int32_t k_eff;           // min(k, n)
+  float n_emit;
+  int32_t all_members;     // n <= k: every photon is a neighbour
+  int32_t pad_;
+};
+
+enum GatherMode : int {
+  kModeRadiance = 0,
+  kModeKnn = 1,
+  kModeAnchor = 2,
+  kModeAdjoint = 3,
+};
+
+struct GatherLaunch {
+  const GatherDesc* descs;     // device array
+  const int64_t* group_prefix; // device, n_inst + 1 entries
+  const uint32_t* perm;        // sorted position -> global query index
+  const int64_t* q_prefix;     // device, n_inst + 1 (global query offsets)
+  uint32_t* counters;          // 2 * kXcds work counters (+ stats)
+  int64_t total_groups;
+  int32_t n_inst;
+  int32_t xcd_aware;
+  int32_t count_stats;
+  int32_t mode;
+};
+
+// ---- host-side photon map --------------------------------------------------------
+struct Context;
+
+struct PhotonMapImpl {
+  Context* ctx = nullptr;
+  int64_t n = 0;
+  bool built = false;
+  int leaf_size = 32;
+  int n_leaves = 0;
+  int first_leaf = 1;
+  int anchor_stride = 8;
+  int n_anchors = 0;
+  int anchor_k = -1;       // k_eff the anchor radii were computed for
+  float qlo[3] = {0, 0, 0};
+  float qscale[3] = {1, 1, 1};
+  // original-order inputs (SoA, 3n each)
+  DevBuf pos_soa, wi_soa, flux_soa;
+  // built structure
+  DevBuf pos4, wif4, flux2, boxes, anchors, anchor_code, inv_perm, codes;
+  DevBuf desc;             // one GatherDesc for single-instance calls
+};
+
+struct Context {
+  int device = 0;
+  int n_cu = 256;
+  int waves_per_cu = 8;
+  int anchor_stride = 8;
+  int leaf_size = 32;
+  int xcd_aware = 1;
+  int timing = 0;
+  int count_stats = 0;
+  pm_stats stats{};
+  // workspace
+  DevBuf ws_keys, ws_vals, ws_keys2, ws_vals2, ws_counts, ws_partials;
+  DevBuf ws_group_anchor, ws_counters, ws_slots;
+  DevBuf ws_stage_in, ws_stage_out, ws_stage_aux;
+  DevBuf ws_tmp;
+  // descriptor staging: pinned host arena mirrored into a device arena;
+  // `stage_done` is recorded after each upload so the host side is only
+  // rewritten once the previous copy has completed.
+  void* stage_host = nullptr;
+  DevBuf stage_dev;
+  size_t stage_cap = 0;
+  size_t stage_off = 0;
+  hipEvent_t stage_done = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+// Descriptor staging (host -> device) for one API call.
+int stage_begin(Context* ctx, size_t max_bytes, hipStream_t s);
+void* stage_put(Context* ctx, const void* src, size_t bytes);  // returns device ptr
+int stage_commit(Context* ctx, hipStream_t s);
+
+// ---- launchers (defined in the .hip files) -----------------------------------------
+// Radix sort of (key, value) pairs on the low `bits` bits of the key.
+// Returns the index (0 or 1) of the buffer pair that holds the result.
+int radix_sort_u32(Context* ctx, uint32_t* keys, uint32_t* vals,
+                   uint32_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
+                   hipStream_t s, int* which);
+int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals,
+                   uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
+                   hipStream_t s, int* which);
+
+int build_photon_map(PhotonMapImpl* map, hipStream_t s);
+int update_flux(PhotonMapImpl* map, const float* flux_dev, hipStream_t s);
+int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s);
+void fill_desc_map_part(const PhotonMapImpl* map, GatherDesc* d);
+
+// Morton keys for all queries of all instances, stable sort, and the
+// nearest radius anchor of every group.  Group anchors land in
+// ctx->ws_group_anchor (global group order).
+int prepare_queries(Context* ctx, int n_inst, const GatherDesc* dev_descs,
+                    const int64_t* dev_q_prefix, const int64_t* dev_g_prefix,
+                    int64_t total_q, int64_t total_groups, hipStream_t s,
+                    const uint32_t** perm_out);
+
+int launch_gather(Context* ctx, const GatherLaunch& L, hipStream_t s);
+int launch_knn_rowsort(const GatherDesc* dev_desc, int64_t m, int32_t* out_idx,
+                       hipStream_t s);
+int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+int launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
+
+}  // namespace pm

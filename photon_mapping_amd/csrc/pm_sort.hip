@@ -1,0 +1,232 @@
+// Stable LSD radix sort of (key, uint32 value) pairs, 8-bit digits.
+//
+// Per digit pass: a tile histogram kernel, an exclusive scan over the
+// digit-major count table, and a scatter kernel that ranks each element
+// inside its tile with wave ballots (peers with the same digit) and per-wave
+// digit counts in LDS, so the relative order of equal digits is preserved.
+// Everything is plain kernels (no memset nodes), so a caller can capture the
+// sort in a HIP graph.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pm_internal.hpp"
+
+namespace pm {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // keys per block
+constexpr int kScanChunk = kSortThreads * kSortItems; // counts per scan block
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void digit_hist_kernel(
+    const K* __restrict__ keys, int64_t n, int shift, uint32_t* __restrict__ counts,
+    int64_t n_tiles) {
+  __shared__ uint32_t h[256];
+  const int t = threadIdx.x;
+  h[t] = 0;
+  __syncthreads();
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+#pragma unroll 4
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = base + static_cast<int64_t>(r) * kSortThreads + t;
+    if (i < n) {
+      const uint32_t d = static_cast<uint32_t>(keys[i] >> shift) & 255u;
+      atomicAdd(&h[d], 1u);
+    }
+  }
+  __syncthreads();
+  counts[static_cast<int64_t>(t) * n_tiles + blockIdx.x] = h[t];
+}
+
+// Exclusive scan helpers over a uint32 array (length up to 2^31 chunks).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum,
+                                                         uint32_t* total) {
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t off = 0;
+  uint32_t tot = 0;
+  for (int i = 0; i < kSortThreads / 64; ++i) {
+    if (i < w) off += wsum[i];
+    tot += wsum[i];
+  }
+  *total = tot;
+  __syncthreads();
+  return off + x - v;
+}
+
+__global__ __launch_bounds__(kSortThreads) void scan_reduce_kernel(
+    const uint32_t* __restrict__ in, int64_t n, uint32_t* __restrict__ partial) {
+  __shared__ uint32_t wsum[kSortThreads / 64];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk;
+  uint32_t s = 0;
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = base + static_cast<int64_t>(threadIdx.x) * kSortItems + r;
+    if (i < n) s += in[i];
+  }
+  uint32_t tot;
+  block_exclusive_scan(s, wsum, &tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// Single block: exclusive scan of the partial sums, in place.
+__global__ __launch_bounds__(kSortThreads) void scan_top_kernel(uint32_t* partial,
+                                                                int64_t n) {
+  __shared__ uint32_t wsum[kSortThreads / 64];
+  uint32_t carry = 0;
+  for (int64_t base = 0; base < n; base += kSortThreads) {
+    const int64_t i = base + threadIdx.x;
+    const uint32_t v = i < n ? partial[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan(v, wsum, &tot);
+    if (i < n) partial[i] = carry + ex;
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kSortThreads) void scan_down_kernel(
+    uint32_t* __restrict__ data, int64_t n, const uint32_t* __restrict__ partial) {
+  __shared__ uint32_t wsum[kSortThreads / 64];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk +
+                       static_cast<int64_t>(threadIdx.x) * kSortItems;
+  uint32_t v[kSortItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = base + r;
+    v[r] = i < n ? data[i] : 0u;
+    s += v[r];
+  }
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan(s, wsum, &tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = base + r;
+    if (i < n) data[i] = run;
+    run += v[r];
+  }
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
+    const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
+    uint32_t* __restrict__ vout, int64_t n, int shift,
+    const uint32_t* __restrict__ offsets, int64_t n_tiles) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wcnt[kSortThreads / 64][256];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  run[t] = offsets[static_cast<int64_t>(t) * n_tiles + blockIdx.x];
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+  for (int r = 0; r < kSortItems; ++r) {
+#pragma unroll
+    for (int i = 0; i < kSortThreads / 64; ++i) wcnt[i][t] = 0;
+    __syncthreads();
+    const int64_t idx = base + static_cast<int64_t>(r) * kSortThreads + t;
+    const bool valid = idx < n;
+    K key = 0;
+    uint32_t val = 0;
+    uint32_t d = 0;
+    if (valid) {
+      key = kin[idx];
+      val = vin[idx];
+      d = static_cast<uint32_t>(key >> shift) & 255u;
+    }
+    const uint64_t vmask = __ballot(valid);
+    uint64_t peers = valid ? vmask : ~vmask;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt_mask));
+    if (valid && rank == 0) wcnt[w][d] = static_cast<uint32_t>(__popcll(peers));
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[d] + rank;
+      for (int i = 0; i < w; ++i) pos += wcnt[i][d];
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    {
+      uint32_t add = 0;
+#pragma unroll
+      for (int i = 0; i < kSortThreads / 64; ++i) add += wcnt[i][t];
+      run[t] += add;
+    }
+    __syncthreads();
+  }
+}
+
+int exclusive_scan(Context* ctx, uint32_t* data, int64_t n, hipStream_t s) {
+  const int64_t n_chunks = (n + kScanChunk - 1) / kScanChunk;
+  if (int rc = ctx->ws_partials.reserve(static_cast<size_t>(n_chunks + 1) * sizeof(uint32_t)))
+    return rc;
+  uint32_t* partial = ctx->ws_partials.as<uint32_t>();
+  hipLaunchKernelGGL(scan_reduce_kernel, dim3(n_chunks), dim3(kSortThreads), 0, s, data,
+                     n, partial);
+  hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kSortThreads), 0, s, partial,
+                     n_chunks);
+  hipLaunchKernelGGL(scan_down_kernel, dim3(n_chunks), dim3(kSortThreads), 0, s, data, n,
+                     partial);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+template <typename K>
+int radix_sort_impl(Context* ctx, K* keys, uint32_t* vals, K* keys_alt,
+                    uint32_t* vals_alt, int64_t n, int bits, hipStream_t s,
+                    int* which) {
+  *which = 0;
+  if (n <= 1 || bits <= 0) return PM_OK;
+  const int64_t n_tiles = (n + kSortTile - 1) / kSortTile;
+  if (int rc = ctx->ws_counts.reserve(static_cast<size_t>(n_tiles) * 256 * sizeof(uint32_t)))
+    return rc;
+  uint32_t* counts = ctx->ws_counts.as<uint32_t>();
+  K* kb[2] = {keys, keys_alt};
+  uint32_t* vb[2] = {vals, vals_alt};
+  int cur = 0;
+  for (int shift = 0; shift < bits; shift += 8) {
+    hipLaunchKernelGGL(digit_hist_kernel<K>, dim3(n_tiles), dim3(kSortThreads), 0, s,
+                       kb[cur], n, shift, counts, n_tiles);
+    if (int rc = exclusive_scan(ctx, counts, n_tiles * 256, s)) return rc;
+    hipLaunchKernelGGL(digit_scatter_kernel<K>, dim3(n_tiles), dim3(kSortThreads), 0, s,
+                       kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, shift, counts,
+                       n_tiles);
+    PM_HIP(hipGetLastError());
+    cur ^= 1;
+  }
+  *which = cur;
+  return PM_OK;
+}
+
+}  // namespace
+
+int radix_sort_u32(Context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                   uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which) {
+  return radix_sort_impl<uint32_t>(ctx, keys, vals, keys_alt, vals_alt, n, bits, s, which);
+}
+
+int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                   uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which) {
+  return radix_sort_impl<uint64_t>(ctx, keys, vals, keys_alt, vals_alt, n, bits, s, which);
+}
+
+}  // namespace pm

@@ -1,0 +1,117 @@
+"""Adjoint of the radiance gather on the GPU.
+
+Checked three ways: against the FP64 oracle gradient on the same neighbour
+sets, between its two modes (neighbour sets reused from the forward pass vs
+recomputed), and against central finite differences of the GPU forward pass
+with common random numbers (same photons, same queries; the estimate is
+linear in flux and albedo, so the differences are exact up to FP32 noise).
+"""
+
+import numpy as np
+import pytest
+
+from util import materials, photons_surfaces, query_records
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(gpu, rng, n=20000, m=2000):
+    pos, wi, flux, nrm = photons_surfaces(rng, n)
+    sel = rng.integers(0, n, m)
+    qpos = pos[sel] + rng.normal(scale=0.002, size=(m, 3)).astype(np.float32)
+    q = query_records(rng, qpos, nrm[sel])
+    mats = materials(3)
+    pmap = gpu.PhotonMap()
+    pmap.set_photons_soa(pos, wi, flux)
+    pmap.build()
+    return pos, wi, flux, q, mats, pmap
+
+
+@pytest.mark.parametrize("k", [16, 100])
+def test_adjoint_matches_fp64_oracle(gpu, oracle, rng, k):
+    pos, wi, flux, q, mats, pmap = _setup(gpu, rng)
+    n_emit = 3e4
+    g = rng.normal(size=(len(q), 3)).astype(np.float32)
+    print(f"adjoint vs fp64 k={k}", flush=True)
+    _, _, key = pmap.gather_radiance(q, mats, k, n_emit)
+    dflux_r, drho_r = pmap.gather_adjoint(q, mats, k, n_emit, g, key=key)
+    dflux_c, drho_c = pmap.gather_adjoint(q, mats, k, n_emit, g, key=None)
+    bidx, br2 = oracle.brute_knn(pos, q[:, :3], k)
+    ef, er = oracle.adjoint_fp64(wi, flux, q, mats, bidx, br2, n_emit, g)
+    scale_f = np.abs(ef).max()
+    for got in (dflux_r, dflux_c):
+        np.testing.assert_allclose(got, ef, rtol=1e-4, atol=1e-5 * scale_f)
+    for got in (drho_r, drho_c):
+        np.testing.assert_allclose(got, er, rtol=1e-4, atol=1e-6 * np.abs(er).max())
+    # photons outside every neighbour set receive exactly nothing
+    used = np.zeros(len(pos), bool)
+    used[bidx[bidx >= 0]] = True
+    assert not dflux_r[:, ~used].any()
+
+
+def test_adjoint_directional_finite_difference(gpu, rng):
+    """<dL/dflux, v> against (L(flux + h v) - L(flux - h v)) / 2h on the GPU
+    forward pass, and the same for the albedos (target: 1e-3 relative)."""
+    pos, wi, flux, q, mats, pmap = _setup(gpu, rng, n=30000, m=4000)
+    k, n_emit = 100, 1e5
+    g = rng.normal(size=(len(q), 3)).astype(np.float32)
+
+    def loss(fl=None, mt=None):
+        if fl is not None:
+            pmap.update_flux(fl)
+        rgb, _, _ = pmap.gather_radiance(q, mats if mt is None else mt, k, n_emit)
+        return float(np.sum(rgb.astype(np.float64) * g))
+
+    _, _, key = pmap.gather_radiance(q, mats, k, n_emit)
+    dflux, drho = pmap.gather_adjoint(q, mats, k, n_emit, g, key=key)
+    v = rng.normal(size=flux.shape).astype(np.float32)
+    h = 1e-3 * float(np.abs(flux).mean())
+    fd = (loss(flux + h * v) - loss(flux - h * v)) / (2 * h)
+    pmap.update_flux(flux)
+    an = float(np.sum(dflux.T.astype(np.float64) * v))
+    assert abs(an - fd) <= 1e-3 * abs(fd), (an, fd)
+    for mi in range(mats.shape[0]):
+        for c in range(3):
+            mp, mm = mats.copy(), mats.copy()
+            hh = 1e-3
+            mp[mi, c] += hh
+            mm[mi, c] -= hh
+            fdr = (loss(mt=mp) - loss(mt=mm)) / (2 * hh)
+            assert abs(drho[mi, c] - fdr) <= 1e-3 * abs(fdr) + 1e-6, (mi, c, drho[mi, c], fdr)
+
+
+def test_adjoint_final_gathering_albedo(gpu, oracle, rng):
+    """With the primary-hit material of a final-gathering bounce, the forward
+    value is rho_fg * L: dflux is scaled by rho_fg and rho_fg receives g * L."""
+    pos, wi, flux, q, mats, pmap = _setup(gpu, rng, n=10000, m=1000)
+    k, n_emit = 32, 1e4
+    fg = rng.integers(0, 3, len(q)).astype(np.int32)
+    g = rng.normal(size=(len(q), 3)).astype(np.float32)
+    rgb, _, key = pmap.gather_radiance(q, mats, k, n_emit)
+    dflux, drho = pmap.gather_adjoint(q, mats, k, n_emit, g, key=key, fg_mat=fg)
+    rho_fg = mats[fg, :3]
+    bidx, br2 = oracle.brute_knn(pos, q[:, :3], k)
+    ef, er = oracle.adjoint_fp64(wi, flux, q, mats, bidx, br2, n_emit, g * rho_fg)
+    L = oracle.radiance_fp64(wi, flux, q, mats, bidx, br2, n_emit)
+    for mi in range(3):
+        er[mi] += np.sum((g * L)[fg == mi], axis=0)
+    np.testing.assert_allclose(dflux, ef, rtol=1e-4, atol=1e-5 * np.abs(ef).max())
+    np.testing.assert_allclose(drho, er, rtol=1e-4, atol=1e-6 * np.abs(er).max())
+
+
+def test_adjoint_all_photons(gpu, oracle, rng):
+    """k >= N: every photon is in every set, in both modes."""
+    pos, wi, flux, nrm = photons_surfaces(rng, 70)
+    q = query_records(rng, pos[:40] + np.float32(0.01), nrm[:40])
+    mats = materials(3)
+    pmap = gpu.PhotonMap()
+    pmap.set_photons_soa(pos, wi, flux)
+    pmap.build()
+    g = rng.normal(size=(40, 3)).astype(np.float32)
+    _, _, key = pmap.gather_radiance(q, mats, 100, 50.0)
+    bidx, br2 = oracle.brute_knn(pos, q[:, :3], 100)
+    ef, er = oracle.adjoint_fp64(wi, flux, q, mats, bidx, br2, 50.0, g)
+    for kk in (key, None):
+        dflux, drho = pmap.gather_adjoint(q, mats, 100, 50.0, g, key=kk)
+        np.testing.assert_allclose(dflux, ef, rtol=1e-4, atol=1e-6 * np.abs(ef).max())
+        np.testing.assert_allclose(drho, er, rtol=1e-4, atol=1e-6 * np.abs(er).max())

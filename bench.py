@@ -43,7 +43,7 @@ def bytes_per_query(k: int) -> int:
     return 40 + 36 * k + 16
 
 
-def cpu_baseline(pos, wi, flux, q, mats, k, n_emit, budget_s=12.0):
+def cpu_baseline(pos, wi, flux, q, mats, k, n_emit, budget_s=12.0, m_total=None):
     """Times the oracle's reference gather on a bounded sample of queries."""
     import numpy as np
 
@@ -69,7 +69,8 @@ def cpu_baseline(pos, wi, flux, q, mats, k, n_emit, budget_s=12.0):
         "unit": "queries/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{size} of the {len(q)} queries drawn at random (seed 0); CPU restatement "
+        "sample": (f"{size} of the {m_total or len(q)} queries drawn at random (seed 0); "
+                   f"CPU restatement "
                    f"of the reference kd-tree search + FP32 estimate, g++ -O3 no -march, "
                    f"OpenMP dynamic over queries; single-threaded tree build over "
                    f"{len(pos)} photons took {build_s:.2f} s (not in the rate)"),
@@ -156,32 +157,46 @@ def main() -> int:
 
     # ---- kernel timing pass (HIP events on the call's stream) --------------------
     ctx.set_option(_lib.PM_OPT_TIMING, 1)
-    ctx.set_option(_lib.PM_OPT_COUNT_STATS, 1)
     gms, sms = [], []
-    fallbacks = refinements = 0
     for _ in range(args.steps):
         step()
         st = ctx.stats()
         gms.append(st["gather_ms"])
         sms.append(st["sort_ms"])
-        fallbacks += st["fallbacks"]
-        refinements += st["refinements"]
     ctx.set_option(_lib.PM_OPT_TIMING, 0)
+    # one more call with work counters (atomics) for the statistics
+    ctx.set_option(_lib.PM_OPT_COUNT_STATS, 1)
+    step()
+    st = ctx.stats()
     ctx.set_option(_lib.PM_OPT_COUNT_STATS, 0)
+    groups = max(int(st["groups"]), 1)
+    work = {
+        "groups": groups,
+        "fallbacks": int(st["fallbacks"]),
+        "refinement_rounds": int(st["refinements"]),
+        "pass1_candidates_per_group": st["candidates1"] / groups,
+        "pass2_candidates_per_group": st["candidates2"] / groups,
+        "cut_lists": int(st["cut_lists"]),
+        # share of wave time per phase of a group (statistics run)
+        "phase_share": dict(zip(
+            ["query+bound", "lists1", "histogram", "thresholds", "lists2", "selection",
+             "stash+out", "queue"],
+            [round(c / max(sum(st["phase_cycles"]), 1), 4) for c in st["phase_cycles"]])),
+        "wave_kcycles_per_group": sum(st["phase_cycles"]) / groups / 1e3,
+    }
     gather_ms = float(np.mean(gms))
     sort_ms = float(np.mean(sms))
     bq = bytes_per_query(k)
     achieved = m * bq / (gather_ms * 1e-3) / 1e9
     log(f"step {ms_per_step:.3f} ms | gather kernel {gather_ms:.3f} ms | sort+anchors "
-        f"{sort_ms:.3f} ms | build {build_ms:.2f} ms | fallbacks {fallbacks} "
-        f"refinements {refinements}")
+        f"{sort_ms:.3f} ms | build {build_ms:.2f} ms | work {work}")
 
     cpu = None
     if not args.no_cpu:
         sel = torch.randperm(m, generator=torch.Generator().manual_seed(0))[:min(m, 2_000_000)]
         qs = q[sel.to(dev)].cpu().numpy()
         cpu = cpu_baseline(pos.cpu().numpy(), wi.cpu().numpy(), flux.cpu().numpy(), qs,
-                           s.materials(), k, n_emit, args.cpu_budget)
+                           s.materials(), k, n_emit, args.cpu_budget, m_total=m)
         log(f"cpu baseline: {cpu['value']:.0f} queries/s on {cpu['cores']} threads")
 
     out = {
@@ -219,8 +234,7 @@ def main() -> int:
             "bytes_per_query": bq,
         },
         "cpu_baseline": cpu,
-        "detail": {"sort_ms": sort_ms, "build_ms": build_ms, "fallbacks": fallbacks,
-                   "refinements": refinements},
+        "detail": {"sort_ms": sort_ms, "build_ms": build_ms, **work},
     }
     print(json.dumps(out), flush=True)
     return 0

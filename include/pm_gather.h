@@ -79,6 +79,13 @@ typedef struct pm_stats {
   int64_t groups;      /* query groups processed by the last launch       */
   int64_t fallbacks;   /* groups that re-ran with an unbounded radius     */
   int64_t refinements; /* histogram refinement passes in the last launch  */
+  int64_t candidates1; /* photons examined by the histogram pass          */
+  int64_t candidates2; /* photons examined by the selection pass          */
+  int64_t cut_lists;   /* candidate lists coarsened to fit (x1000: whole) */
+  /* wave cycles per phase of a query group, summed over groups: query and
+   * radius bound, pass-1 lists, pass-1 histogram, thresholds/refinement,
+   * pass-2 lists, pass-2 selection, stash and outputs, work queue */
+  int64_t phase_cycles[8];
 } pm_stats;
 
 /* Options for pm_context_set_option. */

@@ -55,7 +55,9 @@ class pm_material(C.Structure):
 class pm_stats(C.Structure):
     _fields_ = [("sort_ms", C.c_double), ("gather_ms", C.c_double), ("post_ms", C.c_double),
                 ("build_ms", C.c_double), ("groups", C.c_int64), ("fallbacks", C.c_int64),
-                ("refinements", C.c_int64)]
+                ("refinements", C.c_int64), ("candidates1", C.c_int64),
+                ("candidates2", C.c_int64), ("cut_lists", C.c_int64),
+                ("phase_cycles", C.c_int64 * 8)]
 
 
 class pm_instance(C.Structure):

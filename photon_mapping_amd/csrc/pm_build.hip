@@ -247,35 +247,38 @@ __global__ __launch_bounds__(kThreads) void group_anchor_kernel(
   group_anchor[g] = static_cast<uint32_t>(lo > 0 ? lo - 1 : 0);
 }
 
-// ---- kNN row sort: slots -> original indices, farthest first ------------------------
+// ---- kNN row sort: neighbour indices, farthest first ------------------------------
+// One wave per query row: recompute each neighbour's (d^2, index) key from
+// the original-order positions, bitonic-sort the row descending in LDS, and
+// write the indices back (slots past the set hold -1).
 __global__ __launch_bounds__(kWave) void knn_rowsort_kernel(const GatherDesc* __restrict__ dp,
                                                             int32_t* __restrict__ out_idx) {
   __shared__ uint64_t key[kMaxKnnK];
-  __shared__ int32_t slot[kMaxKnnK];
   const GatherDesc& D = *dp;
   const int64_t q = blockIdx.x;
   const int lane = threadIdx.x;
   const int k = D.k;
+  const int64_t n = D.n;
   int p2 = 64;
   while (p2 < k) p2 <<= 1;
   const float qx = D.q_soa[q], qy = D.q_soa[D.m + q], qz = D.q_soa[2 * D.m + q];
   int32_t* row = out_idx + q * k;
   for (int j = lane; j < p2; j += kWave) {
-    uint64_t kk = 0;  // padding sorts last (descending)
-    int32_t sl = -1;
+    uint64_t kk = 0;  // padding sorts last (descending); real keys are +1
     if (j < k) {
-      sl = row[j];
-      if (sl >= 0) {
-        const float4 p = D.pos4[sl];
-        const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+      const int32_t idx = row[j];
+      if (idx >= 0) {
+        const float dx = qx - D.pos_orig[idx];
+        const float dy = qy - D.pos_orig[n + idx];
+        const float dz = qz - D.pos_orig[2 * n + idx];
         float acc = dx * dx;
         acc = acc + dy * dy;
         acc = acc + dz * dz;
-        kk = ((static_cast<uint64_t>(__float_as_uint(acc)) << 32) | __float_as_uint(p.w)) + 1ull;
+        kk = ((static_cast<uint64_t>(__float_as_uint(acc)) << 32) |
+              static_cast<uint32_t>(idx)) + 1ull;
       }
     }
     key[j] = kk;
-    slot[j] = sl;
   }
   __syncthreads();
   // bitonic sort, descending
@@ -287,16 +290,16 @@ __global__ __launch_bounds__(kWave) void knn_rowsort_kernel(const GatherDesc* __
         const bool desc = ((i & size) == 0);
         const uint64_t a = key[i], b = key[j];
         if ((a < b) == desc) {
-          key[i] = b; key[j] = a;
-          const int32_t s = slot[i]; slot[i] = slot[j]; slot[j] = s;
+          key[i] = b;
+          key[j] = a;
         }
       }
       __syncthreads();
     }
   }
   for (int j = lane; j < k; j += kWave) {
-    const int32_t sl = slot[j];
-    row[j] = (key[j] != 0 && sl >= 0) ? static_cast<int32_t>(__float_as_uint(D.pos4[sl].w)) : -1;
+    const uint64_t kk = key[j];
+    row[j] = kk != 0 ? static_cast<int32_t>((kk - 1ull) & 0xffffffffull) : -1;
   }
 }
 
@@ -338,6 +341,7 @@ void fill_desc_map_part(const PhotonMapImpl* map, GatherDesc* d) {
   d->anchors = map->anchors.as<float4>();
   d->anchor_code = map->anchor_code.as<uint32_t>();
   d->inv_perm = map->inv_perm.as<int32_t>();
+  d->pos_orig = map->pos_soa.as<float>();
   for (int c = 0; c < 3; ++c) {
     d->qlo[c] = map->qlo[c];
     d->qscale[c] = map->qscale[c];

@@ -178,11 +178,18 @@ int run_single(PhotonMapImpl* map, GatherDesc d, int mode, hipStream_t s,
       ctx->stats.post_ms = elapsed_ms(ctx->ev[2], ctx->ev[3]);
     }
     if (ctx->count_stats) {
-      uint32_t c[2] = {0, 0};
-      PM_HIP(hipMemcpy(c, ctx->ws_counters.as<uint32_t>() + 2 * kXcds, sizeof(c),
+      uint32_t c[5] = {0, 0, 0, 0, 0};
+      PM_HIP(hipMemcpy(c, ctx->ws_counters.as<uint32_t>() + kStatBase, sizeof(c),
                        hipMemcpyDeviceToHost));
-      ctx->stats.fallbacks = c[0];
-      ctx->stats.refinements = c[1];
+      ctx->stats.fallbacks = c[kStatFallback - kStatBase];
+      ctx->stats.refinements = c[kStatRefine - kStatBase];
+      ctx->stats.candidates1 = c[kStatCand1 - kStatBase];
+      ctx->stats.candidates2 = c[kStatCand2 - kStatBase];
+      ctx->stats.cut_lists = c[kStatOverflow - kStatBase];
+      uint64_t ph[kPhases];
+      PM_HIP(hipMemcpy(ph, ctx->ws_counters.as<uint32_t>() + kPhaseBase, sizeof(ph),
+                       hipMemcpyDeviceToHost));
+      for (int i = 0; i < kPhases; ++i) ctx->stats.phase_cycles[i] = static_cast<int64_t>(ph[i]);
     }
   }
   return PM_OK;

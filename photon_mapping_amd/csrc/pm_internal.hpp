@@ -20,6 +20,19 @@ constexpr int kAnchorWindow = 4;    // anchors tried on each side of a group
 constexpr int kMaxKnnK = 1024;      // row-sort limit of pm_knn
 constexpr int kXcds = 8;            // MI355X accelerator complex dies
 
+// Layout of the per-launch counter block: kXcds work-queue counters, then
+// statistics (only written when statistics are requested).
+constexpr int kStatBase = 2 * kXcds;
+constexpr int kStatFallback = kStatBase + 0;   // groups re-run unbounded
+constexpr int kStatRefine = kStatBase + 1;     // histogram refinement rounds
+constexpr int kStatCand1 = kStatBase + 2;      // pass-1 candidate photons
+constexpr int kStatCand2 = kStatBase + 3;      // pass-2 candidate photons
+constexpr int kStatOverflow = kStatBase + 4;   // candidate lists cut short
+// Wave cycles per phase of a group (uint64 each, from word kPhaseBase).
+constexpr int kPhaseBase = 32;
+constexpr int kPhases = 8;
+constexpr int kCounterWords = kPhaseBase + 2 * kPhases;
+
 // ---- error plumbing ----------------------------------------------------------
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
@@ -53,6 +66,7 @@ struct alignas(16) GatherDesc {
   const float4* anchors;   // anchor -> (centroid xyz, k-radius bound)
   const uint32_t* anchor_code;  // anchor -> Morton code of its first slot
   const int32_t* inv_perm; // original index -> slot
+  const float* pos_orig;   // 3*n positions in original photon order (SoA)
   // queries
   const pm_query* q;       // RADIANCE / ADJOINT: local query records
   const float* q_soa;      // KNN: 3*m positions
@@ -62,7 +76,7 @@ struct alignas(16) GatherDesc {
   float* out_rgb;
   float* out_r2;
   uint64_t* out_key;
-  int32_t* out_slots;      // KNN: m rows of k slots (unsorted)
+  int32_t* out_slots;      // KNN: m rows of k photon indices (unsorted)
   float* anchor_out;       // ANCHOR: written radius bound per anchor
   // adjoint
   const float* dl_drgb;

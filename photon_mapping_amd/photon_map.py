@@ -110,7 +110,9 @@ class Context:
     def stats(self) -> dict:
         s = _lib.pm_stats()
         call("pm_context_get_stats", self.handle, C.byref(s))
-        return {f: getattr(s, f) for f, _ in s._fields_}
+        out = {f: getattr(s, f) for f, _ in s._fields_}
+        out["phase_cycles"] = list(s.phase_cycles)
+        return out
 
     def close(self) -> None:
         if self.handle:

@@ -88,6 +88,9 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=512)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--leaf", type=int, default=0)
+    ap.add_argument("--split", type=float, default=0.0,
+                    help="query-run split factor (0: library default)")
+    ap.add_argument("--waves", type=int, default=0, help="persistent waves per CU (0: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     args = ap.parse_args()
@@ -120,6 +123,10 @@ def main() -> int:
     m = q.shape[0]
     k = args.k
 
+    if args.split > 0:
+        ctx.set_option(_lib.PM_OPT_RUN_SPLIT, int(round(args.split * 1000)))
+    if args.waves > 0:
+        ctx.set_option(_lib.PM_OPT_WAVES_PER_CU, args.waves)
     pmap = pm.PhotonMap(ctx)
     pmap.set_photons_soa(pos, wi, flux)
     ctx.set_option(_lib.PM_OPT_TIMING, 1)
@@ -174,13 +181,14 @@ def main() -> int:
         "groups": groups,
         "fallbacks": int(st["fallbacks"]),
         "refinement_rounds": int(st["refinements"]),
-        "pass1_candidates_per_group": st["candidates1"] / groups,
-        "pass2_candidates_per_group": st["candidates2"] / groups,
+        "runs_per_group": st["runs"] / groups,
+        "centre_candidates_per_group": st["candidates1"] / groups,
+        "lane_candidates_per_run": st["candidates2"] / max(int(st["runs"]), 1),
         "cut_lists": int(st["cut_lists"]),
         # share of wave time per phase of a group (statistics run)
         "phase_share": dict(zip(
-            ["query+bound", "lists1", "histogram", "thresholds", "lists2", "selection",
-             "stash+out", "queue"],
+            ["query+bound", "centre", "lists", "pass A", "thresholds", "pass B",
+             "finish", "queue"],
             [round(c / max(sum(st["phase_cycles"]), 1), 4) for c in st["phase_cycles"]])),
         "wave_kcycles_per_group": sum(st["phase_cycles"]) / groups / 1e3,
     }
@@ -219,7 +227,7 @@ def main() -> int:
             "photons": args.photons,
             "queries": m,
             "k": k,
-            "leaf_size": args.leaf or 32,
+            "leaf_size": args.leaf or 8,
             "parallelism": "1 GPU",
         },
         "roofline": {

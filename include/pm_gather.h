@@ -79,12 +79,13 @@ typedef struct pm_stats {
   int64_t groups;      /* query groups processed by the last launch       */
   int64_t fallbacks;   /* groups that re-ran with an unbounded radius     */
   int64_t refinements; /* histogram refinement passes in the last launch  */
-  int64_t candidates1; /* photons examined by the histogram pass          */
-  int64_t candidates2; /* photons examined by the selection pass          */
+  int64_t candidates1; /* photons swept for the run-centre radius bounds  */
+  int64_t candidates2; /* photons of the per-lane candidate lists         */
   int64_t cut_lists;   /* candidate lists coarsened to fit (x1000: whole) */
+  int64_t runs;        /* runs of nearby queries (a group holds >= 1)     */
   /* wave cycles per phase of a query group, summed over groups: query and
-   * radius bound, pass-1 lists, pass-1 histogram, thresholds/refinement,
-   * pass-2 lists, pass-2 selection, stash and outputs, work queue */
+   * radius bound, run-centre bound, candidate lists, histogram pass,
+   * thresholds/refinement, selection pass, stash/scatter/outputs, queue */
   int64_t phase_cycles[8];
 } pm_stats;
 
@@ -92,9 +93,10 @@ typedef struct pm_stats {
 #define PM_OPT_TIMING 1         /* 0/1: record HIP events (syncs stream)    */
 #define PM_OPT_WAVES_PER_CU 2   /* persistent waves per CU (default 8)      */
 #define PM_OPT_ANCHOR_STRIDE 3  /* photons per radius anchor (default 8)    */
-#define PM_OPT_LEAF_SIZE 4      /* default kd-tree leaf size (default 32)   */
+#define PM_OPT_LEAF_SIZE 4      /* default kd-tree leaf size (default 8)    */
 #define PM_OPT_XCD_AWARE 5      /* 0/1: per-XCD work queues (default 1)     */
 #define PM_OPT_COUNT_STATS 6    /* 0/1: count fallbacks/refinements         */
+#define PM_OPT_RUN_SPLIT 7      /* query-run split factor x1000 (def. 1000) */
 
 /* ---- errors / version ----------------------------------------------------- */
 /* Thread-local text of the last failing call ("" when none). */

@@ -33,6 +33,7 @@ PM_OPT_ANCHOR_STRIDE = 3
 PM_OPT_LEAF_SIZE = 4
 PM_OPT_XCD_AWARE = 5
 PM_OPT_COUNT_STATS = 6
+PM_OPT_RUN_SPLIT = 7
 
 
 class PmError(RuntimeError):
@@ -56,7 +57,7 @@ class pm_stats(C.Structure):
     _fields_ = [("sort_ms", C.c_double), ("gather_ms", C.c_double), ("post_ms", C.c_double),
                 ("build_ms", C.c_double), ("groups", C.c_int64), ("fallbacks", C.c_int64),
                 ("refinements", C.c_int64), ("candidates1", C.c_int64),
-                ("candidates2", C.c_int64), ("cut_lists", C.c_int64),
+                ("candidates2", C.c_int64), ("cut_lists", C.c_int64), ("runs", C.c_int64),
                 ("phase_cycles", C.c_int64 * 8)]
 
 

@@ -491,6 +491,7 @@ int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s) {
   L.n_inst = 1;
   L.xcd_aware = 0;
   L.count_stats = 0;
+  L.run_split = ctx->run_split;
   L.mode = kModeAnchor;
   if (int rc = launch_gather(ctx, L, s)) return rc;
   map->anchor_k = k_eff;

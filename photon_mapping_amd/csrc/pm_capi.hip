@@ -162,6 +162,7 @@ int run_single(PhotonMapImpl* map, GatherDesc d, int mode, hipStream_t s,
   L.n_inst = 1;
   L.xcd_aware = ctx->xcd_aware;
   L.count_stats = ctx->count_stats;
+  L.run_split = ctx->run_split;
   L.mode = mode;
   if (int rc = launch_gather(ctx, L, s)) return rc;
   tm.mark(2);
@@ -178,7 +179,7 @@ int run_single(PhotonMapImpl* map, GatherDesc d, int mode, hipStream_t s,
       ctx->stats.post_ms = elapsed_ms(ctx->ev[2], ctx->ev[3]);
     }
     if (ctx->count_stats) {
-      uint32_t c[5] = {0, 0, 0, 0, 0};
+      uint32_t c[6] = {0, 0, 0, 0, 0, 0};
       PM_HIP(hipMemcpy(c, ctx->ws_counters.as<uint32_t>() + kStatBase, sizeof(c),
                        hipMemcpyDeviceToHost));
       ctx->stats.fallbacks = c[kStatFallback - kStatBase];
@@ -186,6 +187,7 @@ int run_single(PhotonMapImpl* map, GatherDesc d, int mode, hipStream_t s,
       ctx->stats.candidates1 = c[kStatCand1 - kStatBase];
       ctx->stats.candidates2 = c[kStatCand2 - kStatBase];
       ctx->stats.cut_lists = c[kStatOverflow - kStatBase];
+      ctx->stats.runs = c[kStatRuns - kStatBase];
       uint64_t ph[kPhases];
       PM_HIP(hipMemcpy(ph, ctx->ws_counters.as<uint32_t>() + kPhaseBase, sizeof(ph),
                        hipMemcpyDeviceToHost));
@@ -273,6 +275,10 @@ int pm_context_set_option(pm_context* h, int option, int64_t value) {
       break;
     case PM_OPT_XCD_AWARE: ctx->xcd_aware = value != 0; break;
     case PM_OPT_COUNT_STATS: ctx->count_stats = value != 0; break;
+    case PM_OPT_RUN_SPLIT:
+      if (value < 1 || value > 1000000) return fail(PM_ERR_INVALID_ARG, "run split in 1..1000000");
+      ctx->run_split = static_cast<float>(value) / 1000.0f;
+      break;
     default: return fail(PM_ERR_INVALID_ARG, "unknown option");
   }
   return PM_OK;

@@ -5,36 +5,45 @@
 // exact k-th smallest key
 //   key = (bits(d^2) << 32) | original photon index
 // which orders photons exactly like the reference's
-// std::priority_queue<std::pair<float,int>> (include/photon_map.h:107-126):
-//   pass 1  histogram of d^2 over [0, R0^2] in 32 linear bins (LDS), where R0
-//           is a guaranteed upper bound on the k-NN radius from precomputed
-//           radius anchors; the bin holding the k-th key is found;
-//   pass 2  every photon below that bin is a neighbour and is consumed on the
-//           spot (radiance sum, index write, ...), photons inside the bin are
-//           stashed in LDS (a handful), and a rank selection inside the stash
-//           finishes the set.
+// std::priority_queue<std::pair<float,int>> (include/photon_map.h:107-126).
+//
+// The lanes of a group are cut into runs of nearby queries (usually one run
+// per group).  For a run with centre c and query spread delta:
+//   centre  a wave-parallel histogram of d^2(c, p) over the photons of a
+//           guaranteed bound gives r_c, a radius holding at least k photons
+//           around c.  By the triangle inequality the k nearest photons of
+//           every lane q lie within R_q = r_c + |q - c| of q, and all of them
+//           within r_c + 2 delta of c: that sphere's subtrees are the run's
+//           candidate list (a few hundred photons at leaf size 8);
+//   pass A  per lane, a histogram of d^2 over [0, R_q^2] in 32 linear bins
+//           (LDS) finds the bin holding the k-th key;
+//   pass B  photons below that bin are neighbours and are consumed on the
+//           spot (radiance sum, index write, ...); photons inside the bin are
+//           stashed in registers (a handful), and a rank selection inside the
+//           stash finishes the set.
 // The neighbour set is {key <= tau}; tau's d^2 is r^2.  Bins that hold more
 // than kStashCap keys are split further (rare) until they fit.
 //
-// Memory-level parallelism: the candidate photons of a pass are the subtrees
-// of the photon tree that meet the box of the lanes' query balls.  They are
-// found by a breadth-first walk with one node per lane (one vector load per
-// tree level) and kept in LDS as slot ranges.  A group whose queries straddle
-// a jump of the Morton curve is cut into runs of nearby queries first, each
-// with its own box.  Candidates are staged into LDS 64 at a time by a
-// coalesced vector load, the next chunk being fetched into registers while
-// the current one is consumed through broadcast LDS reads.  The consume loops
-// are branch-free: the rare per-lane events (stash, index write) are recorded
-// in a 64-bit mask per chunk and drained once per chunk.
+// Candidate lists are found by a breadth-first walk of the implicit photon
+// tree that expands several levels per step so that one vector load per lane
+// tests up to 64 nodes at once.  Candidates of the per-lane passes are
+// staged into LDS 64 at a time by a coalesced vector load, the next chunk
+// being fetched into registers while the current one is consumed through
+// broadcast LDS reads.  The consume loops are branch-free: the rare per-lane
+// events (stash, index write) are recorded in a 64-bit mask per chunk and
+// drained once per chunk.
 //
 // d^2 and the Lambert sign tests use the reference's FP32 operation order
 // (include/photon_map.h:33-41, include/core.h:206-209); the file is compiled
 // with -ffp-contract=off and denormals preserved, so they are bit-exact.
+// Every radius bound carries an upward relative margin (>= 1e-5) that covers
+// the rounding of the FP32 arithmetic it is derived with.
 
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
 #include <cstdint>
+#include <type_traits>
 
 #include "pm_internal.hpp"
 
@@ -54,9 +63,10 @@ namespace {
 constexpr float kPiF = 3.14159265359f;  // include/core.h:7
 constexpr uint64_t kKeyMax = ~0ull;
 constexpr int kChunk = 64;        // candidate photons staged per LDS chunk
-constexpr int kMaxRanges = 256;   // candidate subtrees per pass
+constexpr int kMaxRanges = 256;   // candidate subtrees per list
 constexpr int kFront = 256;       // breadth-first frontier capacity
 constexpr int kDrhoSlots = 64;    // adjoint: materials accumulated in LDS
+constexpr int kCentreBins = 256;  // centre histogram bins
 
 template <typename T>
 __device__ __forceinline__ const PM_CONST T* cptr(const T* p) {
@@ -73,8 +83,7 @@ __device__ __forceinline__ float unif(float v) {
 }
 
 struct GatherShared {
-  uint32_t hist[(kBins + 1) * kWave];
-  uint64_t stash[kStashCap * kWave];
+  uint32_t hist[(kBins + 2) * kWave];  // per-lane bins; centre bins alias the start
   // The chunk staging buffers are dead while a candidate list is built, so
   // the breadth-first frontier shares their storage.
   union {
@@ -82,6 +91,7 @@ struct GatherShared {
       float4 pos[kChunk];
       float4 wif[kChunk];
       float2 flux[kChunk];
+      uint32_t slot[kChunk];
     } c;
     uint32_t front[2][kFront];
   } u;
@@ -89,6 +99,7 @@ struct GatherShared {
   uint32_t rpre[kMaxRanges + 1];
   float drho[3 * kDrhoSlots];
 };
+static_assert(kCentreBins <= (kBins + 2) * kWave, "centre bins alias the lane bins");
 
 __device__ __forceinline__ float d2_ref(float qx, float qy, float qz, float px,
                                         float py, float pz) {
@@ -110,8 +121,7 @@ __device__ __forceinline__ float dot_ref(float ax, float ay, float az, float bx,
 }
 
 // Conservative radius^2 for culling: every photon whose FP32 d^2 is <= r2
-// lies within sqrt(cull_bound(r2)) of the query on every axis, despite the
-// rounding of d^2 and of the box arithmetic.
+// lies within sqrt(cull_bound(r2)) of the query, despite the rounding of d^2.
 __device__ __forceinline__ float cull_bound(float r2) {
   return r2 * 1.00002f + 1e-30f;
 }
@@ -122,24 +132,40 @@ __device__ __forceinline__ uint64_t make_key(float d2, uint32_t idx) {
   return (static_cast<uint64_t>(__float_as_uint(d2)) << 32) | idx;
 }
 
-// Histogram bin of d^2 for the range bound r0sq: 32 linear bins over
-// [0, r0sq); bin kBins collects everything beyond.  Monotone in d^2.  A
-// photon with d^2 <= r0sq / (1 + 2^-20) always lands below kBins; the radius
-// bounds carry a larger upward margin than that.
-__device__ __forceinline__ uint32_t bin_of(float d2, float inv_w) {
-  const uint32_t b = static_cast<uint32_t>(d2 * inv_w);
-  return b < static_cast<uint32_t>(kBins) ? b : static_cast<uint32_t>(kBins);
+// Histogram bin of d^2 for a lane bracket [lo2, hi2): with inv_w =
+// kBins / (hi2 - lo2) and c1 = 1 - lo2 * inv_w, t = fma(d2, inv_w, c1) is
+// 1 + kBins (d2 - lo2) / (hi2 - lo2) up to rounding; bin 0 holds d^2 below
+// the bracket, bins 1..kBins split it linearly, bin kBins + 1 holds the rest.
+// Monotone in d^2, which is all the selection relies on: a key in bin 0 is
+// below every key of the higher bins.  A lane outside the current run uses
+// inv_w = 0, c1 = 64 (always bin kBins + 1); an unbounded lane inv_w = 0,
+// c1 = 1 (always bin 1).
+__device__ __forceinline__ uint32_t bin_of(float d2, float inv_w, float c1) {
+  const float t = fmaf(d2, inv_w, c1);
+  return static_cast<uint32_t>(fminf(fmaxf(t, 0.0f), static_cast<float>(kBins + 1)));
 }
 
-// Smallest non-negative float (as bits) whose bin is >= b.  bin_of is
-// monotone in d^2, so this is a lower-bound search over the bit patterns
-// [0, +inf]; bin_of(+inf) = kBins.
-__device__ __forceinline__ uint32_t bin_start_bits(uint32_t b, float inv_w) {
+// Smallest non-negative finite float (as bits) whose bin is >= b, or the
+// bits of +inf when there is none.  The quotient (b - c1) / inv_w lands
+// within a few ulps of the answer, which is then found by stepping; a
+// bisection covers any case the steps do not settle.
+__device__ __forceinline__ uint32_t bin_start_bits(uint32_t b, float inv_w, float c1) {
+  constexpr uint32_t kInfBits = 0x7f800000u;
+  if (b == 0) return 0;
+  auto ge = [&](uint32_t u) { return u < kInfBits && bin_of(__uint_as_float(u), inv_w, c1) >= b; };
+  const float x = (static_cast<float>(b) - c1) / inv_w;
+  uint32_t u = (x >= 0.0f) ? __float_as_uint(x) : 0u;  // NaN -> 0
+  u = u < kInfBits ? u : kInfBits;
+  for (int s = 0; s < 8; ++s) {
+    if (u > 0 && ge(u - 1)) { --u; continue; }
+    if (u < kInfBits && !ge(u)) { ++u; continue; }
+    return u;
+  }
   uint32_t lo = 0;
-  uint32_t hi = 0x7f800000u;
+  uint32_t hi = kInfBits;
   for (int it = 0; it < 32 && lo < hi; ++it) {
     const uint32_t mid = lo + ((hi - lo) >> 1);
-    if (bin_of(__uint_as_float(mid), inv_w) >= b) hi = mid;
+    if (ge(mid)) hi = mid;
     else lo = mid + 1;
   }
   return lo;
@@ -166,10 +192,6 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask, int lane) {
   return static_cast<uint32_t>(__popcll(mask & lt));
 }
 
-// 1.0f when c holds, 0.0f otherwise (a select; keeps the consume loops free
-// of exec-mask branches).
-__device__ __forceinline__ float one_if(bool c) { return c ? 1.0f : 0.0f; }
-
 // Per-lane selection state.
 struct SelState {
   uint64_t klo;   // keys < klo are neighbours
@@ -178,30 +200,44 @@ struct SelState {
   uint32_t cnt;   // number of candidate keys inside [klo, khi)
 };
 
-// Box containing every point within sqrt(r2) of any lane's query
-// (wave-uniform).  Lanes that take no part pass r2 < 0.
-struct GroupBox {
-  float x0, y0, z0, x1, y1, z1;
+// Per-lane stash of the keys inside [klo, khi), in registers: every access
+// uses a compile-time index, inserts go through selects.  With kSlots the
+// photon's slot is kept beside its key.
+template <bool kSlots>
+struct Stash {
+  uint64_t key[kStashCap];
+  uint32_t slot[kSlots ? kStashCap : 1];
+  uint32_t n;
+  __device__ __forceinline__ void reset() {
+#pragma unroll
+    for (int i = 0; i < kStashCap; ++i) key[i] = kKeyMax;
+#pragma unroll
+    for (int i = 0; i < (kSlots ? kStashCap : 1); ++i) slot[i] = 0u;
+    n = 0;
+  }
+  __device__ __forceinline__ void put(uint64_t k, uint32_t s) {
+#pragma unroll
+    for (int i = 0; i < kStashCap; ++i) {
+      const bool here = n == static_cast<uint32_t>(i);
+      key[i] = here ? k : key[i];
+      if (kSlots) slot[i] = here ? s : slot[i];
+    }
+    n += n < static_cast<uint32_t>(kStashCap) ? 1u : 0u;
+  }
 };
 
-__device__ __forceinline__ GroupBox group_box(float qx, float qy, float qz, float r2) {
-  float lo[3] = {INFINITY, INFINITY, INFINITY};
-  float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  if (r2 >= 0.0f) {
-    const float R = sqrtf(cull_bound(r2)) * 1.000004f;
-    const float q[3] = {qx, qy, qz};
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      float a = q[c] - R;
-      float b = q[c] + R;
-      lo[c] = a - (fabsf(a) * 4e-7f + 1e-30f);
-      hi[c] = b + (fabsf(b) * 4e-7f + 1e-30f);
-    }
-  }
-  GroupBox g;
-  g.x0 = wave_min(lo[0]); g.y0 = wave_min(lo[1]); g.z0 = wave_min(lo[2]);
-  g.x1 = wave_max(hi[0]); g.y1 = wave_max(hi[1]); g.z1 = wave_max(hi[2]);
-  return g;
+// Squared distances from c to the nearest point of the box [mn, mx] (0 when
+// c is inside) and to its farthest corner; +inf for an empty box.
+__device__ __forceinline__ void box_dist2(const float4& mn, const float4& mx, float cx,
+                                          float cy, float cz, float& near2, float& far2) {
+  const float ax = fmaxf(fmaxf(mn.x - cx, cx - mx.x), 0.0f);
+  const float ay = fmaxf(fmaxf(mn.y - cy, cy - mx.y), 0.0f);
+  const float az = fmaxf(fmaxf(mn.z - cz, cz - mx.z), 0.0f);
+  near2 = ax * ax + ay * ay + az * az;
+  const float fx = fmaxf(cx - mn.x, mx.x - cx);
+  const float fy = fmaxf(cy - mn.y, mx.y - cy);
+  const float fz = fmaxf(cz - mn.z, mx.z - cz);
+  far2 = fx * fx + fy * fy + fz * fz;
 }
 
 // Slot range [s0, s1) of the photons under tree node f.
@@ -221,76 +257,10 @@ __device__ __forceinline__ void node_slots(DescRef D, uint32_t f, int32_t& s0, i
   s1 = static_cast<int32_t>(b < D.n ? b : D.n);
 }
 
-// Breadth-first walk of the implicit heap-ordered tree (root 1, children 2f
-// and 2f+1, leaves from first_leaf), one frontier node per lane.  A node that
-// misses the box is dropped; a leaf, or a node lying entirely inside the box,
-// is emitted as a candidate subtree.  Emitted subtrees are disjoint and
-// together hold every photon inside the box.  Writes the subtree roots to
-// sh.rnode and the exclusive prefix of their slot counts to sh.rpre; returns
-// the number of subtrees, the total slot count in *total.  When the frontier
-// or the list would overflow, subtrees are taken whole (a superset; counted
-// in *stat when given).
-__device__ uint32_t build_candidates(DescRef D, const GroupBox& g, GatherShared& sh,
-                                     int lane, uint32_t* total, uint32_t* stat) {
-  const uint32_t first_leaf = static_cast<uint32_t>(D.first_leaf);
-  if (lane == 0) sh.u.front[0][0] = 1u;
-  __syncthreads();
-  uint32_t ncur = 1, nr = 0;
-  int cur = 0;
-  bool overflow = false;
-  while (ncur > 0 && !overflow) {
-    uint32_t nnext = 0;
-    for (uint32_t base = 0; base < ncur; base += kWave) {
-      const uint32_t i = base + lane;
-      const bool valid = i < ncur;
-      const uint32_t f = valid ? sh.u.front[cur][i] : 1u;
-      bool emit = false, push = false;
-      if (valid) {
-        const float4 mn = D.boxes[2 * f];
-        const float4 mx = D.boxes[2 * f + 1];
-        const bool hit = !(mx.x < g.x0 || mn.x > g.x1 || mx.y < g.y0 || mn.y > g.y1 ||
-                           mx.z < g.z0 || mn.z > g.z1);
-        const bool inside = mn.x >= g.x0 && mx.x <= g.x1 && mn.y >= g.y0 && mx.y <= g.y1 &&
-                            mn.z >= g.z0 && mx.z <= g.z1;
-        if (hit) {
-          if (f >= first_leaf || inside) emit = true;
-          else push = true;
-        }
-      }
-      uint64_t em = __ballot(emit);
-      uint64_t pmk = __ballot(push);
-      if (nnext + 2u * static_cast<uint32_t>(__popcll(pmk)) > static_cast<uint32_t>(kFront)) {
-        emit = emit || push;
-        push = false;
-        em = __ballot(emit);
-        pmk = 0ull;
-        if (stat && lane == 0) atomicAdd(stat, 1u);
-      }
-      const uint32_t ne = static_cast<uint32_t>(__popcll(em));
-      if (nr + ne > static_cast<uint32_t>(kMaxRanges)) {
-        overflow = true;
-        if (stat && lane == 0) atomicAdd(stat, 1000u);
-        break;
-      }
-      if (emit) sh.rnode[nr + lanes_below(em, lane)] = f;
-      if (push) {
-        const uint32_t at = nnext + 2u * lanes_below(pmk, lane);
-        sh.u.front[cur ^ 1][at] = 2u * f;
-        sh.u.front[cur ^ 1][at + 1] = 2u * f + 1u;
-      }
-      nr += ne;
-      nnext += 2u * static_cast<uint32_t>(__popcll(pmk));
-    }
-    __syncthreads();
-    cur ^= 1;
-    ncur = nnext;
-  }
-  if (overflow) {
-    nr = 1;
-    if (lane == 0) sh.rnode[0] = 1u;
-    __syncthreads();
-  }
-  // exclusive prefix of slot counts
+// Writes the exclusive prefix of the slot counts of the nr list entries in
+// sh.rnode to sh.rpre; returns nr, the total slot count in *total.
+__device__ uint32_t list_prefix(DescRef D, GatherShared& sh, int lane, uint32_t nr,
+                                uint32_t* total) {
   uint32_t run = 0;
   for (uint32_t base = 0; base < nr; base += kWave) {
     const uint32_t i = base + lane;
@@ -315,10 +285,147 @@ __device__ uint32_t build_candidates(DescRef D, const GroupBox& g, GatherShared&
   return static_cast<uint32_t>(uni(static_cast<int>(nr)));
 }
 
-// Iterates the candidate photons of the last build_candidates: body(cnt) is
-// called once per staged chunk, after sh.u.c.pos (and with WF also
-// sh.u.c.wif/flux) hold cnt candidates.  The next chunk's loads are in flight
-// while body runs.
+// Candidate list of the sphere (c, sqrt(r2)), where r2 bounds the squared
+// distance (exact arithmetic) of every photon wanted: the subtrees of the
+// implicit heap-ordered tree (root 1, children 2f and 2f+1, leaves from
+// first_leaf) that meet the sphere, disjoint, holding every photon inside it.
+// The walk is breadth-first and expands e levels per step, e as large as
+// keeps (frontier << e) within one wave, so one vector load per lane tests up
+// to 64 descendants at once.  A leaf, or (unless leaf_only) a node entirely
+// inside the sphere, is emitted.  r2 = +inf lists the root.  Writes the
+// subtree roots to sh.rnode and the exclusive prefix of their slot counts to
+// sh.rpre; returns the number of subtrees, the total slot count in *total.
+// When the list would overflow, the walk restarts with whole inner nodes
+// allowed, then with a depth cap above the level that overflowed (at most
+// 2^8 subtrees, so it fits); a frontier that would overflow emits its nodes
+// whole.  *fine (when given) tells whether the list kept leaf granularity
+// (no coarsening); coarsenings are counted in *stat when given.
+__device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2,
+                               GatherShared& sh, int lane, uint32_t* total, uint32_t* stat,
+                               bool leaf_only = false, bool* fine = nullptr) {
+  const uint32_t first_leaf = static_cast<uint32_t>(D.first_leaf);
+  const int depth_leaf = 31 - __builtin_clz(first_leaf);
+  uint32_t nr = 0;
+  bool whole = !(r2 < INFINITY) || depth_leaf == 0;
+  bool coarse = whole;
+  int max_depth = depth_leaf;
+  const float hit_lim = r2 * 1.00001f;
+  const float in_lim = r2 * 0.99999f;
+  for (int attempt = 0; attempt < 3 && !whole; ++attempt) {
+    nr = 0;
+    bool full = false;
+    int full_depth = 0;
+    if (lane == 0) sh.u.front[0][0] = 1u;
+    __syncthreads();
+    uint32_t ncur = 1;
+    int cur = 0;
+    int depth = 0;
+    while (ncur > 0 && !full) {
+      int e = 1;
+      while (depth + e < max_depth && (ncur << (e + 1)) <= static_cast<uint32_t>(kWave)) ++e;
+      const uint32_t width = ncur << e;
+      const bool last_lvl = depth + e == max_depth;
+      const uint32_t sub = (1u << e) - 1u;
+      uint32_t nnext = 0;
+      for (uint32_t base = 0; base < width; base += kWave) {
+        const uint32_t i = base + lane;
+        uint32_t f = 0;
+        bool emit = false, push = false;
+        if (i < width) {
+          f = (sh.u.front[cur][i >> e] << e) | (i & sub);
+          const float4 mn = D.boxes[2 * f];
+          const float4 mx = D.boxes[2 * f + 1];
+          float n2, f2;
+          box_dist2(mn, mx, cx, cy, cz, n2, f2);
+          if (n2 <= hit_lim) {
+            if (last_lvl || (!leaf_only && f2 <= in_lim)) emit = true;
+            else push = true;
+          }
+        }
+        uint64_t em = __ballot(emit);
+        uint64_t pmk = __ballot(push);
+        if (nnext + static_cast<uint32_t>(__popcll(pmk)) > static_cast<uint32_t>(kFront)) {
+          emit = emit || push;
+          push = false;
+          em = __ballot(emit);
+          pmk = 0ull;
+          coarse = true;
+          if (stat && lane == 0) atomicAdd(stat, 1u);
+        }
+        const uint32_t ne = static_cast<uint32_t>(__popcll(em));
+        if (nr + ne > static_cast<uint32_t>(kMaxRanges)) {
+          full = true;
+          full_depth = depth + e;
+          break;
+        }
+        if (emit) sh.rnode[nr + lanes_below(em, lane)] = f;
+        if (push) sh.u.front[cur ^ 1][nnext + lanes_below(pmk, lane)] = f;
+        nr += ne;
+        nnext += static_cast<uint32_t>(__popcll(pmk));
+      }
+      __syncthreads();
+      cur ^= 1;
+      ncur = nnext;
+      depth += e;
+    }
+    if (!full) break;
+    if (stat && lane == 0) atomicAdd(stat, 1000u);
+    if (leaf_only) {
+      leaf_only = false;
+    } else {
+      coarse = true;
+      max_depth = min(full_depth - 1, 8);
+      if (max_depth < 1) whole = true;
+    }
+  }
+  if (whole) {
+    nr = 1;
+    if (lane == 0) sh.rnode[0] = 1u;
+    __syncthreads();
+  }
+  if (fine) *fine = !coarse && max_depth == depth_leaf;
+  return list_prefix(D, sh, lane, nr, total);
+}
+
+// Drops the entries of the current list whose boxes miss the sphere
+// (c, sqrt(r2)) (r2 as for build_list) and rebuilds the prefix.
+__device__ uint32_t filter_list(DescRef D, GatherShared& sh, int lane, uint32_t nr, float cx,
+                                float cy, float cz, float r2, uint32_t* total) {
+  const float hit_lim = r2 * 1.00001f;
+  uint32_t out = 0;
+  for (uint32_t base = 0; base < nr; base += kWave) {
+    const uint32_t i = base + lane;
+    uint32_t f = 0;
+    bool keep = false;
+    if (i < nr) {
+      f = sh.rnode[i];
+      float n2, f2;
+      box_dist2(D.boxes[2 * f], D.boxes[2 * f + 1], cx, cy, cz, n2, f2);
+      keep = n2 <= hit_lim;
+    }
+    const uint64_t km = __ballot(keep);
+    // every lane's entry was read above; writes land below base + kWave
+    if (keep) sh.rnode[out + lanes_below(km, lane)] = f;
+    out += static_cast<uint32_t>(__popcll(km));
+  }
+  __syncthreads();
+  return list_prefix(D, sh, lane, out, total);
+}
+
+// Slot of the t-th candidate of the current list; r is the lane's cursor
+// into the list and only moves forward (t must not decrease between calls).
+__device__ __forceinline__ int32_t list_slot(DescRef D, const GatherShared& sh, uint32_t nr,
+                                             uint32_t t, uint32_t& r) {
+  while (r + 1 < nr && sh.rpre[r + 1] <= t) ++r;
+  int32_t s0, s1;
+  node_slots(D, sh.rnode[r], s0, s1);
+  return s0 + static_cast<int32_t>(t - sh.rpre[r]);
+}
+
+// Iterates the candidate photons of the current list: body(cnt) is called
+// once per staged chunk, after sh.u.c.pos (and with WF also sh.u.c.wif/flux)
+// hold cnt candidates.  The next chunk's loads are in flight while body
+// runs.
 template <bool WF, typename Body>
 __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int lane,
                                                uint32_t nr, uint32_t total, Body&& body) {
@@ -326,19 +433,13 @@ __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int 
   const float4* __restrict__ P = D.pos4;
   const float4* __restrict__ W = D.wif4;
   const float2* __restrict__ F = D.flux2;
-  // lane-private cursor: flat index t and its subtree r
   uint32_t t = lane;
   uint32_t r = 0;
-  auto locate = [&]() -> int32_t {
-    while (r + 1 < nr && sh.rpre[r + 1] <= t) ++r;
-    int32_t s0, s1;
-    node_slots(D, sh.rnode[r], s0, s1);
-    return s0 + static_cast<int32_t>(t - sh.rpre[r]);
-  };
   float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p;
   float2 f = make_float2(0.f, 0.f);
+  int32_t slot = 0;
   if (t < total) {
-    const int32_t slot = locate();
+    slot = list_slot(D, sh, nr, t, r);
     p = P[slot];
     if (WF) {
       w = W[slot];
@@ -350,11 +451,12 @@ __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int 
     if (WF) {
       sh.u.c.wif[lane] = w;
       sh.u.c.flux[lane] = f;
+      sh.u.c.slot[lane] = static_cast<uint32_t>(slot);
     }
     __syncthreads();
     t += kChunk;
     if (t < total) {
-      const int32_t slot = locate();
+      slot = list_slot(D, sh, nr, t, r);
       p = P[slot];
       if (WF) {
         w = W[slot];
@@ -365,6 +467,29 @@ __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int 
                                                                     : static_cast<uint32_t>(kChunk);
     body(cnt);
     __syncthreads();
+  }
+}
+
+// Lane-parallel sweep of the current list: every lane takes every 64th
+// candidate, four loads in flight at a time, and calls fn(position).
+template <typename Fn>
+__device__ __forceinline__ void sweep_lanes(DescRef D, const GatherShared& sh, int lane,
+                                            uint32_t nr, uint32_t total, Fn&& fn) {
+  const float4* __restrict__ P = D.pos4;
+  uint32_t r = 0;
+  for (uint32_t base = 0; base < total; base += 4 * kWave) {
+    float4 p[4];
+    bool ok[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t t = base + static_cast<uint32_t>(u * kWave + lane);
+      ok[u] = t < total;
+      p[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (ok[u]) p[u] = P[list_slot(D, sh, nr, t, r)];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (ok[u]) fn(p[u]);
   }
 }
 
@@ -389,21 +514,23 @@ __device__ __forceinline__ Cand load_cand(const GatherShared& sh, uint32_t j) {
   return c;
 }
 
-// Calls use(cand) for the cnt staged candidates, reading them in blocks of 8
-// into registers first (the LDS reads of a block are in flight together).
-// use returns a per-lane flag; the flags form a 64-bit mask over the chunk,
-// which is returned for draining.
+// Calls use(cand) for the cnt staged candidates, reading them in blocks
+// (8 positions, or 4 full records with WF) into registers first, so that the
+// LDS reads of a block are in flight together.  use returns a per-lane flag;
+// the flags form a 64-bit mask over the chunk, which is returned for
+// draining.
 template <bool WF, typename Use>
 __device__ __forceinline__ uint64_t consume(const GatherShared& sh, uint32_t cnt, Use&& use) {
+  constexpr uint32_t kBlk = WF ? 4 : 8;
   uint32_t mlo = 0, mhi = 0;
   uint32_t j = 0;
-  for (; j + 8 <= cnt; j += 8) {
-    Cand c[8];
+  for (; j + kBlk <= cnt; j += kBlk) {
+    Cand c[kBlk];
 #pragma unroll
-    for (uint32_t u = 0; u < 8; ++u) c[u] = load_cand<WF>(sh, j + u);
+    for (uint32_t u = 0; u < kBlk; ++u) c[u] = load_cand<WF>(sh, j + u);
     uint32_t bits = 0;
 #pragma unroll
-    for (uint32_t u = 0; u < 8; ++u) bits |= use(c[u]) ? (1u << u) : 0u;
+    for (uint32_t u = 0; u < kBlk; ++u) bits |= use(c[u]) ? (1u << u) : 0u;
     if (j < 32) mlo |= bits << j;
     else mhi |= bits << (j - 32);
   }
@@ -430,18 +557,72 @@ __device__ __forceinline__ void drain(uint64_t m, Fn&& fn) {
 
 __device__ __forceinline__ void clear_hist(GatherShared& sh, int lane) {
   uint4* h4 = reinterpret_cast<uint4*>(sh.hist);
-  for (int i = lane; i < (kBins + 1) * kWave / 4; i += kWave) h4[i] = make_uint4(0, 0, 0, 0);
+  for (int i = lane; i < (kBins + 2) * kWave / 4; i += kWave) h4[i] = make_uint4(0, 0, 0, 0);
+}
+
+__device__ __forceinline__ void lds_inc(uint32_t* p) {
+  __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ __forceinline__ void hist_add(GatherShared& sh, uint32_t b, int lane) {
-  __hip_atomic_fetch_add(&sh.hist[b * kWave + lane], 1u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_WORKGROUP);
+  lds_inc(&sh.hist[b * kWave + lane]);
+}
+
+// Bracket [t_lo, t_hi] of the k-th smallest FP32 d^2(c, p) over the photons
+// of the current list: fewer than k_eff of them have d^2 < t_lo, at least
+// k_eff have d^2 < t_hi.  A wave-parallel histogram of kCentreBins linear
+// bins over [0, rb); returns false when fewer than k_eff photons fall below
+// rb.
+__device__ bool centre_bound(DescRef D, GatherShared& sh, int lane, uint32_t nr,
+                             uint32_t total, float cx, float cy, float cz, float rb,
+                             uint32_t k_eff, float* t_lo, float* t_hi) {
+  uint32_t* ch = sh.hist;
+  for (int i = lane; i < kCentreBins; i += kWave) ch[i] = 0u;
+  __syncthreads();
+  const float inv_w = static_cast<float>(kCentreBins) / rb;
+  sweep_lanes(D, sh, lane, nr, total, [&](const float4& p) {
+    const float t = d2_ref(cx, cy, cz, p.x, p.y, p.z) * inv_w;
+    if (t < static_cast<float>(kCentreBins)) lds_inc(&ch[static_cast<uint32_t>(t)]);
+  });
+  __syncthreads();
+  static_assert(kCentreBins == 4 * kWave, "four centre bins per lane");
+  const uint4 h = reinterpret_cast<const uint4*>(ch)[lane];
+  const uint32_t s = h.x + h.y + h.z + h.w;
+  uint32_t incl = s;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += y;
+  }
+  const uint64_t hit = __ballot(incl >= k_eff);
+  __syncthreads();
+  if (hit == 0ull) return false;
+  const int fl = __builtin_ctzll(hit);
+  uint32_t b = 4u * static_cast<uint32_t>(lane);
+  uint32_t cum = incl - s;
+  if (cum + h.x < k_eff) {
+    cum += h.x;
+    ++b;
+    if (cum + h.y < k_eff) {
+      cum += h.y;
+      ++b;
+      if (cum + h.z < k_eff) ++b;
+    }
+  }
+  b = static_cast<uint32_t>(__shfl(static_cast<int>(b), fl, kWave));
+  b = static_cast<uint32_t>(uni(static_cast<int>(b)));
+  // photons of bins <= b have d^2 * inv_w < b + 1 before rounding of the
+  // product, hence d^2 < (b + 1) / inv_w up to the margin; photons with
+  // d^2 < b / inv_w (less the margin) all lie in bins < b
+  *t_hi = (static_cast<float>(b + 1u) / inv_w) * 1.00001f + 1e-30f;
+  *t_lo = (static_cast<float>(b) / inv_w) * 0.99999f;
+  return true;
 }
 
 // Wave-cycle accounting of the phases of a group, for statistics runs only
-// (acc == nullptr otherwise).  Phases: 0 query and radius bound, 1 pass-1
-// candidate lists, 2 pass-1 histogram, 3 thresholds and refinement, 4 pass-2
-// candidate lists, 5 pass-2 selection, 6 stash and outputs, 7 work queue.
+// (acc == nullptr otherwise).  Phases: 0 query and radius bound, 1 centre
+// bound, 2 candidate lists, 3 pass A, 4 thresholds and refinement, 5 pass B,
+// 6 stash, scatter and outputs, 7 work queue.
 struct PhaseTimer {
   uint64_t* acc;
   uint64_t t;
@@ -455,125 +636,136 @@ struct PhaseTimer {
   }
 };
 
-// Runs one pass over the candidates of every lane, where a lane's candidates
-// are the photons within sqrt(r2) of its query (lanes with r2 < 0 take no
-// part).  The Morton-sorted lanes are cut into runs wherever a query lies
-// more than twice the largest radius away from its predecessor, so that a
-// group straddling a jump of the Morton curve does not scan the space between
-// its pieces.  For each run the candidate list of its lanes is built, then
-// pass(part, nr, total) is called with part = "this lane is in the run".
-template <typename Pass>
-__device__ __forceinline__ void clustered(DescRef D, float qx, float qy, float qz, float r2,
-                                          GatherShared& sh, int lane, uint32_t* counters,
-                                          int count_stats, int cand_stat, PhaseTimer& pt,
-                                          int list_phase, int pass_phase, Pass&& pass) {
-  const bool in = r2 >= 0.0f;
-  const float R = in ? sqrtf(cull_bound(r2)) : 0.0f;
-  const float Rm = wave_max(R);
-  const float px = __shfl_up(qx, 1, kWave);
-  const float py = __shfl_up(qy, 1, kWave);
-  const float pz = __shfl_up(qz, 1, kWave);
-  const int prev_in = __shfl_up(in ? 1 : 0, 1, kWave);
-  const float gap = fmaxf(fabsf(qx - px), fmaxf(fabsf(qy - py), fabsf(qz - pz)));
-  const bool start = in && (lane == 0 || !prev_in || gap > 2.0f * Rm);
-  uint64_t starts = __ballot(start);
-  while (starts != 0ull) {
-    const int c0 = __builtin_ctzll(starts);
-    starts &= starts - 1ull;
-    const int c1 = starts != 0ull ? __builtin_ctzll(starts) : kWave;
-    const bool part = in && lane >= c0 && lane < c1;
-    const GroupBox g = group_box(qx, qy, qz, part ? r2 : -1.0f);
-    uint32_t total = 0;
-    const uint32_t nr = build_candidates(D, g, sh, lane, &total,
-                                         count_stats ? &counters[kStatOverflow] : nullptr);
-    if (count_stats && lane == 0) atomicAdd(&counters[cand_stat], total);
-    pt.mark(list_phase, lane);
-    pass(part, nr, total);
-    pt.mark(pass_phase, lane);
+// Selection for the lanes of one run (part), with centre c, lane distances
+// dq >= |q - c| and their maximum dmax.  r0sq is the lane's guaranteed bound
+// on its k-th FP32 d^2.  Afterwards the neighbour set of a part lane is
+// {key < klo} plus the `need` smallest keys of [klo, khi), and [klo, khi)
+// holds at most kStashCap keys.  Leaves the run's candidate list (covering
+// every key below the lane bounds) in sh.rnode/sh.rpre: returns its entry
+// count and its photon count in *total.
+__device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool part, float r0sq,
+                               float cx, float cy, float cz, float dq, float dmax,
+                               uint32_t k_eff, GatherShared& sh, int lane, uint32_t* counters,
+                               int count_stats, SelState& st, PhaseTimer& pt,
+                               uint32_t* total) {
+  uint32_t* cut_stat = count_stats ? &counters[kStatOverflow] : nullptr;
+  // ---- centre bracket: lane q's k nearest photons lie within X_q of q, hence
+  // within X_q + |q - c| of c, so the smallest such sum bounds the centre's
+  // k-NN radius; a histogram around c then brackets it
+  const float xq = part ? sqrtf(cull_bound(r0sq)) * 1.00002f + dq : INFINITY;
+  const float r0c = wave_min(xq) * 1.00002f;
+  float rc_hi = INFINITY, rc_lo = 0.0f;
+  uint32_t nr = 0;
+  bool fine = false;
+  if (r0c < INFINITY) {
+    // The list of the centre sweep also covers the run's final sphere
+    // (r_c <= r0c up to the margins), which is then a filter of it.
+    const float rl = (r0c * 1.0001f + 2.0f * dmax) * 1.0004f + 2e-30f;
+    nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, true, &fine);
+    if (count_stats && lane == 0) atomicAdd(&counters[kStatCand1], *total);
+    float t_lo = 0.0f, t_hi = INFINITY;
+    if (centre_bound(D, sh, lane, nr, *total, cx, cy, cz, r0c * r0c * 1.0001f + 1e-30f, k_eff,
+                     &t_lo, &t_hi)) {
+      rc_hi = sqrtf(t_hi) * 1.00001f + 1e-30f;
+      rc_lo = sqrtf(t_lo) * 0.99999f;
+    }
   }
-}
+  pt.mark(1, lane);
 
-// Determines the selection state of every lane: afterwards the neighbour set
-// of a lane is {key < klo} plus the `need` smallest keys of [klo, khi), and
-// [klo, khi) holds at most kStashCap keys.
-__device__ void select_range(DescRef D, float qx, float qy, float qz,
-                             bool active, float r0sq, uint32_t k_eff,
-                             GatherShared& sh, int lane, uint32_t* counters,
-                             int count_stats, SelState& st, PhaseTimer& pt) {
+  // ---- lane brackets and the run's sphere ---------------------------------------
+  // k >= k photons lie within rc_hi of c, so within rc_hi + |q - c| of q;
+  // fewer than k lie within rc_lo of c, so fewer than k within
+  // rc_lo - |q - c| of q.  Every neighbour of every lane lies within
+  // rc_hi + 2 dmax of c.
+  const float rq = (rc_hi + dq) * 1.00002f;
+  float hi2 = rq * rq * 1.0001f + 1e-30f;
+  const float lo = fmaxf(rc_lo - dq, 0.0f) * 0.99999f;
+  float lo2 = part ? lo * lo * 0.99999f : 0.0f;
+  const float rs = (rc_hi + 2.0f * dmax) * 1.0002f;
+  float rs2 = rs * rs + 1e-30f;
+
   st.klo = 0; st.khi = 0; st.need = 0; st.cnt = 0;
-  // ---- pass 1: histogram of d^2 over [0, R0^2] -------------------------------
   for (int attempt = 0; attempt < 2; ++attempt) {
-    // (the bins of inactive lanes are never read)
-    const float inv_w = active ? static_cast<float>(kBins) / r0sq : 0.0f;
+    if (attempt == 0 && rs2 < INFINITY && fine) {
+      nr = filter_list(D, sh, lane, nr, cx, cy, cz, rs2, total);
+    } else {
+      nr = build_list(D, cx, cy, cz, rs2, sh, lane, total, cut_stat);
+    }
+    if (count_stats && lane == 0) atomicAdd(&counters[kStatCand2], *total);
+    pt.mark(2, lane);
+    // ---- pass A: histogram of d^2 over the lane bracket [lo2, hi2) ---------------
+    float inv_w = 0.0f, c1 = 64.0f;  // lanes outside the run: always bin kBins + 1
+    if (part) {
+      inv_w = static_cast<float>(kBins) / (hi2 - lo2);
+      c1 = 1.0f - lo2 * inv_w;
+      if (!(inv_w < INFINITY) || !(fabsf(c1) < INFINITY)) {
+        inv_w = 0.0f;  // unbounded (or degenerate) bracket: every key in bin 1
+        c1 = 1.0f;
+      }
+    }
     clear_hist(sh, lane);
     __syncthreads();
-    clustered(D, qx, qy, qz, active ? r0sq : -1.0f, sh, lane, counters, count_stats,
-              kStatCand1, pt, 1, 2, [&](bool part, uint32_t nr, uint32_t total) {
-      for_candidates<false>(D, sh, lane, nr, total, [&](uint32_t cnt) {
-        consume<false>(sh, cnt, [&](const Cand& c) {
-          const float d2 = d2_ref(qx, qy, qz, c.p.x, c.p.y, c.p.z);
-          hist_add(sh, part ? bin_of(d2, inv_w) : static_cast<uint32_t>(kBins), lane);
-          return false;
-        });
+    for_candidates<false>(D, sh, lane, nr, *total, [&](uint32_t cnt) {
+      consume<false>(sh, cnt, [&](const Cand& c) {
+        hist_add(sh, bin_of(d2_ref(qx, qy, qz, c.p.x, c.p.y, c.p.z), inv_w, c1), lane);
+        return false;
       });
     });
     __syncthreads();
-    uint32_t cum = 0, below = 0, cb = 0, bsel = kBins;
+    pt.mark(3, lane);
+    uint32_t cum = 0, below = 0, cb = 0, bsel = kBins + 1;
 #pragma unroll
-    for (int j = 0; j < kBins; ++j) {
+    for (int j = 0; j <= kBins; ++j) {
       const uint32_t h = sh.hist[j * kWave + lane];
-      if (bsel == kBins && cum + h >= k_eff) {
+      if (bsel == kBins + 1 && cum + h >= k_eff) {
         bsel = j;
         below = cum;
         cb = h;
       }
       cum += h;
     }
-    const bool lane_ok = !active || bsel < kBins;
+    // the k-th key must fall inside the bracket (bins 1..kBins)
+    const bool lane_ok = !part || (bsel >= 1 && bsel <= kBins);
     if (!wave_any(!lane_ok)) {
-      if (active) {
-        const uint32_t lo_bits = bin_start_bits(bsel, inv_w);
-        const uint32_t hi_bits = bin_start_bits(bsel + 1, inv_w);
-        st.klo = static_cast<uint64_t>(lo_bits) << 32;
-        st.khi = static_cast<uint64_t>(hi_bits) << 32;
+      if (part) {
+        st.klo = static_cast<uint64_t>(bin_start_bits(bsel, inv_w, c1)) << 32;
+        st.khi = static_cast<uint64_t>(bin_start_bits(bsel + 1, inv_w, c1)) << 32;
         st.need = k_eff - below;
         st.cnt = cb;
       }
       break;
     }
-    // A lane found fewer than k photons inside its bound.  The bounds are
-    // guaranteed, so this is a guard only: retry that lane unbounded.
+    // The brackets are guaranteed, so this is a guard only: retry the lanes
+    // that missed unbounded over every photon.
     if (count_stats && lane == 0) atomicAdd(&counters[kStatFallback], 1u);
-    if (active && !lane_ok) r0sq = INFINITY;
+    if (!lane_ok) {
+      hi2 = INFINITY;
+      lo2 = 0.0f;
+    }
+    rs2 = INFINITY;
   }
-  pt.mark(3, lane);
 
   // ---- refinement: split [klo, khi) until it holds <= kStashCap keys --------
   // Each round shrinks the key span of a refining lane 32-fold, so 64-bit
   // spans end within 13 rounds; the cap only bounds the loop if the counts
   // were ever inconsistent.
   for (int round = 0;
-       round < 16 && wave_any(active && st.cnt > static_cast<uint32_t>(kStashCap));
-       ++round) {
+       round < 16 && wave_any(part && st.cnt > static_cast<uint32_t>(kStashCap)); ++round) {
     if (count_stats && lane == 0) atomicAdd(&counters[kStatRefine], 1u);
-    const bool mine = active && st.cnt > static_cast<uint32_t>(kStashCap);
+    const bool mine = part && st.cnt > static_cast<uint32_t>(kStashCap);
     const uint64_t span = st.khi - st.klo;
     const uint64_t w = mine ? (span + kBins - 1) / kBins : 1ull;
     clear_hist(sh, lane);
     __syncthreads();
-    const float r2hi = __uint_as_float(static_cast<uint32_t>(st.khi >> 32));
-    clustered(D, qx, qy, qz, mine ? r2hi : -1.0f, sh, lane, counters, 0, kStatCand2, pt, 3,
-              3, [&](bool part, uint32_t nr, uint32_t total) {
-      for_candidates<false>(D, sh, lane, nr, total, [&](uint32_t cnt) {
-        for (uint32_t j = 0; j < cnt; ++j) {
-          const float4 p = sh.u.c.pos[j];
-          const float d2 = d2_ref(qx, qy, qz, p.x, p.y, p.z);
-          const uint64_t key = make_key(d2, __float_as_uint(p.w));
-          if (part && key >= st.klo && key < st.khi) {
-            hist_add(sh, static_cast<uint32_t>((key - st.klo) / w), lane);
-          }
+    for_candidates<false>(D, sh, lane, nr, *total, [&](uint32_t cnt) {
+      for (uint32_t j = 0; j < cnt; ++j) {
+        const float4 p = sh.u.c.pos[j];
+        const float d2 = d2_ref(qx, qy, qz, p.x, p.y, p.z);
+        const uint64_t key = make_key(d2, __float_as_uint(p.w));
+        if (mine && key >= st.klo && key < st.khi) {
+          hist_add(sh, static_cast<uint32_t>((key - st.klo) / w), lane);
         }
-      });
+      }
     });
     __syncthreads();
     if (mine) {
@@ -594,12 +786,17 @@ __device__ void select_range(DescRef D, float qx, float qy, float qz,
     }
     __syncthreads();
   }
+  pt.mark(4, lane);
+  return nr;
 }
+
+template <int K>
+using Kind = std::integral_constant<int, K>;
 
 template <int MODE>
 __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict__ perm,
                               GatherShared& sh, int lane, uint32_t* counters,
-                              int count_stats, PhaseTimer& pt) {
+                              int count_stats, float split, PhaseTimer& pt) {
   const int64_t qloc = lg * kWave + lane;
   const bool active = qloc < D.m;
   const uint32_t k_eff = static_cast<uint32_t>(D.k_eff);
@@ -638,10 +835,16 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
     }
   }
 
-  // ---- radius bound R0 ---------------------------------------------------------
-  float r0sq = -1.0f;  // inactive lanes never match anything
+  // ---- radius bound: r0sq >= the lane's k-th FP32 d^2 --------------------------
   const bool reuse = (MODE == kModeAdjoint) && D.key_in != nullptr;
-  if (D.all_members || reuse) {
+  const bool all = D.all_members != 0;
+  uint64_t tau_in = 0;
+  float r0sq = -1.0f;  // inactive lanes never match anything
+  if (reuse) {
+    // the forward pass's key is the largest key of the neighbour set
+    if (active) tau_in = D.key_in[qi];
+    r0sq = __uint_as_float(static_cast<uint32_t>(tau_in >> 32));
+  } else if (all) {
     r0sq = INFINITY;
   } else if (MODE == kModeAnchor) {
     // Union box of the leaves around this anchor that hold >= k photons;
@@ -669,7 +872,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
     const float ex = fmaxf(fabsf(qx - mn.x), fabsf(mx.x - qx));
     const float ey = fmaxf(fabsf(qy - mn.y), fabsf(mx.y - qy));
     const float ez = fmaxf(fabsf(qz - mn.z), fabsf(mx.z - qz));
-    if (active) r0sq = (ex * ex + ey * ey + ez * ez) * 1.0001f + 1e-30f;
+    r0sq = (ex * ex + ey * ey + ez * ez) * 1.0001f + 1e-30f;
   } else {
     // Triangle inequality on precomputed anchors: the k nearest photons of
     // anchor a lie within r_a of its centroid c_a, hence within
@@ -685,200 +888,299 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
       best = fminf(best, an.w + sqrtf(dx * dx + dy * dy + dz * dz));
     }
     best *= 1.00002f;
-    if (active) r0sq = best * best * 1.00002f + 1e-30f;
+    r0sq = best * best * 1.00002f + 1e-30f;
   }
   if (!active) r0sq = -1.0f;
   pt.mark(0, lane);
 
-  SelState st;
-  if (reuse) {
-    // The forward pass's key is the largest key of the neighbour set.
-    const uint64_t tau = active ? D.key_in[qi] : 0ull;
-    st.klo = tau + 1; st.khi = tau + 1; st.need = 0; st.cnt = 0;
-  } else if (D.all_members) {
-    // Every photon is a neighbour; pass 2 tracks the largest key.
-    st.klo = kKeyMax; st.khi = kKeyMax; st.need = 0; st.cnt = 0;
-  } else {
-    select_range(D, qx, qy, qz, active, r0sq, k_eff, sh, lane, counters, count_stats, st,
-                 pt);
-  }
-
-  // ---- pass 2: consume neighbours below klo, stash [klo, khi) -------------------
-  // (the adjoint runs it only to finish the selection; its scatter pass
-  // follows once tau is known)
   constexpr bool kAccum = MODE == kModeRadiance;
   constexpr bool kRows = MODE == kModeKnn;
+  // the estimate only counts photons when the outgoing side passes
+  const bool lane_lambert = lambert && !(cos_o < 0.0f);
   float acc_r = 0.f, acc_g = 0.f, acc_b = 0.f;
-  uint32_t n_stash = 0;
   uint32_t n_written = 0;
   uint64_t key_max = 0;
+  uint64_t tau = 0;
   int32_t* row = nullptr;
   if (kRows) row = D.out_slots + qi * D.k;
-  const float wt_o = one_if(lambert && !(cos_o < 0.0f));  // outgoing side passes
-  if (!reuse) {
-    const bool all = D.all_members != 0;
-    const float r2hi = all ? INFINITY : __uint_as_float(static_cast<uint32_t>(st.khi >> 32));
-    const uint64_t span = st.khi - st.klo;
-    clustered(D, qx, qy, qz, active ? r2hi : -1.0f, sh, lane, counters, count_stats,
-              kStatCand2, pt, 4, 5, [&](bool part, uint32_t nr, uint32_t total) {
-      const float wt_part = part ? 1.0f : 0.0f;
-      for_candidates<kAccum>(D, sh, lane, nr, total, [&](uint32_t cnt) {
-        const uint64_t m = consume<kAccum>(sh, cnt, [&](const Cand& c) {
-          const float d2 = d2_ref(qx, qy, qz, c.p.x, c.p.y, c.p.z);
-          const uint64_t key = make_key(d2, __float_as_uint(c.p.w));
-          const float wm = one_if(key < st.klo) * wt_part;  // neighbour of this lane
-          if (kAccum) {
-            const float cos_i = dot_ref(c.w.x, c.w.y, c.w.z, nx, ny, nz);
-            const float sel = wm * wt_o * one_if(!(cos_i < 0.0f));
-            acc_r = fmaf(sel, c.w.w, acc_r);
-            acc_g = fmaf(sel, c.f.x, acc_g);
-            acc_b = fmaf(sel, c.f.y, acc_b);
-          }
-          if (all) key_max = (part && key > key_max) ? key : key_max;
-          // flagged: inside the stash range, or (kNN rows) a neighbour
-          const bool in_stash = (key - st.klo) < span;
-          return part && (in_stash || (kRows && key < st.klo));
-        });
-        drain(m, [&](uint32_t j) {
-          const float4 p = sh.u.c.pos[j];
-          const float d2 = d2_ref(qx, qy, qz, p.x, p.y, p.z);
-          const uint32_t idx = __float_as_uint(p.w);
-          const uint64_t key = make_key(d2, idx);
-          if (kRows && key < st.klo) {
-            if (n_written < static_cast<uint32_t>(D.k)) row[n_written++] = static_cast<int32_t>(idx);
-          } else if (n_stash < static_cast<uint32_t>(kStashCap)) {
-            sh.stash[n_stash * kWave + lane] = key;
-            ++n_stash;
-          }
-        });
-      });
-    });
-  }
 
-  // ---- finish the set inside the stash ---------------------------------------------
-  uint64_t tau = 0;
-  if (reuse) {
-    tau = st.klo - 1;
-  } else if (D.all_members) {
-    tau = key_max;
-  } else if (active) {
-    for (uint32_t i = 0; i < n_stash; ++i) {
-      const uint64_t ki = sh.stash[i * kWave + lane];
-      uint32_t rank = 0;
-      for (uint32_t j = 0; j < n_stash; ++j) rank += sh.stash[j * kWave + lane] < ki ? 1u : 0u;
-      if (rank + 1 == st.need) tau = ki;
-      if (rank < st.need) {
-        const uint32_t idx = static_cast<uint32_t>(ki & 0xffffffffu);
-        if (kAccum) {
-          const int32_t s = D.inv_perm[idx];
-          const float4 w = D.wif4[s];
-          const float2 f = D.flux2[s];
-          const float cos_i = dot_ref(w.x, w.y, w.z, nx, ny, nz);
-          const float sel = wt_o * one_if(!(cos_i < 0.0f));
-          acc_r = fmaf(sel, w.w, acc_r);
-          acc_g = fmaf(sel, f.x, acc_g);
-          acc_b = fmaf(sel, f.y, acc_b);
-        } else if (kRows) {
-          if (n_written < static_cast<uint32_t>(D.k)) row[n_written++] = static_cast<int32_t>(idx);
-        }
+  const uint32_t n_act = static_cast<uint32_t>(__popcll(__ballot(active)));
+  if (k_eff > 0 && n_act > 0) {
+    // ---- runs of nearby lanes ------------------------------------------------
+    const float scale = all ? INFINITY
+                            : wave_min(active ? sqrtf(cull_bound(r0sq)) : INFINITY);
+    const float thr = split * scale;
+    const float px = __shfl_up(qx, 1, kWave);
+    const float py = __shfl_up(qy, 1, kWave);
+    const float pz = __shfl_up(qz, 1, kWave);
+    const float gap = fmaxf(fabsf(qx - px), fmaxf(fabsf(qy - py), fabsf(qz - pz)));
+    uint64_t starts = __ballot(active && lane > 0 && gap > thr) | 1ull;
+    uint32_t c0 = 0;
+    while (c0 < n_act) {
+      const uint64_t later = c0 >= 63 ? 0ull : (starts >> (c0 + 1)) << (c0 + 1);
+      uint32_t c1 = later ? static_cast<uint32_t>(__builtin_ctzll(later)) : n_act;
+      c1 = c1 < n_act ? c1 : n_act;
+      const bool part = static_cast<uint32_t>(lane) >= c0 && static_cast<uint32_t>(lane) < c1;
+      const float lx = wave_min(part ? qx : INFINITY);
+      const float ly = wave_min(part ? qy : INFINITY);
+      const float lz = wave_min(part ? qz : INFINITY);
+      const float hx = wave_max(part ? qx : -INFINITY);
+      const float hy = wave_max(part ? qy : -INFINITY);
+      const float hz = wave_max(part ? qz : -INFINITY);
+      const float ext = fmaxf(hx - lx, fmaxf(hy - ly, hz - lz));
+      if (c1 - c0 > 1 && ext > thr) {
+        starts |= 1ull << ((c0 + c1) >> 1);
+        continue;
       }
-    }
-  }
-  const float r2 = (k_eff > 0) ? __uint_as_float(static_cast<uint32_t>(tau >> 32)) : 0.0f;
+      if (count_stats && lane == 0) atomicAdd(&counters[kStatRuns], 1u);
+      const float cx = lx + 0.5f * (hx - lx);
+      const float cy = ly + 0.5f * (hy - ly);
+      const float cz = lz + 0.5f * (hz - lz);
+      const float dq = part ? sqrtf(d2_ref(qx, qy, qz, cx, cy, cz)) * 1.00002f + 1e-30f : 0.0f;
+      const float dmax = wave_max(dq);
 
-  if (MODE == kModeAdjoint) {
-    // ---- scatter pass: dL/dflux over the neighbour set {key <= tau} -------------
-    const float denom = (D.n_emit * kPiF) * r2;
-    float g_r = 0.f, g_g = 0.f, g_b = 0.f;
-    float fg_r = 1.f, fg_g = 1.f, fg_b = 1.f;
-    int32_t fgm = -1;
-    if (active && lambert && k_eff > 0) {
-      g_r = D.dl_drgb[3 * qi + 0];
-      g_g = D.dl_drgb[3 * qi + 1];
-      g_b = D.dl_drgb[3 * qi + 2];
-      if (D.fg_mat) {
-        fgm = D.fg_mat[qi];
-        if (fgm >= 0 && fgm < D.n_mats) {
-          const pm_material fm = D.mats[fgm];
-          fg_r = fm.rho[0]; fg_g = fm.rho[1]; fg_b = fm.rho[2];
-        } else {
-          fgm = -1;
-        }
+      // ---- selection state and candidate list of the run --------------------
+      Stash<kAccum> stash;  // a lane stashes only in its own run
+      stash.reset();
+      SelState st;
+      uint32_t total = 0;
+      uint32_t nr = 0;
+      if (reuse) {
+        st.klo = tau_in + 1; st.khi = tau_in + 1; st.need = 0; st.cnt = 0;
+        const float xs = part ? sqrtf(cull_bound(r0sq)) * 1.00002f + dq : 0.0f;
+        const float rs = wave_max(xs) * 1.0002f;
+        nr = build_list(D, cx, cy, cz, rs * rs + 1e-30f, sh, lane, &total,
+                        count_stats ? &counters[kStatOverflow] : nullptr);
+        pt.mark(2, lane);
+      } else if (all) {
+        // every photon is a neighbour; pass B tracks the largest key
+        st.klo = kKeyMax; st.khi = kKeyMax; st.need = 0; st.cnt = 0;
+        nr = build_list(D, cx, cy, cz, INFINITY, sh, lane, &total, nullptr);
+        pt.mark(2, lane);
+      } else {
+        nr = run_select(D, qx, qy, qz, part, r0sq, cx, cy, cz, dq, dmax, k_eff, sh, lane,
+                        counters, count_stats, st, pt, &total);
       }
-    }
-    // gradient reaching the estimate: scaled by the primary albedo of a
-    // final-gathering bounce
-    const float ge_r = g_r * fg_r, ge_g = g_g * fg_g, ge_b = g_b * fg_b;
-    const float c_r = ((rho_r / kPiF) * ge_r) / denom;
-    const float c_g = ((rho_g / kPiF) * ge_g) / denom;
-    const float c_b = ((rho_b / kPiF) * ge_b) / denom;
-    const bool contributes = active && lambert && k_eff > 0;
-    const uint64_t klim = tau + 1;
-    float s_r = 0.f, s_g = 0.f, s_b = 0.f;
-    const int32_t n = D.n;
-    float* __restrict__ dflux = D.dl_dflux;
-    clustered(D, qx, qy, qz, contributes ? r2 : -1.0f, sh, lane, counters, 0, kStatCand2, pt,
-              4, 5, [&](bool part, uint32_t nr, uint32_t total) {
-      for_candidates<true>(D, sh, lane, nr, total, [&](uint32_t cnt) {
-        for (uint32_t j = 0; j < cnt; ++j) {
-          const float4 p = sh.u.c.pos[j];
-          const float4 w = sh.u.c.wif[j];
-          const float2 f = sh.u.c.flux[j];
-          const float d2 = d2_ref(qx, qy, qz, p.x, p.y, p.z);
-          const uint64_t key = make_key(d2, __float_as_uint(p.w));
-          const float cos_i = dot_ref(w.x, w.y, w.z, nx, ny, nz);
-          const bool use = part && key < klim && !(cos_o < 0.0f || cos_i < 0.0f);
-          if (wave_any(use)) {
-            const float sel = use ? 1.0f : 0.0f;
-            s_r = fmaf(sel, w.w, s_r);
-            s_g = fmaf(sel, f.x, s_g);
-            s_b = fmaf(sel, f.y, s_b);
-            const float tr = wave_sum(use ? c_r : 0.0f);
-            const float tg = wave_sum(use ? c_g : 0.0f);
-            const float tb = wave_sum(use ? c_b : 0.0f);
-            if (lane == 0) {
+
+      // ---- pass B: consume neighbours below klo, stash [klo, khi) ---------------
+      // (the adjoint runs it only to finish the selection; its scatter pass
+      // follows once tau is known)
+      if (!reuse) {
+        const bool acc_lane = part && lane_lambert;
+        auto pass_b = [&](auto kind_tag) {
+          constexpr int kind = decltype(kind_tag)::value;  // 0 all, 1 keys, 2 d^2 bits
+          const uint64_t span = st.khi - st.klo;
+          const uint32_t lo32 = static_cast<uint32_t>(st.klo >> 32);
+          const uint32_t span32 = static_cast<uint32_t>(st.khi >> 32) - lo32;
+          for_candidates<kAccum>(D, sh, lane, nr, total, [&](uint32_t cnt) {
+            const uint64_t m = consume<kAccum>(sh, cnt, [&](const Cand& c) {
+              const float d2 = d2_ref(qx, qy, qz, c.p.x, c.p.y, c.p.z);
+              bool nb, flag;
+              if constexpr (kind == 0) {
+                nb = true;
+                const uint64_t key = make_key(d2, __float_as_uint(c.p.w));
+                key_max = (part && key > key_max) ? key : key_max;
+                flag = kRows && part;
+              } else if constexpr (kind == 1) {
+                const uint64_t key = make_key(d2, __float_as_uint(c.p.w));
+                nb = key < st.klo;
+                flag = part && ((key - st.klo) < span || (kRows && nb));
+              } else {
+                const uint32_t db = __float_as_uint(d2);
+                nb = db < lo32;
+                flag = part && ((db - lo32) < span32 || (kRows && nb));
+              }
+              if (kAccum) {
+                const float cos_i = dot_ref(c.w.x, c.w.y, c.w.z, nx, ny, nz);
+                const float sel = (acc_lane && nb && !(cos_i < 0.0f)) ? 1.0f : 0.0f;
+                acc_r = fmaf(sel, c.w.w, acc_r);
+                acc_g = fmaf(sel, c.f.x, acc_g);
+                acc_b = fmaf(sel, c.f.y, acc_b);
+              }
+              return flag;
+            });
+            drain(m, [&](uint32_t j) {
+              const float4 p = sh.u.c.pos[j];
+              const float d2 = d2_ref(qx, qy, qz, p.x, p.y, p.z);
               const uint32_t idx = __float_as_uint(p.w);
-              atomicAdd(&dflux[idx], tr);
-              atomicAdd(&dflux[static_cast<int64_t>(n) + idx], tg);
-              atomicAdd(&dflux[2 * static_cast<int64_t>(n) + idx], tb);
+              const uint64_t key = make_key(d2, idx);
+              if (kRows && key < st.klo) {
+                if (n_written < static_cast<uint32_t>(D.k))
+                  row[n_written++] = static_cast<int32_t>(idx);
+              } else {
+                stash.put(key, kAccum ? sh.u.c.slot[j] : 0u);
+              }
+            });
+          });
+        };
+        if (all) {
+          pass_b(Kind<0>{});
+        } else if (wave_any(part && ((st.klo | st.khi) & 0xffffffffull) != 0ull)) {
+          pass_b(Kind<1>{});
+        } else {
+          pass_b(Kind<2>{});
+        }
+        pt.mark(5, lane);
+      }
+
+      // ---- finish the set inside the stash -----------------------------------
+      // (wave-wide steps run with every lane active; only part lanes hold
+      // stash entries)
+      if (part) tau = reuse ? tau_in : (all ? key_max : tau);
+      if (!reuse && !all) {
+        // rank of each stashed key among the stash; the need smallest are
+        // neighbours, the need-th is tau (unused entries hold kKeyMax)
+        uint32_t members = 0;
+        const uint32_t n_max = static_cast<uint32_t>(uni(static_cast<int>(
+            wave_max(part ? static_cast<float>(stash.n) : 0.0f))));
+#pragma unroll 1
+        for (uint32_t i = 0; i < n_max; ++i) {
+          uint64_t ki = stash.key[0];
+#pragma unroll
+          for (int e = 1; e < kStashCap; ++e) ki = (i == static_cast<uint32_t>(e)) ? stash.key[e] : ki;
+          uint32_t rank = 0;
+#pragma unroll
+          for (int j = 0; j < kStashCap; ++j) rank += stash.key[j] < ki ? 1u : 0u;
+          const bool valid = part && i < stash.n;
+          if (valid && rank + 1 == st.need) tau = ki;
+          if (valid && rank < st.need) members |= 1u << i;
+        }
+        if (kAccum) {
+          // members' photons, two per round so their loads overlap
+          auto slot_of = [&](uint32_t i) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int e = 0; e < kStashCap; ++e) v = (i == static_cast<uint32_t>(e)) ? stash.slot[e] : v;
+            return v;
+          };
+          uint32_t mm = members;
+          while (wave_any(mm != 0u)) {
+            const uint32_t i0 = mm ? static_cast<uint32_t>(__builtin_ctz(mm)) : 32u;
+            mm &= mm - 1u;
+            const uint32_t i1 = mm ? static_cast<uint32_t>(__builtin_ctz(mm)) : 32u;
+            mm &= mm - 1u;
+            const bool v0 = i0 < 32u, v1 = i1 < 32u;
+            const int32_t s0 = static_cast<int32_t>(slot_of(i0));
+            const int32_t s1 = static_cast<int32_t>(slot_of(i1));
+            float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f), w1 = w0;
+            float2 f0 = make_float2(0.f, 0.f), f1 = f0;
+            if (v0) { w0 = D.wif4[s0]; f0 = D.flux2[s0]; }
+            if (v1) { w1 = D.wif4[s1]; f1 = D.flux2[s1]; }
+            const float c0s = dot_ref(w0.x, w0.y, w0.z, nx, ny, nz);
+            const float c1s = dot_ref(w1.x, w1.y, w1.z, nx, ny, nz);
+            const float sel0 = (v0 && lane_lambert && !(c0s < 0.0f)) ? 1.0f : 0.0f;
+            const float sel1 = (v1 && lane_lambert && !(c1s < 0.0f)) ? 1.0f : 0.0f;
+            acc_r = fmaf(sel0, w0.w, acc_r);
+            acc_g = fmaf(sel0, f0.x, acc_g);
+            acc_b = fmaf(sel0, f0.y, acc_b);
+            acc_r = fmaf(sel1, w1.w, acc_r);
+            acc_g = fmaf(sel1, f1.x, acc_g);
+            acc_b = fmaf(sel1, f1.y, acc_b);
+          }
+        } else if (kRows) {
+#pragma unroll
+          for (int i = 0; i < kStashCap; ++i) {
+            if (((members >> i) & 1u) && n_written < static_cast<uint32_t>(D.k))
+              row[n_written++] = static_cast<int32_t>(stash.key[i] & 0xffffffffull);
+          }
+        }
+      }
+
+      if constexpr (MODE == kModeAdjoint) {
+        // ---- scatter pass: dL/dflux over the neighbour set {key <= tau} -------------
+        const float r2 = __uint_as_float(static_cast<uint32_t>(tau >> 32));
+        const float denom = (D.n_emit * kPiF) * r2;
+        const bool contributes = part && lambert;
+        float g_r = 0.f, g_g = 0.f, g_b = 0.f;
+        float fg_r = 1.f, fg_g = 1.f, fg_b = 1.f;
+        int32_t fgm = -1;
+        if (contributes) {
+          g_r = D.dl_drgb[3 * qi + 0];
+          g_g = D.dl_drgb[3 * qi + 1];
+          g_b = D.dl_drgb[3 * qi + 2];
+          if (D.fg_mat) {
+            fgm = D.fg_mat[qi];
+            if (fgm >= 0 && fgm < D.n_mats) {
+              const pm_material fm = D.mats[fgm];
+              fg_r = fm.rho[0]; fg_g = fm.rho[1]; fg_b = fm.rho[2];
+            } else {
+              fgm = -1;
             }
           }
         }
-      });
-    });
-    // dL/drho of the queried material, and of the final-gathering albedo
-    if (contributes) {
-      const float inv = 1.0f / (kPiF * denom);
-      const float dr = ge_r * s_r * inv, dg = ge_g * s_g * inv, db = ge_b * s_b * inv;
-      if (mat < kDrhoSlots) {
-        atomicAdd(&sh.drho[3 * mat + 0], dr);
-        atomicAdd(&sh.drho[3 * mat + 1], dg);
-        atomicAdd(&sh.drho[3 * mat + 2], db);
-      } else {
-        atomicAdd(&D.dl_dalbedo[3 * mat + 0], dr);
-        atomicAdd(&D.dl_dalbedo[3 * mat + 1], dg);
-        atomicAdd(&D.dl_dalbedo[3 * mat + 2], db);
-      }
-      if (fgm >= 0) {
-        // forward value of the estimate L = (rho/pi) * S / denom
-        const float lr = ((rho_r / kPiF) * s_r) / denom;
-        const float lgv = ((rho_g / kPiF) * s_g) / denom;
-        const float lb = ((rho_b / kPiF) * s_b) / denom;
-        if (fgm < kDrhoSlots) {
-          atomicAdd(&sh.drho[3 * fgm + 0], g_r * lr);
-          atomicAdd(&sh.drho[3 * fgm + 1], g_g * lgv);
-          atomicAdd(&sh.drho[3 * fgm + 2], g_b * lb);
-        } else {
-          atomicAdd(&D.dl_dalbedo[3 * fgm + 0], g_r * lr);
-          atomicAdd(&D.dl_dalbedo[3 * fgm + 1], g_g * lgv);
-          atomicAdd(&D.dl_dalbedo[3 * fgm + 2], g_b * lb);
+        // gradient reaching the estimate: scaled by the primary albedo of a
+        // final-gathering bounce
+        const float ge_r = g_r * fg_r, ge_g = g_g * fg_g, ge_b = g_b * fg_b;
+        const float c_r = ((rho_r / kPiF) * ge_r) / denom;
+        const float c_g = ((rho_g / kPiF) * ge_g) / denom;
+        const float c_b = ((rho_b / kPiF) * ge_b) / denom;
+        const uint64_t klim = tau + 1;
+        float s_r = 0.f, s_g = 0.f, s_b = 0.f;
+        const int32_t n = D.n;
+        float* __restrict__ dflux = D.dl_dflux;
+        for_candidates<true>(D, sh, lane, nr, total, [&](uint32_t cnt) {
+          for (uint32_t j = 0; j < cnt; ++j) {
+            const float4 p = sh.u.c.pos[j];
+            const float4 w = sh.u.c.wif[j];
+            const float2 f = sh.u.c.flux[j];
+            const float d2 = d2_ref(qx, qy, qz, p.x, p.y, p.z);
+            const uint64_t key = make_key(d2, __float_as_uint(p.w));
+            const float cos_i = dot_ref(w.x, w.y, w.z, nx, ny, nz);
+            const bool use = contributes && key < klim && !(cos_o < 0.0f || cos_i < 0.0f);
+            if (wave_any(use)) {
+              const float sel = use ? 1.0f : 0.0f;
+              s_r = fmaf(sel, w.w, s_r);
+              s_g = fmaf(sel, f.x, s_g);
+              s_b = fmaf(sel, f.y, s_b);
+              const float tr = wave_sum(use ? c_r : 0.0f);
+              const float tg = wave_sum(use ? c_g : 0.0f);
+              const float tb = wave_sum(use ? c_b : 0.0f);
+              if (lane == 0) {
+                const uint32_t idx = __float_as_uint(p.w);
+                atomicAdd(&dflux[idx], tr);
+                atomicAdd(&dflux[static_cast<int64_t>(n) + idx], tg);
+                atomicAdd(&dflux[2 * static_cast<int64_t>(n) + idx], tb);
+              }
+            }
+          }
+        });
+        // dL/drho of the queried material, and of the final-gathering albedo
+        if (contributes) {
+          const float inv = 1.0f / (kPiF * denom);
+          const float dr = ge_r * s_r * inv, dg = ge_g * s_g * inv, db = ge_b * s_b * inv;
+          if (mat < kDrhoSlots) {
+            atomicAdd(&sh.drho[3 * mat + 0], dr);
+            atomicAdd(&sh.drho[3 * mat + 1], dg);
+            atomicAdd(&sh.drho[3 * mat + 2], db);
+          } else {
+            atomicAdd(&D.dl_dalbedo[3 * mat + 0], dr);
+            atomicAdd(&D.dl_dalbedo[3 * mat + 1], dg);
+            atomicAdd(&D.dl_dalbedo[3 * mat + 2], db);
+          }
+          if (fgm >= 0) {
+            // forward value of the estimate L = (rho/pi) * S / denom
+            const float lr = ((rho_r / kPiF) * s_r) / denom;
+            const float lgv = ((rho_g / kPiF) * s_g) / denom;
+            const float lb = ((rho_b / kPiF) * s_b) / denom;
+            if (fgm < kDrhoSlots) {
+              atomicAdd(&sh.drho[3 * fgm + 0], g_r * lr);
+              atomicAdd(&sh.drho[3 * fgm + 1], g_g * lgv);
+              atomicAdd(&sh.drho[3 * fgm + 2], g_b * lb);
+            } else {
+              atomicAdd(&D.dl_dalbedo[3 * fgm + 0], g_r * lr);
+              atomicAdd(&D.dl_dalbedo[3 * fgm + 1], g_g * lgv);
+              atomicAdd(&D.dl_dalbedo[3 * fgm + 2], g_b * lb);
+            }
+          }
         }
       }
+      pt.mark(6, lane);
+      c0 = c1;
     }
-    return;
   }
 
-  if (!active) return;
+  if (MODE == kModeAdjoint || !active) return;
+  const float r2 = (k_eff > 0) ? __uint_as_float(static_cast<uint32_t>(tau >> 32)) : 0.0f;
   if (MODE == kModeAnchor) {
     D.anchor_out[4 * qi + 3] = sqrtf(r2) * 1.00002f + 1e-30f;
     return;
@@ -942,8 +1244,7 @@ __global__ __launch_bounds__(kWave) void gather_kernel(GatherLaunch L) {
       const int inst = (L.n_inst > 1) ? find_instance(gprefix, L.n_inst, g) : 0;
       DescRef D = descs[inst];
       process_group<MODE>(D, g - gprefix[inst], L.perm, sh, lane, L.counters,
-                          L.count_stats, pt);
-      pt.mark(6, lane);
+                          L.count_stats, L.run_split, pt);
     }
   }
   if (MODE == kModeAdjoint) {

@@ -14,7 +14,7 @@ namespace pm {
 // ---- tuning constants ------------------------------------------------------
 constexpr int kWave = 64;           // CDNA wavefront width
 constexpr int kBins = 32;           // selection histogram bins per query
-constexpr int kStashCap = 16;       // per-query candidates kept past the cut
+constexpr int kStashCap = 12;       // per-query candidates kept past the cut
 constexpr int kMortonBits = 10;     // per axis
 constexpr int kAnchorWindow = 4;    // anchors tried on each side of a group
 constexpr int kMaxKnnK = 1024;      // row-sort limit of pm_knn
@@ -25,9 +25,10 @@ constexpr int kXcds = 8;            // MI355X accelerator complex dies
 constexpr int kStatBase = 2 * kXcds;
 constexpr int kStatFallback = kStatBase + 0;   // groups re-run unbounded
 constexpr int kStatRefine = kStatBase + 1;     // histogram refinement rounds
-constexpr int kStatCand1 = kStatBase + 2;      // pass-1 candidate photons
-constexpr int kStatCand2 = kStatBase + 3;      // pass-2 candidate photons
+constexpr int kStatCand1 = kStatBase + 2;      // centre-bound candidate photons
+constexpr int kStatCand2 = kStatBase + 3;      // per-lane pass candidate photons
 constexpr int kStatOverflow = kStatBase + 4;   // candidate lists cut short
+constexpr int kStatRuns = kStatBase + 5;       // runs of nearby lanes
 // Wave cycles per phase of a group (uint64 each, from word kPhaseBase).
 constexpr int kPhaseBase = 32;
 constexpr int kPhases = 8;
@@ -123,6 +124,8 @@ struct GatherLaunch {
   int32_t xcd_aware;
   int32_t count_stats;
   int32_t mode;
+  float run_split;             // run cut threshold / smallest lane radius bound
+  int32_t pad_;
 };
 
 // ---- host-side photon map --------------------------------------------------------
@@ -132,7 +135,7 @@ struct PhotonMapImpl {
   Context* ctx = nullptr;
   int64_t n = 0;
   bool built = false;
-  int leaf_size = 32;
+  int leaf_size = 8;
   int n_leaves = 0;
   int first_leaf = 1;
   int anchor_stride = 8;
@@ -152,8 +155,9 @@ struct Context {
   int n_cu = 256;
   int waves_per_cu = 8;
   int anchor_stride = 8;
-  int leaf_size = 32;
+  int leaf_size = 8;
   int xcd_aware = 1;
+  float run_split = 1.0f;
   int timing = 0;
   int count_stats = 0;
   pm_stats stats{};

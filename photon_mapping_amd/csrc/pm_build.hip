@@ -358,7 +358,7 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
   Context* ctx = map->ctx;
   const int64_t n = map->n;
   map->built = false;
-  map->anchor_k = -1;
+  for (auto& a : map->aset) a.k = -1;
   if (n == 0) {
     map->n_leaves = 0;
     map->first_leaf = 1;
@@ -463,10 +463,24 @@ int update_flux(PhotonMapImpl* map, const float* flux_dev, hipStream_t s) {
   return PM_OK;
 }
 
-int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s) {
+int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** out) {
+  *out = map->anchors.as<float4>();
   if (map->n == 0 || k_eff >= map->n) return PM_OK;  // every photon is a neighbour
-  if (map->anchor_k == k_eff) return PM_OK;
+  PhotonMapImpl::AnchorSet* slot = &map->aset[0];
+  for (auto& a : map->aset) {
+    if (a.k == k_eff) {
+      a.stamp = ++map->anchor_clock;
+      *out = a.buf.as<float4>();
+      return PM_OK;
+    }
+    if (a.k < 0 || a.stamp < slot->stamp) slot = &a;
+    if (slot->k < 0) break;
+  }
   Context* ctx = map->ctx;
+  const size_t bytes = static_cast<size_t>(map->n_anchors) * sizeof(float4);
+  slot->k = -1;
+  if (int rc = slot->buf.reserve(bytes)) return rc;
+  PM_HIP(hipMemcpyAsync(slot->buf.ptr, map->anchors.ptr, bytes, hipMemcpyDeviceToDevice, s));
   GatherDesc d{};
   fill_desc_map_part(map, &d);
   d.m = map->n_anchors;
@@ -476,7 +490,7 @@ int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s) {
   d.k = k_eff;
   d.k_eff = k_eff;
   d.all_members = 0;
-  d.anchor_out = reinterpret_cast<float*>(map->anchors.ptr);
+  d.anchor_out = slot->buf.as<float>();
   if (int rc = ctx->ws_counters.reserve(64 * sizeof(uint32_t))) return rc;
   int64_t gp[2] = {0, d.n_groups};
   if (int rc = stage_begin(ctx, sizeof(GatherDesc) + sizeof(gp) + 256, s)) return rc;
@@ -494,7 +508,9 @@ int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s) {
   L.run_split = ctx->run_split;
   L.mode = kModeAnchor;
   if (int rc = launch_gather(ctx, L, s)) return rc;
-  map->anchor_k = k_eff;
+  slot->k = k_eff;
+  slot->stamp = ++map->anchor_clock;
+  *out = slot->buf.as<float4>();
   return PM_OK;
 }
 

@@ -126,31 +126,45 @@ struct CallTimer {
   }
 };
 
-// Shared driver of pm_knn / pm_gather_radiance / pm_gather_adjoint on one map.
-int run_single(PhotonMapImpl* map, GatherDesc d, int mode, hipStream_t s,
-               int32_t* knn_out_idx) {
-  Context* ctx = map->ctx;
-  const int64_t m = d.m;
-  d.q_base = 0;
-  d.group_base = 0;
-  d.n_groups = ceil_div(m, kWave);
+// Shared driver of every gather: the instances' descriptors (map part and
+// query/output part filled, sizes in d.m) are laid out in query and group
+// order, staged, the queries of all instances Morton-sorted together, and
+// one persistent launch covers every group.  knn_out_idx (single kNN
+// instance only) receives the sorted rows.
+int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream_t s,
+                  int32_t* knn_out_idx) {
+  const int n_inst = static_cast<int>(ds.size());
+  std::vector<int64_t> qp(n_inst + 1, 0), gp(n_inst + 1, 0);
+  for (int i = 0; i < n_inst; ++i) {
+    GatherDesc& d = ds[i];
+    d.q_base = qp[i];
+    d.group_base = gp[i];
+    d.n_groups = ceil_div(d.m, kWave);
+    qp[i + 1] = qp[i] + d.m;
+    gp[i + 1] = gp[i] + d.n_groups;
+  }
+  const int64_t total_q = qp[n_inst];
+  const int64_t total_groups = gp[n_inst];
+  if (total_q > (int64_t{1} << 32) - 1)
+    return fail(PM_ERR_UNSUPPORTED, "more than 2^32 - 1 queries in one call");
   if (int rc = ctx->ws_counters.reserve(64 * sizeof(uint32_t))) return rc;
-  if (int rc = ctx->ws_group_anchor.reserve(std::max<int64_t>(d.n_groups, 1) * sizeof(uint32_t)))
+  if (int rc = ctx->ws_group_anchor.reserve(std::max<int64_t>(total_groups, 1) * sizeof(uint32_t)))
     return rc;
-  d.group_anchor = ctx->ws_group_anchor.as<uint32_t>();
-  const int64_t qp[2] = {0, m};
-  const int64_t gp[2] = {0, d.n_groups};
-  if (int rc = stage_begin(ctx, sizeof(GatherDesc) + sizeof(qp) + sizeof(gp) + 1024, s))
-    return rc;
-  const GatherDesc* dd = static_cast<const GatherDesc*>(stage_put(ctx, &d, sizeof(d)));
-  const int64_t* dqp = static_cast<const int64_t*>(stage_put(ctx, qp, sizeof(qp)));
-  const int64_t* dgp = static_cast<const int64_t*>(stage_put(ctx, gp, sizeof(gp)));
+  for (int i = 0; i < n_inst; ++i)
+    ds[i].group_anchor = ctx->ws_group_anchor.as<uint32_t>() + ds[i].group_base;
+  const size_t db = ds.size() * sizeof(GatherDesc);
+  const size_t pb = qp.size() * sizeof(int64_t);
+  if (int rc = stage_begin(ctx, db + 2 * pb + 1024, s)) return rc;
+  const GatherDesc* dd = static_cast<const GatherDesc*>(stage_put(ctx, ds.data(), db));
+  const int64_t* dqp = static_cast<const int64_t*>(stage_put(ctx, qp.data(), pb));
+  const int64_t* dgp = static_cast<const int64_t*>(stage_put(ctx, gp.data(), pb));
   if (int rc = stage_commit(ctx, s)) return rc;
 
   CallTimer tm(ctx, s);
   tm.mark(0);
   const uint32_t* perm = nullptr;
-  if (int rc = prepare_queries(ctx, 1, dd, dqp, dgp, m, d.n_groups, s, &perm)) return rc;
+  if (int rc = prepare_queries(ctx, n_inst, dd, dqp, dgp, total_q, total_groups, s, &perm))
+    return rc;
   tm.mark(1);
   GatherLaunch L{};
   L.descs = dd;
@@ -158,8 +172,8 @@ int run_single(PhotonMapImpl* map, GatherDesc d, int mode, hipStream_t s,
   L.perm = perm;
   L.q_prefix = dqp;
   L.counters = ctx->ws_counters.as<uint32_t>();
-  L.total_groups = d.n_groups;
-  L.n_inst = 1;
+  L.total_groups = total_groups;
+  L.n_inst = n_inst;
   L.xcd_aware = ctx->xcd_aware;
   L.count_stats = ctx->count_stats;
   L.run_split = ctx->run_split;
@@ -167,10 +181,11 @@ int run_single(PhotonMapImpl* map, GatherDesc d, int mode, hipStream_t s,
   if (int rc = launch_gather(ctx, L, s)) return rc;
   tm.mark(2);
   if (mode == kModeKnn) {
-    if (int rc = launch_knn_rowsort(dd, m, knn_out_idx, s)) return rc;
+    if (n_inst != 1) return fail(PM_ERR_UNSUPPORTED, "kNN rows of one instance only");
+    if (int rc = launch_knn_rowsort(dd, total_q, knn_out_idx, s)) return rc;
   }
   tm.mark(3);
-  ctx->stats.groups = d.n_groups;
+  ctx->stats.groups = total_groups;
   if (tm.on || ctx->count_stats) {
     PM_HIP(hipStreamSynchronize(s));
     if (tm.on) {
@@ -195,6 +210,12 @@ int run_single(PhotonMapImpl* map, GatherDesc d, int mode, hipStream_t s,
     }
   }
   return PM_OK;
+}
+
+int run_single(PhotonMapImpl* map, GatherDesc d, int mode, hipStream_t s,
+               int32_t* knn_out_idx) {
+  std::vector<GatherDesc> ds(1, d);
+  return run_instances(map->ctx, ds, mode, s, knn_out_idx);
 }
 
 int check_map(pm_photonmap* h) {
@@ -463,9 +484,11 @@ int pm_knn(pm_photonmap* h, int64_t m, const float* q_soa, int k, int32_t* out_i
       if (int rc = launch_fill_f32(dr2, 0.0f, m, s)) return rc;
   } else {
     const int k_eff = static_cast<int>(std::min<int64_t>(k, n));
-    if (int rc = ensure_anchors(map, k_eff, s)) return rc;
+    const float4* anchors = nullptr;
+    if (int rc = ensure_anchors(map, k_eff, s, &anchors)) return rc;
     GatherDesc d{};
     fill_desc_map_part(map, &d);
+    d.anchors = anchors;
     d.q_soa = dq;
     d.out_slots = didx;
     d.out_r2 = dr2;
@@ -530,9 +553,11 @@ int pm_gather_radiance(pm_photonmap* h, int64_t m, const pm_query* q, const pm_m
       if (int rc = launch_fill_u32(reinterpret_cast<uint32_t*>(dkey), 0u, 2 * m, s)) return rc;
   } else {
     const int k_eff = static_cast<int>(std::min<int64_t>(k, n));
-    if (int rc = ensure_anchors(map, k_eff, s)) return rc;
+    const float4* anchors = nullptr;
+    if (int rc = ensure_anchors(map, k_eff, s, &anchors)) return rc;
     GatherDesc d{};
     fill_desc_map_part(map, &d);
+    d.anchors = anchors;
     d.q = dq;
     d.mats = dm;
     d.n_mats = n_mats;
@@ -612,9 +637,11 @@ int pm_gather_adjoint(pm_photonmap* h, int64_t m, const pm_query* q, const pm_ma
     PM_HIP(hipGetLastError());
   }
   const int k_eff = static_cast<int>(std::min<int64_t>(k, n));
-  if (int rc = ensure_anchors(map, k_eff, s)) return rc;
+  const float4* anchors = nullptr;
+  if (int rc = ensure_anchors(map, k_eff, s, &anchors)) return rc;
   GatherDesc d{};
   fill_desc_map_part(map, &d);
+  d.anchors = anchors;
   d.q = dq;
   d.mats = dm;
   d.n_mats = n_mats;
@@ -639,8 +666,65 @@ int pm_gather_adjoint(pm_photonmap* h, int64_t m, const pm_query* q, const pm_ma
   return PM_OK;
 }
 
-int pm_gather_batched(pm_context*, int, const pm_instance*, void*) {
-  return fail(PM_ERR_UNSUPPORTED, "pm_gather_batched is not available in this build");
+int pm_gather_batched(pm_context* h, int n_inst, const pm_instance* inst, void* stream) {
+  if (!h) return fail(PM_ERR_INVALID_ARG, "null context");
+  Context* ctx = reinterpret_cast<Context*>(h);
+  if (n_inst < 0 || (n_inst > 0 && !inst)) return fail(PM_ERR_INVALID_ARG, "bad instance list");
+  if (n_inst == 0) return PM_OK;
+  if (int rc = set_device(ctx)) return rc;
+  hipStream_t s = as_stream(stream);
+  // validate every instance before anything is launched
+  for (int i = 0; i < n_inst; ++i) {
+    const pm_instance& in = inst[i];
+    if (int rc = check_map(in.map)) return rc;
+    if (reinterpret_cast<PhotonMapImpl*>(in.map)->ctx != ctx)
+      return fail(PM_ERR_INVALID_ARG, "instance map belongs to another context");
+    if (in.k < 1) return fail(PM_ERR_INVALID_ARG, "k must be >= 1");
+    if (in.m < 0) return fail(PM_ERR_INVALID_ARG, "negative query count");
+    if (in.m > (int64_t{1} << 31) - 1) return fail(PM_ERR_UNSUPPORTED, "m must be < 2^31");
+    if (in.m > 0 && (!in.q || !in.out_rgb))
+      return fail(PM_ERR_INVALID_ARG, "null query or output");
+    if (in.n_mats < 0 || (in.n_mats > 0 && !in.mats))
+      return fail(PM_ERR_INVALID_ARG, "bad materials");
+  }
+  std::vector<GatherDesc> ds;
+  ds.reserve(n_inst);
+  for (int i = 0; i < n_inst; ++i) {
+    const pm_instance& in = inst[i];
+    auto* map = reinterpret_cast<PhotonMapImpl*>(in.map);
+    if (in.m == 0) continue;
+    if (map->n == 0) {
+      if (int rc = launch_fill_f32(in.out_rgb, 0.0f, 3 * in.m, s)) return rc;
+      if (in.out_r2)
+        if (int rc = launch_fill_f32(in.out_r2, 0.0f, in.m, s)) return rc;
+      if (in.out_key)
+        if (int rc = launch_fill_u32(reinterpret_cast<uint32_t*>(in.out_key), 0u, 2 * in.m, s))
+          return rc;
+      continue;
+    }
+    // the same anchor set as a single call with this k, so that every
+    // instance is computed exactly as pm_gather_radiance computes it
+    const int k_eff = static_cast<int>(std::min<int64_t>(in.k, map->n));
+    const float4* anchors = nullptr;
+    if (int rc = ensure_anchors(map, k_eff, s, &anchors)) return rc;
+    GatherDesc d{};
+    fill_desc_map_part(map, &d);
+    d.anchors = anchors;
+    d.q = in.q;
+    d.mats = in.mats;
+    d.n_mats = in.n_mats;
+    d.out_rgb = in.out_rgb;
+    d.out_r2 = in.out_r2;
+    d.out_key = in.out_key;
+    d.m = in.m;
+    d.k = in.k;
+    d.k_eff = k_eff;
+    d.n_emit = static_cast<float>(in.n_emit);
+    d.all_members = d.k_eff >= map->n ? 1 : 0;
+    ds.push_back(d);
+  }
+  if (ds.empty()) return PM_OK;
+  return run_instances(ctx, ds, kModeRadiance, s, nullptr);
 }
 
 }  // extern "C"

@@ -67,6 +67,10 @@ constexpr int kMaxRanges = 256;   // candidate subtrees per list
 constexpr int kFront = 256;       // breadth-first frontier capacity
 constexpr int kDrhoSlots = 64;    // adjoint: materials accumulated in LDS
 constexpr int kCentreBins = 256;  // centre histogram bins
+constexpr int kSlotCap = 512;     // photons of an explicit slot list
+// A candidate list is either nr subtrees (sh.rnode / sh.rpre) or, when nr
+// is this value, an explicit list of photon slots (sh.cslot).
+constexpr uint32_t kSlotList = 0xffffffffu;
 
 template <typename T>
 __device__ __forceinline__ const PM_CONST T* cptr(const T* p) {
@@ -97,6 +101,7 @@ struct GatherShared {
   } u;
   uint32_t rnode[kMaxRanges];
   uint32_t rpre[kMaxRanges + 1];
+  uint32_t cslot[kSlotCap];
   float drho[3 * kDrhoSlots];
 };
 static_assert(kCentreBins <= (kBins + 2) * kWave, "centre bins alias the lane bins");
@@ -413,9 +418,11 @@ __device__ uint32_t filter_list(DescRef D, GatherShared& sh, int lane, uint32_t 
 }
 
 // Slot of the t-th candidate of the current list; r is the lane's cursor
-// into the list and only moves forward (t must not decrease between calls).
+// into a subtree list and only moves forward (t must not decrease between
+// calls).
 __device__ __forceinline__ int32_t list_slot(DescRef D, const GatherShared& sh, uint32_t nr,
                                              uint32_t t, uint32_t& r) {
+  if (nr == kSlotList) return static_cast<int32_t>(sh.cslot[t]);
   while (r + 1 < nr && sh.rpre[r + 1] <= t) ++r;
   int32_t s0, s1;
   node_slots(D, sh.rnode[r], s0, s1);
@@ -471,7 +478,9 @@ __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int 
 }
 
 // Lane-parallel sweep of the current list: every lane takes every 64th
-// candidate, four loads in flight at a time, and calls fn(position).
+// candidate, four loads in flight at a time, and calls fn(u, ok, position,
+// slot) for its four candidates of a step with every lane active (ok tells
+// whether the lane holds a candidate).
 template <typename Fn>
 __device__ __forceinline__ void sweep_lanes(DescRef D, const GatherShared& sh, int lane,
                                             uint32_t nr, uint32_t total, Fn&& fn) {
@@ -479,18 +488,42 @@ __device__ __forceinline__ void sweep_lanes(DescRef D, const GatherShared& sh, i
   uint32_t r = 0;
   for (uint32_t base = 0; base < total; base += 4 * kWave) {
     float4 p[4];
+    int32_t slot[4];
     bool ok[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t t = base + static_cast<uint32_t>(u * kWave + lane);
       ok[u] = t < total;
       p[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (ok[u]) p[u] = P[list_slot(D, sh, nr, t, r)];
+      slot[u] = 0;
+      if (ok[u]) {
+        slot[u] = list_slot(D, sh, nr, t, r);
+        p[u] = P[slot[u]];
+      }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (ok[u]) fn(p[u]);
+    for (int u = 0; u < 4; ++u) fn(u, ok[u], p[u], slot[u]);
   }
+}
+
+// Explicit slot list of the photons of the current subtree list whose FP32
+// d^2 from c is <= lim2, written to sh.cslot.  Returns the count, or
+// kSlotList when more than kSlotCap photons qualify (sh.cslot then holds a
+// prefix only and must not be used).
+__device__ uint32_t collect_slots(DescRef D, GatherShared& sh, int lane, uint32_t nr,
+                                  uint32_t total, float cx, float cy, float cz, float lim2) {
+  uint32_t n = 0;
+  bool over = false;
+  sweep_lanes(D, sh, lane, nr, total, [&](int, bool ok, const float4& p, int32_t slot) {
+    const bool keep = ok && d2_ref(cx, cy, cz, p.x, p.y, p.z) <= lim2;
+    const uint64_t km = __ballot(keep);
+    const uint32_t at = n + lanes_below(km, lane);
+    if (keep && at < static_cast<uint32_t>(kSlotCap)) sh.cslot[at] = static_cast<uint32_t>(slot);
+    n += static_cast<uint32_t>(__popcll(km));
+    over = over || n > static_cast<uint32_t>(kSlotCap);
+  });
+  __syncthreads();
+  return over ? kSlotList : static_cast<uint32_t>(uni(static_cast<int>(n)));
 }
 
 // A staged candidate as the consume loops see it.
@@ -580,9 +613,9 @@ __device__ bool centre_bound(DescRef D, GatherShared& sh, int lane, uint32_t nr,
   for (int i = lane; i < kCentreBins; i += kWave) ch[i] = 0u;
   __syncthreads();
   const float inv_w = static_cast<float>(kCentreBins) / rb;
-  sweep_lanes(D, sh, lane, nr, total, [&](const float4& p) {
+  sweep_lanes(D, sh, lane, nr, total, [&](int, bool ok, const float4& p, int32_t) {
     const float t = d2_ref(cx, cy, cz, p.x, p.y, p.z) * inv_w;
-    if (t < static_cast<float>(kCentreBins)) lds_inc(&ch[static_cast<uint32_t>(t)]);
+    if (ok && t < static_cast<float>(kCentreBins)) lds_inc(&ch[static_cast<uint32_t>(t)]);
   });
   __syncthreads();
   static_assert(kCentreBins == 4 * kWave, "four centre bins per lane");
@@ -661,7 +694,7 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
     // The list of the centre sweep also covers the run's final sphere
     // (r_c <= r0c up to the margins), which is then a filter of it.
     const float rl = (r0c * 1.0001f + 2.0f * dmax) * 1.0004f + 2e-30f;
-    nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, true, &fine);
+    nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, false, &fine);
     if (count_stats && lane == 0) atomicAdd(&counters[kStatCand1], *total);
     float t_lo = 0.0f, t_hi = INFINITY;
     if (centre_bound(D, sh, lane, nr, *total, cx, cy, cz, r0c * r0c * 1.0001f + 1e-30f, k_eff,
@@ -686,7 +719,15 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
 
   st.klo = 0; st.khi = 0; st.need = 0; st.cnt = 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
-    if (attempt == 0 && rs2 < INFINITY && fine) {
+    uint32_t ns = kSlotList;
+    if (attempt == 0 && rs2 < INFINITY) {
+      // exactly the photons of the run's sphere, when they fit
+      ns = collect_slots(D, sh, lane, nr, *total, cx, cy, cz, rs2 * 1.00001f);
+    }
+    if (ns != kSlotList) {
+      nr = kSlotList;
+      *total = ns;
+    } else if (attempt == 0 && rs2 < INFINITY && fine) {
       nr = filter_list(D, sh, lane, nr, cx, cy, cz, rs2, total);
     } else {
       nr = build_list(D, cx, cy, cz, rs2, sh, lane, total, cut_stat);

@@ -140,7 +140,16 @@ struct PhotonMapImpl {
   int first_leaf = 1;
   int anchor_stride = 8;
   int n_anchors = 0;
-  int anchor_k = -1;       // k_eff the anchor radii were computed for
+  // radius bounds of the anchors, one set per k (least recently used
+  // replaced); each holds the centroids with w = the k-NN radius bound
+  struct AnchorSet {
+    int k = -1;
+    uint64_t stamp = 0;
+    DevBuf buf;
+  };
+  static constexpr int kAnchorSets = 4;
+  AnchorSet aset[kAnchorSets];
+  uint64_t anchor_clock = 0;
   float qlo[3] = {0, 0, 0};
   float qscale[3] = {1, 1, 1};
   // original-order inputs (SoA, 3n each)
@@ -194,7 +203,9 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals,
 
 int build_photon_map(PhotonMapImpl* map, hipStream_t s);
 int update_flux(PhotonMapImpl* map, const float* flux_dev, hipStream_t s);
-int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s);
+// Anchor radius bounds of the map for k_eff (computed on first use, then
+// cached); *out receives the anchor array the gather must read.
+int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** out);
 void fill_desc_map_part(const PhotonMapImpl* map, GatherDesc* d);
 
 // Morton keys for all queries of all instances, stable sort, and the

@@ -294,3 +294,45 @@ class PhotonMap:
             self._drop_device()
         except Exception:
             pass
+
+
+def gather_batched(instances, ctx: Context | None = None, want_key: bool = True):
+    """Many independent radiance gathers in one persistent launch
+    (pm_gather_batched).  instances: sequence of (PhotonMap, queries (m, 10)
+    float32 records, materials (n_mats, 4) records, k, n_emit), queries and
+    materials as torch tensors on the GPU.  Returns a list of (rgb (m, 3),
+    r2 (m,), key (m,) or None) on the current stream, each equal bit for bit
+    to PhotonMap.gather_radiance of that instance."""
+    import torch
+
+    instances = list(instances)
+    if not instances:
+        return []
+    ctx = ctx or instances[0][0].ctx
+    arr = (_lib.pm_instance * len(instances))()
+    outs = []
+    keep = []
+    stream = None
+    for i, (pmap, q, mats, k, n_emit) in enumerate(instances):
+        dev = q.device
+        stream = stream or _stream_of(q)
+        m = int(q.shape[0])
+        mats = torch.as_tensor(mats, device=dev)
+        rgb = torch.empty((m, 3), dtype=torch.float32, device=dev)
+        r2 = torch.empty((m,), dtype=torch.float32, device=dev)
+        key = torch.empty((m,), dtype=torch.int64, device=dev) if want_key else None
+        keep += [q, mats]
+        a = arr[i]
+        a.map = pmap.handle
+        a.m = m
+        a.q = ptr(q)
+        a.mats = ptr(mats)
+        a.n_mats = int(mats.shape[0])
+        a.k = int(k)
+        a.n_emit = float(n_emit)
+        a.out_rgb = ptr(rgb)
+        a.out_r2 = ptr(r2)
+        a.out_key = ptr(key)
+        outs.append((rgb, r2, key))
+    call("pm_gather_batched", ctx.handle, len(instances), arr, stream)
+    return outs

@@ -43,6 +43,22 @@ def bytes_per_query(k: int) -> int:
     return 40 + 36 * k + 16
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "gather_kernel_radiance_box1M_k100_fg512x64_traffic.md"
+
+
+def traffic_per_launch():
+    """HBM bytes per gather launch measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (scaled by the same counters on a plain copy kernel), read
+    from the committed profile summary; None when it has not been measured."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(pos, wi, flux, q, mats, k, n_emit, budget_s=12.0, m_total=None):
     """Times the oracle's reference gather on a bounded sample of queries."""
     import numpy as np
@@ -92,6 +108,10 @@ def main() -> int:
                     help="query-run split factor (0: library default)")
     ap.add_argument("--waves", type=int, default=0, help="persistent waves per CU (0: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain 1 GiB read + 1 GiB write kernel (torch.neg) three "
+                         "times, the calibration of the HBM traffic counters")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     args = ap.parse_args()
 
@@ -194,6 +214,41 @@ def main() -> int:
     }
     gather_ms = float(np.mean(gms))
     sort_ms = float(np.mean(sms))
+
+    # ---- adjoint: reuse the forward keys, and recompute the sets ------------------
+    adj = None
+    if not args.no_adjoint:
+        g = torch.randn((m, 3), dtype=torch.float32, device=dev,
+                        generator=torch.Generator(device=dev).manual_seed(0))
+        dflux = torch.zeros((3, args.photons), dtype=torch.float32, device=dev)
+        dalb = torch.zeros((mats.shape[0], 3), dtype=torch.float32, device=dev)
+        step()  # fresh keys of this forward pass
+        adj = {}
+        ctx.set_option(_lib.PM_OPT_TIMING, 1)
+        for name, kk in (("reuse", key), ("recompute", None)):
+            times = []
+            for _ in range(max(2, args.steps // 2)):
+                _lib.call("pm_gather_adjoint", pmap.handle, m, _lib.ptr(q), _lib.ptr(mats),
+                          mats.shape[0], k, n_emit, _lib.ptr(g), _lib.ptr(kk), None,
+                          _lib.ptr(dflux), _lib.ptr(dalb), _lib.PM_MEM_DEVICE, sptr)
+                st_a = ctx.stats()
+                times.append(st_a["sort_ms"] + st_a["gather_ms"] + st_a["post_ms"])
+            adj[f"adjoint_{name}_ms"] = float(np.mean(times))
+        ctx.set_option(_lib.PM_OPT_TIMING, 0)
+        fwd_ms = sort_ms + gather_ms
+        adj["forward_ms"] = fwd_ms
+        adj["fwd_plus_adj_over_fwd_reuse"] = (fwd_ms + adj["adjoint_reuse_ms"]) / fwd_ms
+        adj["fwd_plus_adj_over_fwd_recompute"] = (fwd_ms + adj["adjoint_recompute_ms"]) / fwd_ms
+        log(f"adjoint: {adj}")
+
+    if args.copy_calib:
+        a = torch.ones((1 << 28,), dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x torch.neg over 2^28 float32 (1 GiB read + 1 GiB write)")
+        del a, b
     bq = bytes_per_query(k)
     achieved = m * bq / (gather_ms * 1e-3) / 1e9
     log(f"step {ms_per_step:.3f} ms | gather kernel {gather_ms:.3f} ms | sort+anchors "
@@ -236,13 +291,13 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_per_launch(),
             "kernel": "gather_kernel<0>",
             "kernel_ms": gather_ms,
             "bytes_per_query": bq,
         },
         "cpu_baseline": cpu,
-        "detail": {"sort_ms": sort_ms, "build_ms": build_ms, **work},
+        "detail": {"sort_ms": sort_ms, "build_ms": build_ms, **work, **(adj or {})},
     }
     print(json.dumps(out), flush=True)
     return 0

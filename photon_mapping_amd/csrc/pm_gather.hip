@@ -1128,10 +1128,11 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
       }
 
       if constexpr (MODE == kModeAdjoint) {
-        // ---- scatter pass: dL/dflux over the neighbour set {key <= tau} -------------
+        // ---- scatter: dL/dflux and dL/drho over the neighbour set {key <= tau} --------
         const float r2 = __uint_as_float(static_cast<uint32_t>(tau >> 32));
         const float denom = (D.n_emit * kPiF) * r2;
-        const bool contributes = part && lambert;
+        // a query contributes only when its own outgoing side passes
+        const bool contributes = part && lane_lambert;
         float g_r = 0.f, g_g = 0.f, g_b = 0.f;
         float fg_r = 1.f, fg_g = 1.f, fg_b = 1.f;
         int32_t fgm = -1;
@@ -1155,37 +1156,24 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         const float c_r = ((rho_r / kPiF) * ge_r) / denom;
         const float c_g = ((rho_g / kPiF) * ge_g) / denom;
         const float c_b = ((rho_b / kPiF) * ge_b) / denom;
-        const uint64_t klim = tau + 1;
+
+        // (1) per query: the flux sum S of the estimate, for dL/drho
         float s_r = 0.f, s_g = 0.f, s_b = 0.f;
-        const int32_t n = D.n;
-        float* __restrict__ dflux = D.dl_dflux;
+        const uint32_t t_hi = static_cast<uint32_t>(tau >> 32);
+        const uint32_t t_lo = static_cast<uint32_t>(tau & 0xffffffffull);
         for_candidates<true>(D, sh, lane, nr, total, [&](uint32_t cnt) {
-          for (uint32_t j = 0; j < cnt; ++j) {
-            const float4 p = sh.u.c.pos[j];
-            const float4 w = sh.u.c.wif[j];
-            const float2 f = sh.u.c.flux[j];
-            const float d2 = d2_ref(qx, qy, qz, p.x, p.y, p.z);
-            const uint64_t key = make_key(d2, __float_as_uint(p.w));
-            const float cos_i = dot_ref(w.x, w.y, w.z, nx, ny, nz);
-            const bool use = contributes && key < klim && !(cos_o < 0.0f || cos_i < 0.0f);
-            if (wave_any(use)) {
-              const float sel = use ? 1.0f : 0.0f;
-              s_r = fmaf(sel, w.w, s_r);
-              s_g = fmaf(sel, f.x, s_g);
-              s_b = fmaf(sel, f.y, s_b);
-              const float tr = wave_sum(use ? c_r : 0.0f);
-              const float tg = wave_sum(use ? c_g : 0.0f);
-              const float tb = wave_sum(use ? c_b : 0.0f);
-              if (lane == 0) {
-                const uint32_t idx = __float_as_uint(p.w);
-                atomicAdd(&dflux[idx], tr);
-                atomicAdd(&dflux[static_cast<int64_t>(n) + idx], tg);
-                atomicAdd(&dflux[2 * static_cast<int64_t>(n) + idx], tb);
-              }
-            }
-          }
+          consume<true>(sh, cnt, [&](const Cand& c) {
+            const float d2 = d2_ref(qx, qy, qz, c.p.x, c.p.y, c.p.z);
+            const uint32_t db = __float_as_uint(d2);
+            const bool nb = db < t_hi || (db == t_hi && __float_as_uint(c.p.w) <= t_lo);
+            const float cos_i = dot_ref(c.w.x, c.w.y, c.w.z, nx, ny, nz);
+            const float sel = (contributes && nb && !(cos_i < 0.0f)) ? 1.0f : 0.0f;
+            s_r = fmaf(sel, c.w.w, s_r);
+            s_g = fmaf(sel, c.f.x, s_g);
+            s_b = fmaf(sel, c.f.y, s_b);
+            return false;
+          });
         });
-        // dL/drho of the queried material, and of the final-gathering albedo
         if (contributes) {
           const float inv = 1.0f / (kPiF * denom);
           const float dr = ge_r * s_r * inv, dg = ge_g * s_g * inv, db = ge_b * s_b * inv;
@@ -1214,6 +1202,62 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
             }
           }
         }
+
+        // (2) per photon, one lane each: sum of the coefficients of the
+        // queries whose set holds it (the same FP32 key test as above), then
+        // one atomic per channel.  The contributing queries are read from
+        // LDS (the lane histograms are free here).
+        float4* qa = reinterpret_cast<float4*>(sh.hist);
+        float4* qb = qa + kWave;
+        float4* qc = qb + kWave;
+        static_assert(3 * kWave * 4 <= (kBins + 2) * kWave, "query staging fits the bins");
+        qa[lane] = make_float4(qx, qy, qz, __uint_as_float(t_hi));
+        qb[lane] = make_float4(nx, ny, nz, __uint_as_float(t_lo));
+        qc[lane] = make_float4(c_r, c_g, c_b, 0.0f);
+        const uint64_t cm = __ballot(contributes);
+        __syncthreads();
+        if (cm != 0ull) {
+          const float4* __restrict__ P = D.pos4;
+          const float4* __restrict__ W = D.wif4;
+          float* __restrict__ dflux = D.dl_dflux;
+          const int64_t n = D.n;
+          uint32_t r = 0;
+          for (uint32_t base = 0; base < total; base += kWave) {
+            const uint32_t t = base + static_cast<uint32_t>(lane);
+            const bool ok = t < total;
+            float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p;
+            if (ok) {
+              const int32_t slot = list_slot(D, sh, nr, t, r);
+              p = P[slot];
+              w = W[slot];
+            }
+            const uint32_t idx = __float_as_uint(p.w);
+            float a_r = 0.f, a_g = 0.f, a_b = 0.f;
+            uint64_t mm = cm;
+            while (mm != 0ull) {
+              const int i = __builtin_ctzll(mm);
+              mm &= mm - 1ull;
+              const float4 A = qa[i];
+              const float4 B = qb[i];
+              const float4 C = qc[i];
+              const float d2 = d2_ref(A.x, A.y, A.z, p.x, p.y, p.z);
+              const uint32_t db = __float_as_uint(d2);
+              const uint32_t th = __float_as_uint(A.w);
+              const bool nb = db < th || (db == th && idx <= __float_as_uint(B.w));
+              const float cos_i = dot_ref(w.x, w.y, w.z, B.x, B.y, B.z);
+              const float sel = (nb && !(cos_i < 0.0f)) ? 1.0f : 0.0f;
+              a_r = fmaf(sel, C.x, a_r);
+              a_g = fmaf(sel, C.y, a_g);
+              a_b = fmaf(sel, C.z, a_b);
+            }
+            if (ok && (a_r != 0.0f || a_g != 0.0f || a_b != 0.0f)) {
+              atomicAdd(&dflux[idx], a_r);
+              atomicAdd(&dflux[n + idx], a_g);
+              atomicAdd(&dflux[2 * n + idx], a_b);
+            }
+          }
+        }
+        __syncthreads();
       }
       pt.mark(6, lane);
       c0 = c1;

@@ -93,6 +93,91 @@ def cpu_baseline(pos, wi, flux, q, mats, k, n_emit, budget_s=12.0, m_total=None)
     }
 
 
+def batched_bench(ctx, dev, n_inst=64, frames=3, seed=0):
+    """Config-4 batching workload: n_inst box-scene instances of uneven size
+    (wall albedos uniform in [0.1, 0.9]; stored photons log-uniform in
+    [1e4, 4e6]; k from {32, 64, 100, 200}; square resolution from {128, 256,
+    512, 1024}; 16 spp final-gathering queries), timed three ways per frame:
+    one plain pm_gather_radiance launch per instance, one pm_gather_batched
+    launch, and the prepared plan replayed as a captured HIP graph.  The
+    spheres of the config are left out (the query producer casts rays by
+    brute force over the box's triangles)."""
+    import numpy as np
+    import torch
+
+    import photon_mapping_amd as pm
+    from photon_mapping_amd import scene as sc
+
+    rng = np.random.default_rng(seed)
+    inst, sizes = [], []
+    for i in range(n_inst):
+        s = sc.box_scene_instance(rng)
+        n = int(round(math.exp(rng.uniform(math.log(1e4), math.log(4e6)))))
+        k = int(rng.choice([32, 64, 100, 200]))
+        res = int(rng.choice([128, 256, 512, 1024]))
+        pos, wi, flux = sc.surrogate_photons(s, n, seed=1000 + i)
+        q = sc.camera_queries(s, res, res, 16, depth=1, seed=1000 + i)
+        pmap = pm.PhotonMap(ctx)
+        pmap.set_photons_soa(pos, wi, flux)
+        pmap.build()
+        del pos, wi, flux
+        inst.append((pmap, q, torch.from_numpy(s.materials()).to(dev), k, float(n)))
+        sizes.append((n, k, res))
+    total_q = sum(int(q.shape[0]) for _, q, *_ in inst)
+    torch.cuda.synchronize()
+
+    def frame_plain():
+        return [p.gather_radiance(q, mt, k, ne) for p, q, mt, k, ne in inst]
+
+    def frame_batched():
+        return pm.gather_batched(inst, ctx=ctx)
+
+    def timed(fn):
+        fn()  # warm-up (anchors for each map and k on the first pass)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / frames
+
+    t_plain = timed(frame_plain)
+    t_batched = timed(frame_batched)
+    plan = pm.BatchedPlan(inst, ctx=ctx)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        plan.launch()
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        plan.launch()
+    t_graph = timed(graph.replay)
+    # the graph's last replay against one eager batched frame, bit for bit
+    expect = frame_batched()
+    torch.cuda.synchronize()
+    same = all(torch.equal(a[0], b[0]) and torch.equal(a[2], b[2])
+               for a, b in zip(plan.outputs, expect))
+    del graph
+    plan.close()
+    return {
+        "instances": n_inst,
+        "queries_per_frame": total_q,
+        "photons_total": int(sum(n for n, _, _ in sizes)),
+        "plain_launches_ms_per_frame": 1e3 * t_plain,
+        "batched_ms_per_frame": 1e3 * t_batched,
+        "graph_ms_per_frame": 1e3 * t_graph,
+        "plain_queries_per_s": total_q / t_plain,
+        "batched_queries_per_s": total_q / t_batched,
+        "graph_queries_per_s": total_q / t_graph,
+        "batched_over_plain": t_plain / t_batched,
+        "graph_over_plain": t_plain / t_graph,
+        "frames_per_s_graph": 1.0 / t_graph,
+        "graph_equals_eager": bool(same),
+        "seed": 0,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +194,8 @@ def main() -> int:
     ap.add_argument("--waves", type=int, default=0, help="persistent waves per CU (0: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 64-instance batching measurement")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 1 GiB read + 1 GiB write kernel (torch.neg) three "
                          "times, the calibration of the HBM traffic counters")
@@ -241,6 +328,11 @@ def main() -> int:
         adj["fwd_plus_adj_over_fwd_recompute"] = (fwd_ms + adj["adjoint_recompute_ms"]) / fwd_ms
         log(f"adjoint: {adj}")
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_bench(ctx, dev)
+        log(f"batched: {batched}")
+
     if args.copy_calib:
         a = torch.ones((1 << 28,), dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
@@ -297,7 +389,8 @@ def main() -> int:
             "bytes_per_query": bq,
         },
         "cpu_baseline": cpu,
-        "detail": {"sort_ms": sort_ms, "build_ms": build_ms, **work, **(adj or {})},
+        "detail": {"sort_ms": sort_ms, "build_ms": build_ms, **work, **(adj or {}),
+                   "batched": batched},
     }
     print(json.dumps(out), flush=True)
     return 0

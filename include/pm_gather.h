@@ -199,6 +199,23 @@ typedef struct pm_instance {
 int pm_gather_batched(pm_context* ctx, int n_inst, const pm_instance* inst,
                       void* stream);
 
+/* ---- prepared batched gather (no reference counterpart) ---------------------------
+ * A batched gather prepared once and launched many times, e.g. once per frame
+ * inside a HIP graph capture.  pm_plan_create validates the instances,
+ * computes the radius anchors they need, keeps its own copies of them and of
+ * the instance descriptors in device memory, and reserves every workspace of
+ * the launch (it synchronises the stream).  pm_plan_launch then only enqueues
+ * kernels on the stream -- no host synchronisation, allocation or memset --
+ * reading the queries and writing the outputs at the addresses given at
+ * creation; each launch equals pm_gather_batched on those instances bit for
+ * bit.  The maps must stay built while the plan lives (their flux may change
+ * through pm_photonmap_update_flux); launches of one plan must not overlap. */
+typedef struct pm_plan pm_plan;
+int pm_plan_create(pm_context* ctx, int n_inst, const pm_instance* inst,
+                   void* stream, pm_plan** out_plan);
+int pm_plan_launch(pm_plan* plan, void* stream);
+int pm_plan_destroy(pm_plan* plan);
+
 #ifdef __cplusplus
 }
 #endif

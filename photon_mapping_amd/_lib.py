@@ -97,6 +97,9 @@ SIGNATURES = {
     "pm_gather_adjoint": (None, [VP, I64, VP, VP, I32, I32, DBL, VP, VP, VP, VP, VP, I32,
                                  VP]),
     "pm_gather_batched": (None, [VP, I32, VP, VP]),
+    "pm_plan_create": (None, [VP, I32, VP, VP, VP]),
+    "pm_plan_launch": (None, [VP, VP]),
+    "pm_plan_destroy": (None, [VP]),
     "pmx_fg_queries": (None, [VP, VP, I32, VP, I32, I32, I32, I32, C.c_uint64, VP, VP]),
     "pmx_surrogate_photons": (None, [VP, VP, I32, VP, I32, VP, I64, C.c_uint64, VP, VP,
                                      VP]),

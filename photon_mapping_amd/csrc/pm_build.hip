@@ -225,26 +225,24 @@ __global__ __launch_bounds__(kThreads) void query_keys_kernel(
   vals[i] = static_cast<uint32_t>(i);
 }
 
+// Nearest radius anchor in Morton order of every sorted query: the last
+// anchor whose first code is <= the query's code (upper bound, one back).
 template <typename K>
-__global__ __launch_bounds__(kThreads) void group_anchor_kernel(
-    const GatherDesc* __restrict__ descs, const int64_t* __restrict__ q_prefix,
-    const int64_t* __restrict__ g_prefix, int n_inst, int64_t total_groups,
-    const K* __restrict__ sorted_keys, uint32_t* __restrict__ group_anchor) {
-  const int64_t g = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (g >= total_groups) return;
-  const int inst = n_inst > 1 ? find_inst(g_prefix, n_inst, g) : 0;
+__global__ __launch_bounds__(kThreads) void query_anchor_kernel(
+    const GatherDesc* __restrict__ descs, const int64_t* __restrict__ q_prefix, int n_inst,
+    int64_t total_q, const K* __restrict__ sorted_keys, uint32_t* __restrict__ q_anchor) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= total_q) return;
+  const int inst = n_inst > 1 ? find_inst(q_prefix, n_inst, i) : 0;
   const GatherDesc& D = descs[inst];
-  const int64_t lg = g - g_prefix[inst];
-  const int64_t mid = min(lg * kWave + kWave / 2, D.m - 1);
-  const uint32_t code = static_cast<uint32_t>(sorted_keys[q_prefix[inst] + mid] & 0xffffffffu);
-  // upper bound over the anchor codes, then one step back
+  const uint32_t code = static_cast<uint32_t>(sorted_keys[i] & 0xffffffffu);
   int lo = 0, hi = D.n_anchors;
   while (lo < hi) {
     const int m = (lo + hi) >> 1;
     if (D.anchor_code[m] <= code) lo = m + 1;
     else hi = m;
   }
-  group_anchor[g] = static_cast<uint32_t>(lo > 0 ? lo - 1 : 0);
+  q_anchor[i] = static_cast<uint32_t>(lo > 0 ? lo - 1 : 0);
 }
 
 // ---- kNN row sort: neighbour indices, farthest first ------------------------------
@@ -392,21 +390,21 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
   }
 
   // 2-3. Morton codes, stable sort
-  if (int rc = ctx->ws_keys.reserve(n * sizeof(uint32_t))) return rc;
-  if (int rc = ctx->ws_vals.reserve(n * sizeof(uint32_t))) return rc;
-  if (int rc = ctx->ws_keys2.reserve(n * sizeof(uint32_t))) return rc;
-  if (int rc = ctx->ws_vals2.reserve(n * sizeof(uint32_t))) return rc;
+  if (int rc = ctx->ws.keys.reserve(n * sizeof(uint32_t))) return rc;
+  if (int rc = ctx->ws.vals.reserve(n * sizeof(uint32_t))) return rc;
+  if (int rc = ctx->ws.keys2.reserve(n * sizeof(uint32_t))) return rc;
+  if (int rc = ctx->ws.vals2.reserve(n * sizeof(uint32_t))) return rc;
   Quant qz;
   for (int c = 0; c < 3; ++c) { qz.lo[c] = map->qlo[c]; qz.scale[c] = map->qscale[c]; }
-  uint32_t* k0 = ctx->ws_keys.as<uint32_t>();
-  uint32_t* v0 = ctx->ws_vals.as<uint32_t>();
-  uint32_t* k1 = ctx->ws_keys2.as<uint32_t>();
-  uint32_t* v1 = ctx->ws_vals2.as<uint32_t>();
+  uint32_t* k0 = ctx->ws.keys.as<uint32_t>();
+  uint32_t* v0 = ctx->ws.vals.as<uint32_t>();
+  uint32_t* k1 = ctx->ws.keys2.as<uint32_t>();
+  uint32_t* v1 = ctx->ws.vals2.as<uint32_t>();
   hipLaunchKernelGGL(photon_codes_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s,
                      pos, n, qz, k0, v0);
   PM_HIP(hipGetLastError());
   int which = 0;
-  if (int rc = radix_sort_u32(ctx, k0, v0, k1, v1, n, 3 * kMortonBits, s, &which)) return rc;
+  if (int rc = radix_sort_u32(&ctx->ws, k0, v0, k1, v1, n, 3 * kMortonBits, s, &which)) return rc;
   const uint32_t* sorted_codes = which ? k1 : k0;
   const uint32_t* order = which ? v1 : v0;
 
@@ -491,7 +489,7 @@ int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** 
   d.k_eff = k_eff;
   d.all_members = 0;
   d.anchor_out = slot->buf.as<float>();
-  if (int rc = ctx->ws_counters.reserve(64 * sizeof(uint32_t))) return rc;
+  if (int rc = ctx->ws.counters.reserve(64 * sizeof(uint32_t))) return rc;
   int64_t gp[2] = {0, d.n_groups};
   if (int rc = stage_begin(ctx, sizeof(GatherDesc) + sizeof(gp) + 256, s)) return rc;
   GatherLaunch L{};
@@ -500,7 +498,7 @@ int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** 
   if (int rc = stage_commit(ctx, s)) return rc;
   L.perm = nullptr;
   L.q_prefix = nullptr;
-  L.counters = ctx->ws_counters.as<uint32_t>();
+  L.counters = ctx->ws.counters.as<uint32_t>();
   L.total_groups = d.n_groups;
   L.n_inst = 1;
   L.xcd_aware = 0;
@@ -514,7 +512,19 @@ int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** 
   return PM_OK;
 }
 
-int prepare_queries(Context* ctx, int n_inst, const GatherDesc* dev_descs,
+int prepare_reserve(Workspace* ws, int n_inst, int64_t total_q, int64_t total_groups) {
+  const size_t kbytes = n_inst > 1 ? sizeof(uint64_t) : sizeof(uint32_t);
+  const int64_t nq = std::max<int64_t>(total_q, 1);
+  if (int rc = ws->keys.reserve(nq * kbytes)) return rc;
+  if (int rc = ws->keys2.reserve(nq * kbytes)) return rc;
+  if (int rc = ws->vals.reserve(nq * sizeof(uint32_t))) return rc;
+  if (int rc = ws->vals2.reserve(nq * sizeof(uint32_t))) return rc;
+  if (int rc = ws->q_anchor.reserve(nq * sizeof(uint32_t))) return rc;
+  if (int rc = ws->counters.reserve(64 * sizeof(uint32_t))) return rc;
+  return sort_reserve(ws, nq);
+}
+
+int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
                     const int64_t* dev_q_prefix, const int64_t* dev_g_prefix,
                     int64_t total_q, int64_t total_groups, hipStream_t s,
                     const uint32_t** perm_out) {
@@ -523,41 +533,34 @@ int prepare_queries(Context* ctx, int n_inst, const GatherDesc* dev_descs,
     return PM_OK;
   }
   const bool wide = n_inst > 1;
-  const size_t kbytes = wide ? sizeof(uint64_t) : sizeof(uint32_t);
-  if (int rc = ctx->ws_keys.reserve(total_q * kbytes)) return rc;
-  if (int rc = ctx->ws_keys2.reserve(total_q * kbytes)) return rc;
-  if (int rc = ctx->ws_vals.reserve(total_q * sizeof(uint32_t))) return rc;
-  if (int rc = ctx->ws_vals2.reserve(total_q * sizeof(uint32_t))) return rc;
-  if (int rc = ctx->ws_group_anchor.reserve(std::max<int64_t>(total_groups, 1) * sizeof(uint32_t)))
-    return rc;
-  uint32_t* v0 = ctx->ws_vals.as<uint32_t>();
-  uint32_t* v1 = ctx->ws_vals2.as<uint32_t>();
+  if (int rc = prepare_reserve(ws, n_inst, total_q, total_groups)) return rc;
+  uint32_t* v0 = ws->vals.as<uint32_t>();
+  uint32_t* v1 = ws->vals2.as<uint32_t>();
   int which = 0;
   const dim3 grid(ceil_div(total_q, kThreads));
-  const dim3 ggrid(ceil_div(std::max<int64_t>(total_groups, 1), kThreads));
   if (!wide) {
-    uint32_t* k0 = ctx->ws_keys.as<uint32_t>();
-    uint32_t* k1 = ctx->ws_keys2.as<uint32_t>();
+    uint32_t* k0 = ws->keys.as<uint32_t>();
+    uint32_t* k1 = ws->keys2.as<uint32_t>();
     hipLaunchKernelGGL(query_keys_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
                        dev_q_prefix, n_inst, total_q, k0, v0);
     PM_HIP(hipGetLastError());
-    if (int rc = radix_sort_u32(ctx, k0, v0, k1, v1, total_q, 3 * kMortonBits, s, &which))
+    if (int rc = radix_sort_u32(ws, k0, v0, k1, v1, total_q, 3 * kMortonBits, s, &which))
       return rc;
-    hipLaunchKernelGGL(group_anchor_kernel<uint32_t>, ggrid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, dev_g_prefix, n_inst, total_groups, which ? k1 : k0,
-                       ctx->ws_group_anchor.as<uint32_t>());
+    hipLaunchKernelGGL(query_anchor_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
+                       dev_q_prefix, n_inst, total_q, which ? k1 : k0,
+                       ws->q_anchor.as<uint32_t>());
   } else {
-    uint64_t* k0 = ctx->ws_keys.as<uint64_t>();
-    uint64_t* k1 = ctx->ws_keys2.as<uint64_t>();
+    uint64_t* k0 = ws->keys.as<uint64_t>();
+    uint64_t* k1 = ws->keys2.as<uint64_t>();
     int ib = 1;
     while ((1 << ib) < n_inst) ++ib;
     hipLaunchKernelGGL(query_keys_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
                        dev_q_prefix, n_inst, total_q, k0, v0);
     PM_HIP(hipGetLastError());
-    if (int rc = radix_sort_u64(ctx, k0, v0, k1, v1, total_q, 32 + ib, s, &which)) return rc;
-    hipLaunchKernelGGL(group_anchor_kernel<uint64_t>, ggrid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, dev_g_prefix, n_inst, total_groups, which ? k1 : k0,
-                       ctx->ws_group_anchor.as<uint32_t>());
+    if (int rc = radix_sort_u64(ws, k0, v0, k1, v1, total_q, 32 + ib, s, &which)) return rc;
+    hipLaunchKernelGGL(query_anchor_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
+                       dev_q_prefix, n_inst, total_q, which ? k1 : k0,
+                       ws->q_anchor.as<uint32_t>());
   }
   PM_HIP(hipGetLastError());
   *perm_out = which ? v1 : v0;

@@ -147,11 +147,10 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
   const int64_t total_groups = gp[n_inst];
   if (total_q > (int64_t{1} << 32) - 1)
     return fail(PM_ERR_UNSUPPORTED, "more than 2^32 - 1 queries in one call");
-  if (int rc = ctx->ws_counters.reserve(64 * sizeof(uint32_t))) return rc;
-  if (int rc = ctx->ws_group_anchor.reserve(std::max<int64_t>(total_groups, 1) * sizeof(uint32_t)))
-    return rc;
+  if (int rc = ctx->ws.counters.reserve(64 * sizeof(uint32_t))) return rc;
+  if (int rc = prepare_reserve(&ctx->ws, n_inst, total_q, total_groups)) return rc;
   for (int i = 0; i < n_inst; ++i)
-    ds[i].group_anchor = ctx->ws_group_anchor.as<uint32_t>() + ds[i].group_base;
+    ds[i].q_anchor = ctx->ws.q_anchor.as<uint32_t>() + ds[i].q_base;
   const size_t db = ds.size() * sizeof(GatherDesc);
   const size_t pb = qp.size() * sizeof(int64_t);
   if (int rc = stage_begin(ctx, db + 2 * pb + 1024, s)) return rc;
@@ -163,7 +162,7 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
   CallTimer tm(ctx, s);
   tm.mark(0);
   const uint32_t* perm = nullptr;
-  if (int rc = prepare_queries(ctx, n_inst, dd, dqp, dgp, total_q, total_groups, s, &perm))
+  if (int rc = prepare_queries(&ctx->ws, n_inst, dd, dqp, dgp, total_q, total_groups, s, &perm))
     return rc;
   tm.mark(1);
   GatherLaunch L{};
@@ -171,7 +170,7 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
   L.group_prefix = dgp;
   L.perm = perm;
   L.q_prefix = dqp;
-  L.counters = ctx->ws_counters.as<uint32_t>();
+  L.counters = ctx->ws.counters.as<uint32_t>();
   L.total_groups = total_groups;
   L.n_inst = n_inst;
   L.xcd_aware = ctx->xcd_aware;
@@ -195,7 +194,7 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
     }
     if (ctx->count_stats) {
       uint32_t c[6] = {0, 0, 0, 0, 0, 0};
-      PM_HIP(hipMemcpy(c, ctx->ws_counters.as<uint32_t>() + kStatBase, sizeof(c),
+      PM_HIP(hipMemcpy(c, ctx->ws.counters.as<uint32_t>() + kStatBase, sizeof(c),
                        hipMemcpyDeviceToHost));
       ctx->stats.fallbacks = c[kStatFallback - kStatBase];
       ctx->stats.refinements = c[kStatRefine - kStatBase];
@@ -204,7 +203,7 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
       ctx->stats.cut_lists = c[kStatOverflow - kStatBase];
       ctx->stats.runs = c[kStatRuns - kStatBase];
       uint64_t ph[kPhases];
-      PM_HIP(hipMemcpy(ph, ctx->ws_counters.as<uint32_t>() + kPhaseBase, sizeof(ph),
+      PM_HIP(hipMemcpy(ph, ctx->ws.counters.as<uint32_t>() + kPhaseBase, sizeof(ph),
                        hipMemcpyDeviceToHost));
       for (int i = 0; i < kPhases; ++i) ctx->stats.phase_cycles[i] = static_cast<int64_t>(ph[i]);
     }
@@ -224,6 +223,174 @@ int check_map(pm_photonmap* h) {
   if (!map->built) return fail(PM_ERR_NOT_BUILT, "photon map has not been built");
   return set_device(map->ctx);
 }
+
+// Zeroes the outputs of an instance whose map holds no photons (empty set,
+// r^2 = 0, key 0), with fill kernels (safe inside a graph capture).
+struct ZeroFill {
+  float* rgb = nullptr;
+  float* r2 = nullptr;
+  uint64_t* key = nullptr;
+  int64_t m = 0;
+  int launch(hipStream_t s) const {
+    if (int rc = launch_fill_f32(rgb, 0.0f, 3 * m, s)) return rc;
+    if (r2)
+      if (int rc = launch_fill_f32(r2, 0.0f, m, s)) return rc;
+    if (key)
+      if (int rc = launch_fill_u32(reinterpret_cast<uint32_t*>(key), 0u, 2 * m, s)) return rc;
+    return PM_OK;
+  }
+};
+
+// Validates a batch and builds the descriptors of its non-empty instances
+// (map, query and output parts; run_instances lays out the rest), with the
+// anchor set of each instance's k, exactly as a single call builds it.
+int batch_descriptors(Context* ctx, int n_inst, const pm_instance* inst, hipStream_t s,
+                      std::vector<GatherDesc>& ds, std::vector<ZeroFill>& fills) {
+  for (int i = 0; i < n_inst; ++i) {
+    const pm_instance& in = inst[i];
+    if (int rc = check_map(in.map)) return rc;
+    if (reinterpret_cast<PhotonMapImpl*>(in.map)->ctx != ctx)
+      return fail(PM_ERR_INVALID_ARG, "instance map belongs to another context");
+    if (in.k < 1) return fail(PM_ERR_INVALID_ARG, "k must be >= 1");
+    if (in.m < 0) return fail(PM_ERR_INVALID_ARG, "negative query count");
+    if (in.m > (int64_t{1} << 31) - 1) return fail(PM_ERR_UNSUPPORTED, "m must be < 2^31");
+    if (in.m > 0 && (!in.q || !in.out_rgb))
+      return fail(PM_ERR_INVALID_ARG, "null query or output");
+    if (in.n_mats < 0 || (in.n_mats > 0 && !in.mats))
+      return fail(PM_ERR_INVALID_ARG, "bad materials");
+  }
+  ds.clear();
+  fills.clear();
+  for (int i = 0; i < n_inst; ++i) {
+    const pm_instance& in = inst[i];
+    auto* map = reinterpret_cast<PhotonMapImpl*>(in.map);
+    if (in.m == 0) continue;
+    if (map->n == 0) {
+      ZeroFill z;
+      z.rgb = in.out_rgb;
+      z.r2 = in.out_r2;
+      z.key = in.out_key;
+      z.m = in.m;
+      fills.push_back(z);
+      continue;
+    }
+    const int k_eff = static_cast<int>(std::min<int64_t>(in.k, map->n));
+    const float4* anchors = nullptr;
+    if (int rc = ensure_anchors(map, k_eff, s, &anchors)) return rc;
+    GatherDesc d{};
+    fill_desc_map_part(map, &d);
+    d.anchors = anchors;
+    d.q = in.q;
+    d.mats = in.mats;
+    d.n_mats = in.n_mats;
+    d.out_rgb = in.out_rgb;
+    d.out_r2 = in.out_r2;
+    d.out_key = in.out_key;
+    d.m = in.m;
+    d.k = in.k;
+    d.k_eff = k_eff;
+    d.n_emit = static_cast<float>(in.n_emit);
+    d.all_members = d.k_eff >= map->n ? 1 : 0;
+    ds.push_back(d);
+  }
+  return PM_OK;
+}
+
+// A prepared batched gather: private anchor copies, descriptors and
+// workspace, so that a launch only enqueues kernels.
+struct PlanImpl {
+  Context* ctx = nullptr;
+  std::vector<GatherDesc> ds;
+  std::vector<ZeroFill> fills;
+  Workspace ws;
+  DevBuf meta;     // descriptors, query prefix, group prefix
+  DevBuf anchors;  // copies of the anchor sets the descriptors use
+  int64_t total_q = 0, total_groups = 0;
+  const GatherDesc* dd = nullptr;
+  const int64_t* dqp = nullptr;
+  const int64_t* dgp = nullptr;
+
+  int prepare(hipStream_t s) {
+    const int n_inst = static_cast<int>(ds.size());
+    if (n_inst == 0) return PM_OK;
+    // private copies of the anchor radii (the map's cache may replace a set)
+    std::vector<const float4*> src;
+    std::vector<size_t> off;
+    size_t abytes = 0;
+    for (const GatherDesc& d : ds) {
+      size_t o = abytes;
+      for (size_t j = 0; j < src.size(); ++j)
+        if (src[j] == d.anchors) o = off[j];
+      if (o == abytes) {
+        src.push_back(d.anchors);
+        off.push_back(abytes);
+        abytes += static_cast<size_t>(d.n_anchors) * sizeof(float4);
+      }
+    }
+    if (int rc = anchors.reserve(std::max<size_t>(abytes, 16))) return rc;
+    for (size_t j = 0; j < src.size(); ++j) {
+      const size_t bytes = (j + 1 < src.size() ? off[j + 1] : abytes) - off[j];
+      PM_HIP(hipMemcpyAsync(static_cast<char*>(anchors.ptr) + off[j], src[j], bytes,
+                            hipMemcpyDeviceToDevice, s));
+    }
+    std::vector<int64_t> qp(n_inst + 1, 0), gp(n_inst + 1, 0);
+    for (int i = 0; i < n_inst; ++i) {
+      GatherDesc& d = ds[i];
+      for (size_t j = 0; j < src.size(); ++j)
+        if (src[j] == d.anchors) {
+          d.anchors = reinterpret_cast<const float4*>(static_cast<char*>(anchors.ptr) + off[j]);
+          break;
+        }
+      d.q_base = qp[i];
+      d.group_base = gp[i];
+      d.n_groups = ceil_div(d.m, kWave);
+      qp[i + 1] = qp[i] + d.m;
+      gp[i + 1] = gp[i] + d.n_groups;
+    }
+    total_q = qp[n_inst];
+    total_groups = gp[n_inst];
+    if (total_q > (int64_t{1} << 32) - 1)
+      return fail(PM_ERR_UNSUPPORTED, "more than 2^32 - 1 queries in one plan");
+    if (int rc = prepare_reserve(&ws, n_inst, total_q, total_groups)) return rc;
+    for (GatherDesc& d : ds) d.q_anchor = ws.q_anchor.as<uint32_t>() + d.q_base;
+    const size_t db = ds.size() * sizeof(GatherDesc);
+    const size_t pb = qp.size() * sizeof(int64_t);
+    if (int rc = meta.reserve(db + 2 * pb)) return rc;
+    std::vector<char> host(db + 2 * pb);
+    std::memcpy(host.data(), ds.data(), db);
+    std::memcpy(host.data() + db, qp.data(), pb);
+    std::memcpy(host.data() + db + pb, gp.data(), pb);
+    PM_HIP(hipMemcpyAsync(meta.ptr, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    PM_HIP(hipStreamSynchronize(s));
+    dd = meta.as<GatherDesc>();
+    dqp = reinterpret_cast<const int64_t*>(static_cast<char*>(meta.ptr) + db);
+    dgp = reinterpret_cast<const int64_t*>(static_cast<char*>(meta.ptr) + db + pb);
+    return PM_OK;
+  }
+
+  int launch(hipStream_t s) {
+    for (const ZeroFill& z : fills)
+      if (int rc = z.launch(s)) return rc;
+    if (ds.empty()) return PM_OK;
+    const int n_inst = static_cast<int>(ds.size());
+    const uint32_t* perm = nullptr;
+    if (int rc = prepare_queries(&ws, n_inst, dd, dqp, dgp, total_q, total_groups, s, &perm))
+      return rc;
+    GatherLaunch L{};
+    L.descs = dd;
+    L.group_prefix = dgp;
+    L.perm = perm;
+    L.q_prefix = dqp;
+    L.counters = ws.counters.as<uint32_t>();
+    L.total_groups = total_groups;
+    L.n_inst = n_inst;
+    L.xcd_aware = ctx->xcd_aware;
+    L.count_stats = 0;
+    L.run_split = ctx->run_split;
+    L.mode = kModeRadiance;
+    return launch_gather(ctx, L, s);
+  }
+};
 
 }  // namespace
 }  // namespace pm
@@ -673,58 +840,50 @@ int pm_gather_batched(pm_context* h, int n_inst, const pm_instance* inst, void* 
   if (n_inst == 0) return PM_OK;
   if (int rc = set_device(ctx)) return rc;
   hipStream_t s = as_stream(stream);
-  // validate every instance before anything is launched
-  for (int i = 0; i < n_inst; ++i) {
-    const pm_instance& in = inst[i];
-    if (int rc = check_map(in.map)) return rc;
-    if (reinterpret_cast<PhotonMapImpl*>(in.map)->ctx != ctx)
-      return fail(PM_ERR_INVALID_ARG, "instance map belongs to another context");
-    if (in.k < 1) return fail(PM_ERR_INVALID_ARG, "k must be >= 1");
-    if (in.m < 0) return fail(PM_ERR_INVALID_ARG, "negative query count");
-    if (in.m > (int64_t{1} << 31) - 1) return fail(PM_ERR_UNSUPPORTED, "m must be < 2^31");
-    if (in.m > 0 && (!in.q || !in.out_rgb))
-      return fail(PM_ERR_INVALID_ARG, "null query or output");
-    if (in.n_mats < 0 || (in.n_mats > 0 && !in.mats))
-      return fail(PM_ERR_INVALID_ARG, "bad materials");
-  }
   std::vector<GatherDesc> ds;
-  ds.reserve(n_inst);
-  for (int i = 0; i < n_inst; ++i) {
-    const pm_instance& in = inst[i];
-    auto* map = reinterpret_cast<PhotonMapImpl*>(in.map);
-    if (in.m == 0) continue;
-    if (map->n == 0) {
-      if (int rc = launch_fill_f32(in.out_rgb, 0.0f, 3 * in.m, s)) return rc;
-      if (in.out_r2)
-        if (int rc = launch_fill_f32(in.out_r2, 0.0f, in.m, s)) return rc;
-      if (in.out_key)
-        if (int rc = launch_fill_u32(reinterpret_cast<uint32_t*>(in.out_key), 0u, 2 * in.m, s))
-          return rc;
-      continue;
-    }
-    // the same anchor set as a single call with this k, so that every
-    // instance is computed exactly as pm_gather_radiance computes it
-    const int k_eff = static_cast<int>(std::min<int64_t>(in.k, map->n));
-    const float4* anchors = nullptr;
-    if (int rc = ensure_anchors(map, k_eff, s, &anchors)) return rc;
-    GatherDesc d{};
-    fill_desc_map_part(map, &d);
-    d.anchors = anchors;
-    d.q = in.q;
-    d.mats = in.mats;
-    d.n_mats = in.n_mats;
-    d.out_rgb = in.out_rgb;
-    d.out_r2 = in.out_r2;
-    d.out_key = in.out_key;
-    d.m = in.m;
-    d.k = in.k;
-    d.k_eff = k_eff;
-    d.n_emit = static_cast<float>(in.n_emit);
-    d.all_members = d.k_eff >= map->n ? 1 : 0;
-    ds.push_back(d);
-  }
+  std::vector<ZeroFill> fills;
+  if (int rc = batch_descriptors(ctx, n_inst, inst, s, ds, fills)) return rc;
+  for (const ZeroFill& z : fills)
+    if (int rc = z.launch(s)) return rc;
   if (ds.empty()) return PM_OK;
   return run_instances(ctx, ds, kModeRadiance, s, nullptr);
+}
+
+int pm_plan_create(pm_context* h, int n_inst, const pm_instance* inst, void* stream,
+                   pm_plan** out_plan) {
+  if (!out_plan) return fail(PM_ERR_INVALID_ARG, "null output handle");
+  *out_plan = nullptr;
+  if (!h) return fail(PM_ERR_INVALID_ARG, "null context");
+  Context* ctx = reinterpret_cast<Context*>(h);
+  if (n_inst < 1 || !inst) return fail(PM_ERR_INVALID_ARG, "bad instance list");
+  if (int rc = set_device(ctx)) return rc;
+  hipStream_t s = as_stream(stream);
+  auto* plan = new (std::nothrow) PlanImpl();
+  if (!plan) return fail(PM_ERR_OOM, "host allocation failed");
+  plan->ctx = ctx;
+  int rc = batch_descriptors(ctx, n_inst, inst, s, plan->ds, plan->fills);
+  if (rc == PM_OK) rc = plan->prepare(s);
+  if (rc != PM_OK) {
+    delete plan;
+    return rc;
+  }
+  *out_plan = reinterpret_cast<pm_plan*>(plan);
+  return PM_OK;
+}
+
+int pm_plan_launch(pm_plan* h, void* stream) {
+  if (!h) return fail(PM_ERR_INVALID_ARG, "null plan");
+  auto* plan = reinterpret_cast<PlanImpl*>(h);
+  if (int rc = set_device(plan->ctx)) return rc;
+  return plan->launch(as_stream(stream));
+}
+
+int pm_plan_destroy(pm_plan* h) {
+  if (!h) return PM_OK;
+  auto* plan = reinterpret_cast<PlanImpl*>(h);
+  (void)set_device(plan->ctx);
+  delete plan;
+  return PM_OK;
 }
 
 }  // extern "C"

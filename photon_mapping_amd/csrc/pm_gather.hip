@@ -918,12 +918,14 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
     // Triangle inequality on precomputed anchors: the k nearest photons of
     // anchor a lie within r_a of its centroid c_a, hence within
     // r_a + |q - c_a| of q.  The anchor radii are already rounded upward.
-    const PM_CONST float4* A = cptr(D.anchors);
-    const int32_t center = uni(static_cast<int32_t>(D.group_anchor[lg]));
+    const float4* __restrict__ A = D.anchors;
+    const int32_t center = static_cast<int32_t>(D.q_anchor[active ? qloc : lg * kWave]);
     const int32_t a0 = max(center - kAnchorWindow, 0);
     const int32_t a1 = min(center + kAnchorWindow, D.n_anchors - 1);
     float best = INFINITY;
-    for (int32_t a = a0; a <= a1; ++a) {
+#pragma unroll
+    for (int32_t o = -kAnchorWindow; o <= kAnchorWindow; ++o) {
+      const int32_t a = min(max(center + o, a0), a1);
       const float4 an = A[a];
       const float dx = qx - an.x, dy = qy - an.y, dz = qz - an.z;
       best = fminf(best, an.w + sqrtf(dx * dx + dy * dy + dz * dz));

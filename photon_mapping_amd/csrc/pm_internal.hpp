@@ -56,6 +56,14 @@ struct DevBuf {
   T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Device scratch of one gather call: query keys and permutation (double
+// buffered for the radix sort), sort histograms and scan partials, the
+// nearest anchor of every sorted query, and the work-queue / statistics
+// counters.
+struct Workspace {
+  DevBuf keys, vals, keys2, vals2, counts, partials, q_anchor, counters;
+};
+
 // ---- gather instance descriptor (device-resident, read with scalar loads) ----
 // One per photon-map x query-batch pairing.  All pointers are device memory.
 struct alignas(16) GatherDesc {
@@ -72,7 +80,7 @@ struct alignas(16) GatherDesc {
   const pm_query* q;       // RADIANCE / ADJOINT: local query records
   const float* q_soa;      // KNN: 3*m positions
   const pm_material* mats;
-  const uint32_t* group_anchor;  // local group -> nearest anchor index
+  const uint32_t* q_anchor;  // sorted query position -> nearest anchor (Morton order)
   // outputs
   float* out_rgb;
   float* out_r2;
@@ -171,8 +179,8 @@ struct Context {
   int count_stats = 0;
   pm_stats stats{};
   // workspace
-  DevBuf ws_keys, ws_vals, ws_keys2, ws_vals2, ws_counts, ws_partials;
-  DevBuf ws_group_anchor, ws_counters, ws_slots;
+  Workspace ws;
+  DevBuf ws_slots;
   DevBuf ws_stage_in, ws_stage_out, ws_stage_aux;
   DevBuf ws_tmp;
   // descriptor staging: pinned host arena mirrored into a device arena;
@@ -194,10 +202,12 @@ int stage_commit(Context* ctx, hipStream_t s);
 // ---- launchers (defined in the .hip files) -----------------------------------------
 // Radix sort of (key, value) pairs on the low `bits` bits of the key.
 // Returns the index (0 or 1) of the buffer pair that holds the result.
-int radix_sort_u32(Context* ctx, uint32_t* keys, uint32_t* vals,
+int radix_sort_u32(Workspace* ws, uint32_t* keys, uint32_t* vals,
                    uint32_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
                    hipStream_t s, int* which);
-int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals,
+// Reserves the sort histograms and scan partials for n keys.
+int sort_reserve(Workspace* ws, int64_t n);
+int radix_sort_u64(Workspace* ws, uint64_t* keys, uint32_t* vals,
                    uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
                    hipStream_t s, int* which);
 
@@ -209,9 +219,11 @@ int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** 
 void fill_desc_map_part(const PhotonMapImpl* map, GatherDesc* d);
 
 // Morton keys for all queries of all instances, stable sort, and the
-// nearest radius anchor of every group.  Group anchors land in
-// ctx->ws_group_anchor (global group order).
-int prepare_queries(Context* ctx, int n_inst, const GatherDesc* dev_descs,
+// nearest radius anchor of every sorted query (ws->q_anchor, global sorted
+// order).
+// Reserves every buffer prepare_queries and the gather launch use.
+int prepare_reserve(Workspace* ws, int n_inst, int64_t total_q, int64_t total_groups);
+int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
                     const int64_t* dev_q_prefix, const int64_t* dev_g_prefix,
                     int64_t total_q, int64_t total_groups, hipStream_t s,
                     const uint32_t** perm_out);

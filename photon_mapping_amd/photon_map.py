@@ -336,3 +336,63 @@ def gather_batched(instances, ctx: Context | None = None, want_key: bool = True)
         outs.append((rgb, r2, key))
     call("pm_gather_batched", ctx.handle, len(instances), arr, stream)
     return outs
+
+
+class BatchedPlan:
+    """A prepared batched gather (pm_plan_create): the outputs are allocated
+    here once, and ``launch()`` enqueues the whole gather on the current
+    stream with kernels only, so it can be captured in a torch.cuda.CUDAGraph
+    and replayed per frame.  ``outputs`` holds (rgb, r2, key) per instance."""
+
+    def __init__(self, instances, ctx: Context | None = None):
+        import torch
+
+        instances = list(instances)
+        if not instances:
+            raise ValueError("a plan needs at least one instance")
+        self.ctx = ctx or instances[0][0].ctx
+        self._arr = (_lib.pm_instance * len(instances))()
+        self._keep = []
+        self.outputs = []
+        for i, (pmap, q, mats, k, n_emit) in enumerate(instances):
+            dev = q.device
+            m = int(q.shape[0])
+            mats = torch.as_tensor(mats, device=dev)
+            rgb = torch.empty((m, 3), dtype=torch.float32, device=dev)
+            r2 = torch.empty((m,), dtype=torch.float32, device=dev)
+            key = torch.empty((m,), dtype=torch.int64, device=dev)
+            self._keep += [pmap, q, mats]
+            a = self._arr[i]
+            a.map = pmap.handle
+            a.m = m
+            a.q = ptr(q)
+            a.mats = ptr(mats)
+            a.n_mats = int(mats.shape[0])
+            a.k = int(k)
+            a.n_emit = float(n_emit)
+            a.out_rgb = ptr(rgb)
+            a.out_r2 = ptr(r2)
+            a.out_key = ptr(key)
+            self.outputs.append((rgb, r2, key))
+        h = C.c_void_p()
+        call("pm_plan_create", self.ctx.handle, len(instances), self._arr,
+             _stream_of(instances[0][1]), C.byref(h))
+        self.handle = h
+        self._device = instances[0][1].device
+
+    def launch(self) -> None:
+        import torch
+
+        call("pm_plan_launch", self.handle,
+             C.c_void_p(torch.cuda.current_stream(self._device).cuda_stream))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            call("pm_plan_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -136,6 +136,14 @@ def box_scene() -> Scene:
                  0.25 * 0.25, cam)
 
 
+def box_scene_instance(rng) -> Scene:
+    """The box scene with the five walls' albedos drawn uniform in [0.1, 0.9]
+    (the inner boxes share the floor/ceiling/back material)."""
+    s = box_scene()
+    s.albedo[:3] = rng.uniform(0.1, 0.9, (3, 3)).astype(np.float32)
+    return s
+
+
 def _camera_struct(cam: dict) -> pmx_camera:
     c = pmx_camera()
     for k in ("position", "forward", "right", "up"):

@@ -104,6 +104,6 @@ def test_autograd_fits_albedo(gpu, rng):
         loss = ((gather_radiance(pmap, qt, flux_t, alb, types, k, n_emit) - target) ** 2).mean()
         loss.backward()
         opt.step()
-        first = first if first is not None else float(loss)
-    assert float(loss) < 1e-3 * first
+        first = first if first is not None else float(loss.detach())
+    assert float(loss.detach()) < 1e-3 * first
     assert torch.allclose(alb.detach(), true_alb, rtol=2e-2, atol=2e-3)

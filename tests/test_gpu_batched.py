@@ -75,3 +75,75 @@ def test_batched_rejects_foreign_context(gpu, rng):
     with pytest.raises(gpu.PmError):
         gpu.gather_batched([(pmap, torch.from_numpy(q).to(dev), mats, 8, 1.0)],
                            ctx=gpu.Context.default())
+
+
+def _plan_instances(gpu, rng, dev, mats):
+    import torch
+
+    inst = []
+    for n, m, k, gen in [(30000, 4000, 64, "surfaces"), (8000, 1500, 100, "clustered"),
+                         (3000, 900, 200, "surfaces")]:
+        pos, wi, flux, q = _instance_data(rng, n, m, gen)
+        pmap = gpu.PhotonMap()
+        pmap.set_photons_soa(torch.from_numpy(pos).to(dev), torch.from_numpy(wi).to(dev),
+                             torch.from_numpy(flux).to(dev))
+        pmap.build()
+        inst.append((pmap, torch.from_numpy(q).to(dev), mats, k, 500.0 + n))
+    return inst
+
+
+def test_plan_launch_equals_batched(gpu, rng):
+    import torch
+
+    dev = torch.device("cuda:0")
+    mats = torch.from_numpy(materials(3, with_specular=True)).to(dev)
+    inst = _plan_instances(gpu, rng, dev, mats)
+    print("plan launch vs batched", flush=True)
+    plan = gpu.BatchedPlan(inst)
+    expect = gpu.gather_batched(inst)
+    for _ in range(2):
+        plan.launch()
+        torch.cuda.synchronize()
+        for (rgb, r2, key), (e_rgb, e_r2, e_key) in zip(plan.outputs, expect):
+            assert torch.equal(rgb.view(torch.int32), e_rgb.view(torch.int32))
+            assert torch.equal(r2.view(torch.int32), e_r2.view(torch.int32))
+            assert torch.equal(key, e_key)
+    plan.close()
+
+
+def test_plan_graph_replays(gpu, rng):
+    """One frame captured in a HIP graph and replayed four times, the photon
+    flux changed between replays (eager work in between); every replay equals
+    the eager batched gather of that frame bit for bit."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    mats = torch.from_numpy(materials(3)).to(dev)
+    inst = _plan_instances(gpu, rng, dev, mats)
+    base_flux = [torch.rand((p.getNPhotons(), 3), device=dev) + 0.1 for p, *_ in inst]
+    plan = gpu.BatchedPlan(inst)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        plan.launch()  # warm-up outside the capture
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        plan.launch()
+    torch.cuda.synchronize()
+    for frame in range(4):
+        print(f"graph replay {frame}", flush=True)
+        for (pmap, *_), f in zip(inst, base_flux):
+            pmap.update_flux(f * (1.0 + frame))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        expect = gpu.gather_batched(inst)
+        torch.cuda.synchronize()
+        for (rgb, r2, key), (e_rgb, e_r2, e_key) in zip(plan.outputs, expect):
+            assert torch.equal(rgb.view(torch.int32), e_rgb.view(torch.int32))
+            assert torch.equal(r2.view(torch.int32), e_r2.view(torch.int32))
+            assert torch.equal(key, e_key)
+    del graph
+    plan.close()

@@ -1,6 +1,7 @@
 """ctypes bindings of libpm_gather.so (the C ABI in include/pm_gather.h).
 
-The library is loaded from the package directory only.  There is no CPU
+The library is loaded from the package directory (or from the path in the
+PM_GATHER_LIB environment variable).  There is no CPU
 fallback: if the HIP library is missing or does not load, every entry point
 raises.
 """
@@ -8,10 +9,13 @@ raises.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import threading
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libpm_gather.so"
+# PM_GATHER_LIB selects another build of the library (tuning variants).
+LIB_PATH = Path(os.environ.get("PM_GATHER_LIB") or
+                Path(__file__).resolve().parent / "libpm_gather.so")
 
 PM_OK = 0
 PM_ERR_INVALID_ARG = 1

@@ -56,8 +56,13 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
 
 
-def build_library(verbose: bool = False) -> Path:
+def build_library(verbose: bool = False, out: Path | None = None) -> Path:
+    """Builds the library (to ``out``, default photon_mapping_amd/libpm_gather.so).
+    PM_HIPCC_DEFINES (space-separated -D flags) passes tuning macros such as
+    -DPM_GATHER_MIN_WAVES=3 to every translation unit."""
     hipcc = _hipcc()
+    defines = os.environ.get("PM_HIPCC_DEFINES", "").split()
+    lib_path = Path(out) if out else LIB_PATH
     BUILD_DIR.mkdir(exist_ok=True)
     sources = [CSRC / s for s in HIP_SOURCES]
     objects = []
@@ -65,19 +70,19 @@ def build_library(verbose: bool = False) -> Path:
     for src in sources:
         obj = BUILD_DIR / (src.stem + ".o")
         objects.append(obj)
-        cmds.append([hipcc, *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src),
-                     "-o", str(obj)])
+        cmds.append([hipcc, *HIPCC_FLAGS, *defines, f"-I{INCLUDE}", f"-I{CSRC}", "-c",
+                     str(src), "-o", str(obj)])
     workers = min(len(cmds), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         for cmd, fut in zip(cmds, [ex.submit(_run, c) for c in cmds]):
             if verbose:
                 print(" ".join(cmd), flush=True)
             fut.result()
-    tmp = LIB_PATH.with_suffix(".so.tmp")
+    tmp = lib_path.with_suffix(".so.tmp")
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
           *[str(o) for o in objects]])
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, lib_path)
+    return lib_path
 
 
 def build_oracle() -> Path:

@@ -303,8 +303,9 @@ __device__ uint32_t list_prefix(DescRef D, GatherShared& sh, int lane, uint32_t 
 // When the list would overflow, the walk restarts with whole inner nodes
 // allowed, then with a depth cap above the level that overflowed (at most
 // 2^8 subtrees, so it fits); a frontier that would overflow emits its nodes
-// whole.  *fine (when given) tells whether the list kept leaf granularity
-// (no coarsening); coarsenings are counted in *stat when given.
+// whole.  *fine (when given) tells whether every entry is a single leaf
+// (asked with leaf_only and no coarsening was needed); coarsenings are
+// counted in *stat when given.
 __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2,
                                GatherShared& sh, int lane, uint32_t* total, uint32_t* stat,
                                bool leaf_only = false, bool* fine = nullptr) {
@@ -388,7 +389,7 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
     if (lane == 0) sh.rnode[0] = 1u;
     __syncthreads();
   }
-  if (fine) *fine = !coarse && max_depth == depth_leaf;
+  if (fine) *fine = leaf_only && !coarse && !whole && max_depth == depth_leaf;
   return list_prefix(D, sh, lane, nr, total);
 }
 
@@ -506,15 +507,48 @@ __device__ __forceinline__ void sweep_lanes(DescRef D, const GatherShared& sh, i
   }
 }
 
+// Lane-parallel sweep of the current list like sweep_lanes; when every entry
+// is a single leaf (leaves), a lane takes a whole leaf instead, eight
+// contiguous slots per call round, so no lane walks the list.
+template <typename Fn>
+__device__ __forceinline__ void sweep_list(DescRef D, const GatherShared& sh, int lane,
+                                           uint32_t nr, uint32_t total, bool leaves, Fn&& fn) {
+  if (!leaves) {
+    sweep_lanes(D, sh, lane, nr, total, fn);
+    return;
+  }
+  const float4* __restrict__ P = D.pos4;
+  const int32_t leaf_size = D.leaf_size;
+  for (uint32_t base = 0; base < nr; base += kWave) {
+    const uint32_t i = base + static_cast<uint32_t>(lane);
+    int32_t s0 = 0, s1 = 0;
+    if (i < nr) node_slots(D, sh.rnode[i], s0, s1);
+    for (int32_t j0 = 0; j0 < leaf_size; j0 += 8) {
+      float4 p[8];
+      bool ok[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int32_t sl = s0 + j0 + u;
+        ok[u] = sl < s1;
+        p[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok[u]) p[u] = P[sl];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) fn(u, ok[u], p[u], s0 + j0 + u);
+    }
+  }
+}
+
 // Explicit slot list of the photons of the current subtree list whose FP32
 // d^2 from c is <= lim2, written to sh.cslot.  Returns the count, or
 // kSlotList when more than kSlotCap photons qualify (sh.cslot then holds a
 // prefix only and must not be used).
 __device__ uint32_t collect_slots(DescRef D, GatherShared& sh, int lane, uint32_t nr,
-                                  uint32_t total, float cx, float cy, float cz, float lim2) {
+                                  uint32_t total, bool leaves, float cx, float cy, float cz,
+                                  float lim2) {
   uint32_t n = 0;
   bool over = false;
-  sweep_lanes(D, sh, lane, nr, total, [&](int, bool ok, const float4& p, int32_t slot) {
+  sweep_list(D, sh, lane, nr, total, leaves, [&](int, bool ok, const float4& p, int32_t slot) {
     const bool keep = ok && d2_ref(cx, cy, cz, p.x, p.y, p.z) <= lim2;
     const uint64_t km = __ballot(keep);
     const uint32_t at = n + lanes_below(km, lane);
@@ -607,13 +641,13 @@ __device__ __forceinline__ void hist_add(GatherShared& sh, uint32_t b, int lane)
 // bins over [0, rb); returns false when fewer than k_eff photons fall below
 // rb.
 __device__ bool centre_bound(DescRef D, GatherShared& sh, int lane, uint32_t nr,
-                             uint32_t total, float cx, float cy, float cz, float rb,
-                             uint32_t k_eff, float* t_lo, float* t_hi) {
+                             uint32_t total, bool leaves, float cx, float cy, float cz,
+                             float rb, uint32_t k_eff, float* t_lo, float* t_hi) {
   uint32_t* ch = sh.hist;
   for (int i = lane; i < kCentreBins; i += kWave) ch[i] = 0u;
   __syncthreads();
   const float inv_w = static_cast<float>(kCentreBins) / rb;
-  sweep_lanes(D, sh, lane, nr, total, [&](int, bool ok, const float4& p, int32_t) {
+  sweep_list(D, sh, lane, nr, total, leaves, [&](int, bool ok, const float4& p, int32_t) {
     const float t = d2_ref(cx, cy, cz, p.x, p.y, p.z) * inv_w;
     if (ok && t < static_cast<float>(kCentreBins)) lds_inc(&ch[static_cast<uint32_t>(t)]);
   });
@@ -694,11 +728,11 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
     // The list of the centre sweep also covers the run's final sphere
     // (r_c <= r0c up to the margins), which is then a filter of it.
     const float rl = (r0c * 1.0001f + 2.0f * dmax) * 1.0004f + 2e-30f;
-    nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, false, &fine);
+    nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, true, &fine);
     if (count_stats && lane == 0) atomicAdd(&counters[kStatCand1], *total);
     float t_lo = 0.0f, t_hi = INFINITY;
-    if (centre_bound(D, sh, lane, nr, *total, cx, cy, cz, r0c * r0c * 1.0001f + 1e-30f, k_eff,
-                     &t_lo, &t_hi)) {
+    if (centre_bound(D, sh, lane, nr, *total, fine, cx, cy, cz, r0c * r0c * 1.0001f + 1e-30f,
+                     k_eff, &t_lo, &t_hi)) {
       rc_hi = sqrtf(t_hi) * 1.00001f + 1e-30f;
       rc_lo = sqrtf(t_lo) * 0.99999f;
     }
@@ -722,7 +756,7 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
     uint32_t ns = kSlotList;
     if (attempt == 0 && rs2 < INFINITY) {
       // exactly the photons of the run's sphere, when they fit
-      ns = collect_slots(D, sh, lane, nr, *total, cx, cy, cz, rs2 * 1.00001f);
+      ns = collect_slots(D, sh, lane, nr, *total, fine, cx, cy, cz, rs2 * 1.00001f);
     }
     if (ns != kSlotList) {
       nr = kSlotList;
@@ -1303,8 +1337,14 @@ __device__ __forceinline__ int find_instance(const PM_CONST int64_t* prefix, int
   return uni(lo);
 }
 
+// Minimum resident waves per SIMD the register allocation must allow
+// (a tuning macro; 1 lets the compiler use up to 512 registers).
+#ifndef PM_GATHER_MIN_WAVES
+#define PM_GATHER_MIN_WAVES 1
+#endif
+
 template <int MODE>
-__global__ __launch_bounds__(kWave) void gather_kernel(GatherLaunch L) {
+__global__ __launch_bounds__(kWave, PM_GATHER_MIN_WAVES) void gather_kernel(GatherLaunch L) {
   __shared__ GatherShared sh;
   const int lane = threadIdx.x;
   if (MODE == kModeAdjoint) {

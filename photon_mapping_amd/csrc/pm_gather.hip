@@ -176,20 +176,68 @@ __device__ __forceinline__ uint32_t bin_start_bits(uint32_t b, float inv_w, floa
   return lo;
 }
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+// ---- wave-wide scans and reductions over DPP row shifts and broadcasts ------
+// (every lane of the wave must be active).  After row_shr 1, 2, 4, 8 each
+// lane holds the prefix of its 16-lane row; row_bcast 15 and 31 carry the
+// row totals across rows, so lane 63 ends with the total.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+      static_cast<int>(old), static_cast<int>(v), CTRL, ROW_MASK, 0xf, false));
+}
+
+// Inclusive prefix sum over the lanes.
+__device__ __forceinline__ uint32_t wave_scan_add(uint32_t v) {
+  v += dpp_u32<0x111, 0xf>(0u, v);
+  v += dpp_u32<0x112, 0xf>(0u, v);
+  v += dpp_u32<0x114, 0xf>(0u, v);
+  v += dpp_u32<0x118, 0xf>(0u, v);
+  v += dpp_u32<0x142, 0xa>(0u, v);
+  v += dpp_u32<0x143, 0xc>(0u, v);
   return v;
 }
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, kWave));
-  return unif(v);
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+  v = min(v, dpp_u32<0x111, 0xf>(~0u, v));
+  v = min(v, dpp_u32<0x112, 0xf>(~0u, v));
+  v = min(v, dpp_u32<0x114, 0xf>(~0u, v));
+  v = min(v, dpp_u32<0x118, 0xf>(~0u, v));
+  v = min(v, dpp_u32<0x142, 0xa>(~0u, v));
+  v = min(v, dpp_u32<0x143, 0xc>(~0u, v));
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), kWave - 1));
 }
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
-  return unif(v);
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  v = max(v, dpp_u32<0x111, 0xf>(0u, v));
+  v = max(v, dpp_u32<0x112, 0xf>(0u, v));
+  v = max(v, dpp_u32<0x114, 0xf>(0u, v));
+  v = max(v, dpp_u32<0x118, 0xf>(0u, v));
+  v = max(v, dpp_u32<0x142, 0xa>(0u, v));
+  v = max(v, dpp_u32<0x143, 0xc>(0u, v));
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), kWave - 1));
+}
+
+// Order-preserving map of floats onto unsigned integers (and back).
+__device__ __forceinline__ uint32_t fkey(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float funkey(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// Wave-uniform minimum / maximum of a float.
+__device__ __forceinline__ float wave_min(float v) { return funkey(wave_min_u32(fkey(v))); }
+__device__ __forceinline__ float wave_max(float v) { return funkey(wave_max_u32(fkey(v))); }
+
+// Square roots for radius bounds, from the hardware instruction (1 ulp);
+// every bound carries a relative margin far above that.  The upper form
+// never underestimates a denormal argument, the lower never overestimates.
+__device__ __forceinline__ float sqrt_up(float x) {
+  return __builtin_amdgcn_sqrtf(fmaxf(x, FLT_MIN));
+}
+__device__ __forceinline__ float sqrt_down(float x) {
+  return x < FLT_MIN ? 0.0f : __builtin_amdgcn_sqrtf(x);
 }
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask, int lane) {
@@ -275,14 +323,9 @@ __device__ uint32_t list_prefix(DescRef D, GatherShared& sh, int lane, uint32_t 
       node_slots(D, sh.rnode[i], s0, s1);
       cnt = static_cast<uint32_t>(s1 - s0);
     }
-    uint32_t incl = cnt;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-      const uint32_t y = __shfl_up(incl, o, kWave);
-      if (lane >= o) incl += y;
-    }
+    const uint32_t incl = wave_scan_add(cnt);
     if (i < nr) sh.rpre[i] = run + incl - cnt;
-    run += static_cast<uint32_t>(__shfl(incl, kWave - 1, kWave));
+    run += static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
   }
   if (lane == 0) sh.rpre[nr] = run;
   __syncthreads();
@@ -655,12 +698,7 @@ __device__ bool centre_bound(DescRef D, GatherShared& sh, int lane, uint32_t nr,
   static_assert(kCentreBins == 4 * kWave, "four centre bins per lane");
   const uint4 h = reinterpret_cast<const uint4*>(ch)[lane];
   const uint32_t s = h.x + h.y + h.z + h.w;
-  uint32_t incl = s;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const uint32_t y = __shfl_up(incl, o, kWave);
-    if (lane >= o) incl += y;
-  }
+  const uint32_t incl = wave_scan_add(s);
   const uint64_t hit = __ballot(incl >= k_eff);
   __syncthreads();
   if (hit == 0ull) return false;
@@ -676,8 +714,7 @@ __device__ bool centre_bound(DescRef D, GatherShared& sh, int lane, uint32_t nr,
       if (cum + h.z < k_eff) ++b;
     }
   }
-  b = static_cast<uint32_t>(__shfl(static_cast<int>(b), fl, kWave));
-  b = static_cast<uint32_t>(uni(static_cast<int>(b)));
+  b = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b), fl));
   // photons of bins <= b have d^2 * inv_w < b + 1 before rounding of the
   // product, hence d^2 < (b + 1) / inv_w up to the margin; photons with
   // d^2 < b / inv_w (less the margin) all lie in bins < b
@@ -719,7 +756,7 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
   // ---- centre bracket: lane q's k nearest photons lie within X_q of q, hence
   // within X_q + |q - c| of c, so the smallest such sum bounds the centre's
   // k-NN radius; a histogram around c then brackets it
-  const float xq = part ? sqrtf(cull_bound(r0sq)) * 1.00002f + dq : INFINITY;
+  const float xq = part ? sqrt_up(cull_bound(r0sq)) * 1.00002f + dq : INFINITY;
   const float r0c = wave_min(xq) * 1.00002f;
   float rc_hi = INFINITY, rc_lo = 0.0f;
   uint32_t nr = 0;
@@ -733,8 +770,8 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
     float t_lo = 0.0f, t_hi = INFINITY;
     if (centre_bound(D, sh, lane, nr, *total, fine, cx, cy, cz, r0c * r0c * 1.0001f + 1e-30f,
                      k_eff, &t_lo, &t_hi)) {
-      rc_hi = sqrtf(t_hi) * 1.00001f + 1e-30f;
-      rc_lo = sqrtf(t_lo) * 0.99999f;
+      rc_hi = sqrt_up(t_hi) * 1.00001f + 1e-30f;
+      rc_lo = sqrt_down(t_lo) * 0.99999f;
     }
   }
   pt.mark(1, lane);
@@ -962,7 +999,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
       const int32_t a = min(max(center + o, a0), a1);
       const float4 an = A[a];
       const float dx = qx - an.x, dy = qy - an.y, dz = qz - an.z;
-      best = fminf(best, an.w + sqrtf(dx * dx + dy * dy + dz * dz));
+      best = fminf(best, an.w + sqrt_up(dx * dx + dy * dy + dz * dz));
     }
     best *= 1.00002f;
     r0sq = best * best * 1.00002f + 1e-30f;
@@ -985,7 +1022,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
   if (k_eff > 0 && n_act > 0) {
     // ---- runs of nearby lanes ------------------------------------------------
     const float scale = all ? INFINITY
-                            : wave_min(active ? sqrtf(cull_bound(r0sq)) : INFINITY);
+                            : wave_min(active ? sqrt_up(cull_bound(r0sq)) : INFINITY);
     const float thr = split * scale;
     const float px = __shfl_up(qx, 1, kWave);
     const float py = __shfl_up(qy, 1, kWave);
@@ -1013,7 +1050,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
       const float cx = lx + 0.5f * (hx - lx);
       const float cy = ly + 0.5f * (hy - ly);
       const float cz = lz + 0.5f * (hz - lz);
-      const float dq = part ? sqrtf(d2_ref(qx, qy, qz, cx, cy, cz)) * 1.00002f + 1e-30f : 0.0f;
+      const float dq = part ? sqrt_up(d2_ref(qx, qy, qz, cx, cy, cz)) * 1.00002f + 1e-30f : 0.0f;
       const float dmax = wave_max(dq);
 
       // ---- selection state and candidate list of the run --------------------
@@ -1024,7 +1061,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
       uint32_t nr = 0;
       if (reuse) {
         st.klo = tau_in + 1; st.khi = tau_in + 1; st.need = 0; st.cnt = 0;
-        const float xs = part ? sqrtf(cull_bound(r0sq)) * 1.00002f + dq : 0.0f;
+        const float xs = part ? sqrt_up(cull_bound(r0sq)) * 1.00002f + dq : 0.0f;
         const float rs = wave_max(xs) * 1.0002f;
         nr = build_list(D, cx, cy, cz, rs * rs + 1e-30f, sh, lane, &total,
                         count_stats ? &counters[kStatOverflow] : nullptr);
@@ -1303,7 +1340,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
   if (MODE == kModeAdjoint || !active) return;
   const float r2 = (k_eff > 0) ? __uint_as_float(static_cast<uint32_t>(tau >> 32)) : 0.0f;
   if (MODE == kModeAnchor) {
-    D.anchor_out[4 * qi + 3] = sqrtf(r2) * 1.00002f + 1e-30f;
+    D.anchor_out[4 * qi + 3] = sqrt_up(r2) * 1.00002f + 1e-30f;
     return;
   }
   if (D.out_r2) D.out_r2[qi] = r2;

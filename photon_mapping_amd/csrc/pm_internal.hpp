@@ -174,7 +174,7 @@ struct Context {
   int anchor_stride = 8;
   int leaf_size = 8;
   int xcd_aware = 1;
-  float run_split = 1.0f;
+  float run_split = 0.75f;
   int timing = 0;
   int count_stats = 0;
   pm_stats stats{};

@@ -196,6 +196,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 64-instance batching measurement")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner run (reported beside the default setting)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 1 GiB read + 1 GiB write kernel (torch.neg) three "
                          "times, the calibration of the HBM traffic counters")
@@ -333,6 +335,17 @@ def main() -> int:
         batched = batched_bench(ctx, dev)
         log(f"batched: {batched}")
 
+    tuned = None
+    if not args.no_autotune:
+        # on a context of its own, after every measurement above
+        tctx = pm.Context(0)
+        tmap = pm.PhotonMap(tctx)
+        tmap.set_photons_soa(pos, wi, flux)
+        tmap.build()
+        tuned = pm.autotune(tmap, q, mats, k, n_emit)
+        log(f"autotune: {tuned}")
+        del tmap
+
     if args.copy_calib:
         a = torch.ones((1 << 28,), dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
@@ -390,7 +403,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "detail": {"sort_ms": sort_ms, "build_ms": build_ms, **work, **(adj or {}),
-                   "batched": batched},
+                   "batched": batched, "autotune": tuned},
     }
     print(json.dumps(out), flush=True)
     return 0

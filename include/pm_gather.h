@@ -216,6 +216,25 @@ int pm_plan_create(pm_context* ctx, int n_inst, const pm_instance* inst,
 int pm_plan_launch(pm_plan* plan, void* stream);
 int pm_plan_destroy(pm_plan* plan);
 
+/* ---- autotuner (no reference counterpart) ----------------------------------------
+ * Times pm_gather_radiance on the caller's workload (device pointers) under a
+ * small grid of settings, coordinate by coordinate: leaf size {8, 16, 32}
+ * (the map is rebuilt), query-run split factor {0.5 .. 1.25} and persistent
+ * waves per CU {4, 6, 8, 12}.  The fastest setting stays in force (map leaf
+ * size, context options); results do not depend on it.  Synchronises. */
+typedef struct pm_tune_report {
+  int32_t leaf_size;    /* chosen leaf size                                  */
+  int32_t waves_per_cu; /* chosen persistent waves per CU                    */
+  double run_split;     /* chosen query-run split factor                     */
+  double best_ms;       /* query sort + gather time of the chosen setting    */
+  double initial_ms;    /* the same with the settings found on entry         */
+  int32_t n_tried;      /* settings timed                                    */
+  int32_t pad_;
+} pm_tune_report;
+int pm_autotune(pm_photonmap* map, int64_t m, const pm_query* q,
+                const pm_material* mats, int n_mats, int k, double n_emit,
+                void* stream, pm_tune_report* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,8 +6,8 @@ the torch autograd wrapper of the gather.
 """
 
 from ._lib import PmError  # noqa: F401
-from .photon_map import (BatchedPlan, Context, Photon, PhotonMap,  # noqa: F401
+from .photon_map import (BatchedPlan, Context, Photon, PhotonMap, autotune,  # noqa: F401
                          gather_batched, make_materials, make_queries)
 
-__all__ = ["BatchedPlan", "Context", "Photon", "PhotonMap", "PmError", "gather_batched",
-           "make_materials", "make_queries"]
+__all__ = ["BatchedPlan", "Context", "Photon", "PhotonMap", "PmError", "autotune",
+           "gather_batched", "make_materials", "make_queries"]

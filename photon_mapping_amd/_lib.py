@@ -65,6 +65,12 @@ class pm_stats(C.Structure):
                 ("phase_cycles", C.c_int64 * 8)]
 
 
+class pm_tune_report(C.Structure):
+    _fields_ = [("leaf_size", C.c_int32), ("waves_per_cu", C.c_int32), ("run_split", C.c_double),
+                ("best_ms", C.c_double), ("initial_ms", C.c_double), ("n_tried", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
 class pm_instance(C.Structure):
     _fields_ = [("map", C.c_void_p), ("m", C.c_int64), ("q", C.c_void_p),
                 ("mats", C.c_void_p), ("n_mats", C.c_int32), ("k", C.c_int32),
@@ -104,6 +110,7 @@ SIGNATURES = {
     "pm_plan_create": (None, [VP, I32, VP, VP, VP]),
     "pm_plan_launch": (None, [VP, VP]),
     "pm_plan_destroy": (None, [VP]),
+    "pm_autotune": (None, [VP, I64, VP, VP, I32, I32, C.c_double, VP, VP]),
     "pmx_fg_queries": (None, [VP, VP, I32, VP, I32, I32, I32, I32, C.c_uint64, VP, VP]),
     "pmx_surrogate_photons": (None, [VP, VP, I32, VP, I32, VP, I64, C.c_uint64, VP, VP,
                                      VP]),

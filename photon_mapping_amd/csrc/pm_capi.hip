@@ -886,4 +886,78 @@ int pm_plan_destroy(pm_plan* h) {
   return PM_OK;
 }
 
+int pm_autotune(pm_photonmap* h, int64_t m, const pm_query* q, const pm_material* mats,
+                int n_mats, int k, double n_emit, void* stream, pm_tune_report* out) {
+  if (int rc = check_map(h)) return rc;
+  if (!out) return fail(PM_ERR_INVALID_ARG, "null report");
+  if (m <= 0 || !q || k < 1) return fail(PM_ERR_INVALID_ARG, "empty or invalid workload");
+  auto* map = reinterpret_cast<PhotonMapImpl*>(h);
+  Context* ctx = map->ctx;
+  hipStream_t s = as_stream(stream);
+  DevBuf rgb;
+  if (int rc = rgb.reserve(static_cast<size_t>(m) * 3 * sizeof(float))) return rc;
+  const int timing0 = ctx->timing, stats0 = ctx->count_stats;
+  ctx->timing = 1;
+  ctx->count_stats = 0;
+  int tried = 0;
+  // best of two timed calls after one warm-up (anchors, workspaces)
+  auto time_it = [&](double* ms) -> int {
+    double best = 1e30;
+    for (int rep = 0; rep < 3; ++rep) {
+      if (int rc = pm_gather_radiance(h, m, q, mats, n_mats, k, n_emit, rgb.as<float>(), nullptr,
+                                      nullptr, PM_MEM_DEVICE, stream))
+        return rc;
+      if (rep > 0) best = std::min(best, ctx->stats.sort_ms + ctx->stats.gather_ms);
+    }
+    *ms = best;
+    ++tried;
+    return PM_OK;
+  };
+  auto restore = [&](int rc) {
+    ctx->timing = timing0;
+    ctx->count_stats = stats0;
+    return rc;
+  };
+  double initial = 0.0;
+  if (int rc = time_it(&initial)) return restore(rc);
+  double best = initial;
+  int best_leaf = map->leaf_size;
+  for (int leaf : {8, 16, 32}) {
+    if (leaf == best_leaf) continue;
+    if (int rc = pm_photonmap_build(h, leaf, stream)) return restore(rc);
+    double t = 0.0;
+    if (int rc = time_it(&t)) return restore(rc);
+    if (t < best) { best = t; best_leaf = leaf; }
+  }
+  if (map->leaf_size != best_leaf)
+    if (int rc = pm_photonmap_build(h, best_leaf, stream)) return restore(rc);
+  float best_split = ctx->run_split;
+  for (float sp : {0.5f, 0.6f, 0.75f, 0.9f, 1.0f, 1.25f}) {
+    if (sp == best_split) continue;
+    ctx->run_split = sp;
+    double t = 0.0;
+    if (int rc = time_it(&t)) return restore(rc);
+    if (t < best) { best = t; best_split = sp; }
+  }
+  ctx->run_split = best_split;
+  int best_waves = ctx->waves_per_cu;
+  for (int w : {4, 6, 8, 12}) {
+    if (w == best_waves) continue;
+    ctx->waves_per_cu = w;
+    double t = 0.0;
+    if (int rc = time_it(&t)) return restore(rc);
+    if (t < best) { best = t; best_waves = w; }
+  }
+  ctx->waves_per_cu = best_waves;
+  PM_HIP(hipStreamSynchronize(s));
+  out->leaf_size = best_leaf;
+  out->waves_per_cu = best_waves;
+  out->run_split = best_split;
+  out->best_ms = best;
+  out->initial_ms = initial;
+  out->n_tried = tried;
+  out->pad_ = 0;
+  return restore(PM_OK);
+}
+
 }  // extern "C"

@@ -296,6 +296,19 @@ class PhotonMap:
             pass
 
 
+def autotune(pmap: PhotonMap, queries, materials, k: int, n_emit: float) -> dict:
+    """pm_autotune on a device workload (torch tensors): picks the leaf size
+    (rebuilding the map), the query-run split factor and the persistent waves
+    per CU of the map's context by timing the gather; returns the report."""
+    import torch
+
+    mats = torch.as_tensor(materials, device=queries.device)
+    rep = _lib.pm_tune_report()
+    call("pm_autotune", pmap.handle, int(queries.shape[0]), ptr(queries), ptr(mats),
+         int(mats.shape[0]), int(k), float(n_emit), _stream_of(queries), C.byref(rep))
+    return {f: getattr(rep, f) for f, _ in rep._fields_ if f != "pad_"}
+
+
 def gather_batched(instances, ctx: Context | None = None, want_key: bool = True):
     """Many independent radiance gathers in one persistent launch
     (pm_gather_batched).  instances: sequence of (PhotonMap, queries (m, 10)

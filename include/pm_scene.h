@@ -52,6 +52,27 @@ int pmx_fg_queries(pm_context* ctx, const pmx_triangle* tris, int n_tri,
                    const pmx_camera* cam, int width, int height, int spp,
                    int depth, uint64_t seed, pm_query* out, int32_t* fg_mat_opt);
 
+/* Photon tracing of include/integrator.h:393-469 over the triangle scene:
+ * each path leaves a light triangle (uniform choice, uniform point, cosine
+ * direction; initial flux Le * n_lights * area * pi), stores a photon
+ * (flux, position, -direction) at every Lambert hit, continues with Russian
+ * roulette from its second bounce (p = min(max flux component, 1)) and
+ * bounces as its material samples it (Lambert: cosine; mirror; glass with
+ * Schlick Fresnel, include/material.h:73-245), with the light-leak cosine
+ * term of include/integrator.h:25-46.  Paths are numbered from 0 with a
+ * counter-based RNG and their photons concatenated in (path, bounce) order
+ * until n_store photons are stored; *n_emit receives the number of paths
+ * emitted (the estimate's divisor).  Triangles with is_light = 1 are the
+ * lights, all with radiance light_le.  mats (n_mat records) and mat_ior
+ * (n_mat floats, may be NULL: 1.5) are host memory; the SoA outputs (3 *
+ * n_store floats each) are device memory.  Runs on the null stream and
+ * synchronises. */
+int pmx_trace_photons(pm_context* ctx, const pmx_triangle* tris, int n_tri,
+                      const pm_material* mats, const float* mat_ior, int n_mat,
+                      const float* light_le, int64_t n_store, int max_depth,
+                      uint64_t seed, float* pos_soa, float* wi_soa, float* flux_soa,
+                      int64_t* n_emit);
+
 /* Surrogate photon array: n photons uniform by area over the triangles,
  * wi cosine-distributed about the face normal, flux = power * albedo of the
  * face's material * (0.5 + 0.5 u).  tris, mat_albedo (n_mat*3 floats) and

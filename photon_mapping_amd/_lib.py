@@ -112,6 +112,8 @@ SIGNATURES = {
     "pm_plan_destroy": (None, [VP]),
     "pm_autotune": (None, [VP, I64, VP, VP, I32, I32, C.c_double, VP, VP]),
     "pmx_fg_queries": (None, [VP, VP, I32, VP, I32, I32, I32, I32, C.c_uint64, VP, VP]),
+    "pmx_trace_photons": (None, [VP, VP, I32, VP, VP, I32, VP, I64, I32, C.c_uint64, VP, VP,
+                                 VP, VP]),
     "pmx_surrogate_photons": (None, [VP, VP, I32, VP, I32, VP, I64, C.c_uint64, VP, VP,
                                      VP]),
 }

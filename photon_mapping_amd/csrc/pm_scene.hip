@@ -1,10 +1,11 @@
-// Synthetic-workload producers: camera/final-gathering query generation and a
-// surrogate photon array over a small triangle scene (declared in
-// include/pm_scene.h).  Brute-force closest hit over all triangles: the
-// scenes these serve have a few dozen triangles.
+// Workload producers over a small triangle scene (declared in
+// include/pm_scene.h): camera/final-gathering query generation, photon
+// tracing, and a surrogate photon array.  Brute-force closest hit over all
+// triangles: the scenes these serve have a few dozen triangles.
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -187,12 +188,220 @@ __global__ __launch_bounds__(kThreads) void surrogate_photons_kernel(
   flux[2 * n + i] = pb * albedo[3 * m + 2] * s;
 }
 
+// ---- photon tracing (include/integrator.h:393-469) -----------------------------
+
+struct TraceScene {
+  const pmx_triangle* tris;
+  int n_tri;
+  const pm_material* mats;
+  const float* ior;  // per material (dielectric), may be null (1.5)
+  int n_mat;
+  const int32_t* lights;  // light triangle indices
+  int n_lights;
+  float le[3];
+};
+
+__device__ __forceinline__ V3 geometric_normal(const pmx_triangle& tr) {
+  const V3 p0 = ld3(tr.v[0]);
+  return normalize(cross(ld3(tr.v[1]) - p0, ld3(tr.v[2]) - p0));
+}
+
+// Cosine term of a photon bounce (include/integrator.h:25-46, FROM_LIGHT):
+// zero when the shading and geometric normals disagree about a side.
+__device__ __forceinline__ float cos_term_light(V3 wo, V3 wi, V3 ns, V3 ng) {
+  const float wi_ns = dot(wi, ns), wi_ng = dot(wi, ng);
+  const float wo_ns = dot(wo, ns), wo_ng = dot(wo, ng);
+  if (wi_ng * wi_ns <= 0.0f || wo_ng * wo_ns <= 0.0f) return 0.0f;
+  return fabsf(wo_ns) * fabsf(wi_ng) / fabsf(wo_ng);
+}
+
+// Schlick Fresnel reflectance (include/material.h:43-48).
+__device__ __forceinline__ float schlick(float cos_i, float eta_i, float eta_t) {
+  const float f0 = (eta_i - eta_t) * (eta_i - eta_t) / ((eta_i + eta_t) * (eta_i + eta_t));
+  const float c = fmaxf(1.0f - fabsf(cos_i), 0.0f);
+  return f0 + (1.0f - f0) * c * c * c * c * c;
+}
+
+// One photon path per thread.  COUNT: photons stored per path into counts;
+// otherwise the photons are written at offsets (original photon order =
+// (path, bounce) order), those at or past n_store dropped.
+template <bool COUNT>
+__global__ __launch_bounds__(kThreads) void trace_kernel(
+    TraceScene S, int64_t path0, int64_t n_paths, uint64_t seed, int max_depth,
+    uint32_t* __restrict__ counts, const int64_t* __restrict__ offsets, int64_t n_store,
+    float* __restrict__ pos, float* __restrict__ wi, float* __restrict__ flux) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n_paths) return;
+  const uint64_t path = static_cast<uint64_t>(path0 + i);
+  uint32_t dim = 0;
+  // light, point and direction (include/integrator.h:265-288, include/light.h:27-44)
+  int li = static_cast<int>(rnd(seed, path, dim++) * S.n_lights);
+  li = li < S.n_lights - 1 ? li : S.n_lights - 1;
+  const pmx_triangle& lt = S.tris[S.lights[li]];
+  const V3 l0 = ld3(lt.v[0]);
+  const V3 le1 = ld3(lt.v[1]) - l0, le2 = ld3(lt.v[2]) - l0;
+  const V3 lc = cross(le1, le2);
+  const float area = 0.5f * sqrtf(dot(lc, lc));
+  const float su0 = sqrtf(rnd(seed, path, dim++));
+  const float b0 = 1.0f - su0, b1 = rnd(seed, path, dim++) * su0;
+  V3 o = l0 * (1.0f - b0 - b1) + ld3(lt.v[1]) * b0 + ld3(lt.v[2]) * b1;
+  const V3 ln = normalize(shading_normal(lt, b0, b1));
+  V3 t, b;
+  basis(ln, t, b);
+  V3 d = cosine_dir(rnd(seed, path, dim), rnd(seed, path, dim + 1), t, ln, b);
+  dim += 2;
+  // Le / (choose pdf * point pdf * direction pdf) * cos = Le * n_lights * area * pi
+  const float w0 = static_cast<float>(S.n_lights) * area * kPiF;
+  float tr_r = S.le[0] * w0, tr_g = S.le[1] * w0, tr_b = S.le[2] * w0;
+  uint32_t stored = 0;
+  for (int k = 0; k < max_depth; ++k) {
+    if (!(tr_r >= 0.0f && tr_g >= 0.0f && tr_b >= 0.0f)) break;  // NaN or negative
+    const Hit h = closest_hit(S.tris, S.n_tri, o, d, INFINITY);
+    if (h.tri < 0) break;
+    const pmx_triangle& tri = S.tris[h.tri];
+    const V3 x = o + d * h.t;
+    const int m = (tri.material >= 0 && tri.material < S.n_mat) ? tri.material : 0;
+    const int type = S.mats[m].type;
+    const V3 wo = d * -1.0f;
+    if (type == PM_MATERIAL_LAMBERT) {
+      if (!COUNT) {
+        const int64_t at = offsets[i] + static_cast<int64_t>(stored);
+        if (at < n_store) {
+          pos[at] = x.x; pos[n_store + at] = x.y; pos[2 * n_store + at] = x.z;
+          wi[at] = wo.x; wi[n_store + at] = wo.y; wi[2 * n_store + at] = wo.z;
+          flux[at] = tr_r; flux[n_store + at] = tr_g; flux[2 * n_store + at] = tr_b;
+        }
+      }
+      ++stored;
+    }
+    // Russian roulette from the second bounce
+    if (k > 0) {
+      const float p_rr = fminf(fmaxf(tr_r, fmaxf(tr_g, tr_b)), 1.0f);
+      if (rnd(seed, path, dim++) >= p_rr) break;
+      tr_r /= p_rr; tr_g /= p_rr; tr_b /= p_rr;
+    }
+    // next direction in the shading frame (include/material.h:73-245)
+    const V3 ns = normalize(shading_normal(tri, h.u, h.v));
+    const V3 ng = geometric_normal(tri);
+    V3 ft, fb;
+    basis(ns, ft, fb);
+    const V3 wol = v3(dot(wo, ft), dot(wo, ns), dot(wo, fb));
+    V3 wil;
+    float scale = 0.0f;  // f / pdf (the albedo is applied below)
+    if (type == PM_MATERIAL_LAMBERT) {
+      const V3 cd = cosine_dir(rnd(seed, path, dim), rnd(seed, path, dim + 1), v3(1.f, 0.f, 0.f),
+                               v3(0.f, 1.f, 0.f), v3(0.f, 0.f, 1.f));
+      dim += 2;
+      wil = cd;
+      scale = (wol.y < 0.0f || wil.y < 0.0f || wil.y <= 0.0f) ? 0.0f : 1.0f / wil.y;
+    } else if (type == PM_MATERIAL_MIRROR) {
+      wil = v3(-wol.x, wol.y, -wol.z);
+      scale = 1.0f / fabsf(wil.y);
+    } else {
+      const float eta = S.ior ? S.ior[m] : 1.5f;
+      const bool outside = wol.y > 0.0f;
+      const float eta_o = outside ? 1.0f : eta, eta_i = outside ? eta : 1.0f;
+      const V3 n = outside ? v3(0.f, 1.f, 0.f) : v3(0.f, -1.f, 0.f);
+      const float cos_o = dot(wol, n);
+      const float fr = schlick(cos_o, eta_o, eta_i);
+      const V3 refl = wol * -1.0f + n * (2.0f * cos_o);
+      wil = refl;
+      if (rnd(seed, path, dim++) >= fr) {
+        const V3 th = (wol - n * cos_o) * (-eta_o / eta_i);
+        const float th2 = dot(th, th);
+        if (th2 <= 1.0f) wil = th + n * (-sqrtf(fmaxf(1.0f - th2, 0.0f)));
+      }
+      scale = 1.0f / fabsf(wil.y);
+    }
+    const V3 wiw = ft * wil.x + ns * wil.y + fb * wil.z;
+    // Lambert: f = rho / pi, pdf = cos / pi; specular: f = rho / |cos|, pdf = 1
+    const float c = cos_term_light(wo, wiw, ns, ng) * scale;
+    const pm_material& mt = S.mats[m];
+    tr_r *= mt.rho[0] * c;
+    tr_g *= mt.rho[1] * c;
+    tr_b *= mt.rho[2] * c;
+    o = x;
+    d = wiw;
+  }
+  if (COUNT) counts[i] = stored;
+}
+
 }  // namespace
 }  // namespace pm
 
 using namespace pm;
 
 extern "C" {
+
+int pmx_trace_photons(pm_context* h, const pmx_triangle* tris, int n_tri,
+                      const pm_material* mats, const float* mat_ior, int n_mat,
+                      const float* light_le, int64_t n_store, int max_depth, uint64_t seed,
+                      float* pos_soa, float* wi_soa, float* flux_soa, int64_t* n_emit) {
+  if (!h || !tris || !mats || !light_le || !pos_soa || !wi_soa || !flux_soa || !n_emit)
+    return fail(PM_ERR_INVALID_ARG, "null argument");
+  if (n_tri < 1 || n_tri > kMaxSceneTris || n_mat < 1 || n_store < 1 || max_depth < 1)
+    return fail(PM_ERR_INVALID_ARG, "bad sizes");
+  auto* ctx = reinterpret_cast<Context*>(h);
+  PM_HIP(hipSetDevice(ctx->device));
+  std::vector<int32_t> lights;
+  for (int i = 0; i < n_tri; ++i)
+    if (tris[i].is_light) lights.push_back(i);
+  if (lights.empty()) return fail(PM_ERR_INVALID_ARG, "the scene has no light triangle");
+  DevBuf dt, dm, di, dl, dcount, doff;
+  if (int rc = dt.reserve(n_tri * sizeof(pmx_triangle))) return rc;
+  if (int rc = dm.reserve(n_mat * sizeof(pm_material))) return rc;
+  if (int rc = dl.reserve(lights.size() * sizeof(int32_t))) return rc;
+  PM_HIP(hipMemcpy(dt.ptr, tris, n_tri * sizeof(pmx_triangle), hipMemcpyHostToDevice));
+  PM_HIP(hipMemcpy(dm.ptr, mats, n_mat * sizeof(pm_material), hipMemcpyHostToDevice));
+  PM_HIP(hipMemcpy(dl.ptr, lights.data(), lights.size() * sizeof(int32_t),
+                   hipMemcpyHostToDevice));
+  if (mat_ior) {
+    if (int rc = di.reserve(n_mat * sizeof(float))) return rc;
+    PM_HIP(hipMemcpy(di.ptr, mat_ior, n_mat * sizeof(float), hipMemcpyHostToDevice));
+  }
+  TraceScene S{dt.as<pmx_triangle>(), n_tri, dm.as<pm_material>(),
+               mat_ior ? di.as<float>() : nullptr, n_mat, dl.as<int32_t>(),
+               static_cast<int>(lights.size()), {light_le[0], light_le[1], light_le[2]}};
+  // pass 1: photons stored per path, chunk by chunk until n_store are
+  // reached; after a pilot chunk, each chunk is sized from the photons per
+  // path seen so far
+  constexpr int64_t kMinChunk = int64_t{1} << 16;
+  constexpr int64_t kMaxChunk = int64_t{1} << 22;
+  if (int rc = dcount.reserve(kMaxChunk * sizeof(uint32_t))) return rc;
+  std::vector<int64_t> offs;  // exclusive prefix per path
+  int64_t acc = 0, paths = 0;
+  int64_t chunk = kMinChunk;
+  std::vector<uint32_t> cnt(kMaxChunk);
+  while (acc < n_store) {
+    if (paths > (int64_t{1} << 40)) return fail(PM_ERR_INVALID_ARG, "no photon is ever stored");
+    hipLaunchKernelGGL(trace_kernel<true>, dim3((chunk + kThreads - 1) / kThreads),
+                       dim3(kThreads), 0, nullptr, S, paths, chunk, seed, max_depth,
+                       dcount.as<uint32_t>(), nullptr, int64_t{0}, nullptr, nullptr, nullptr);
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipMemcpy(cnt.data(), dcount.ptr, chunk * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < chunk && acc < n_store; ++j) {
+      offs.push_back(acc);
+      acc += cnt[j];
+      ++paths;
+    }
+    if (acc < n_store) {
+      const double per_path = std::max(static_cast<double>(acc) / static_cast<double>(paths), 1e-3);
+      const double want = 1.1 * static_cast<double>(n_store - acc) / per_path + 1024.0;
+      chunk = std::min<int64_t>(kMaxChunk, std::max<int64_t>(kMinChunk, static_cast<int64_t>(want)));
+    }
+  }
+  // the path holding photon n_store - 1 is the last one emitted
+  *n_emit = paths;
+  // pass 2: the same paths again, writing their photons in (path, bounce) order
+  if (int rc = doff.reserve(offs.size() * sizeof(int64_t))) return rc;
+  PM_HIP(hipMemcpy(doff.ptr, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(trace_kernel<false>, dim3((paths + kThreads - 1) / kThreads),
+                     dim3(kThreads), 0, nullptr, S, int64_t{0}, paths, seed, max_depth, nullptr,
+                     doff.as<int64_t>(), n_store, pos_soa, wi_soa, flux_soa);
+  PM_HIP(hipGetLastError());
+  PM_HIP(hipDeviceSynchronize());
+  return PM_OK;
+}
 
 int pmx_fg_queries(pm_context* h, const pmx_triangle* tris, int n_tri, const pmx_camera* cam,
                    int width, int height, int spp, int depth, uint64_t seed, pm_query* out,

@@ -167,6 +167,27 @@ def surrogate_photons(scene: Scene, n: int, seed: int = 0, device: str = "cuda")
     return tuple(a.t() for a in soa)
 
 
+def trace_photons(scene: Scene, n: int, seed: int = 0, max_depth: int = 100,
+                  device: str = "cuda"):
+    """Traced photon map (pmx_trace_photons): exactly n photons stored in
+    (path, bounce) order.  Returns (pos, wi, flux) as (n, 3) float32 torch
+    tensors on the device and the number of emitted paths (the estimate's
+    divisor)."""
+    import torch
+    from .photon_map import Context
+
+    ctx = Context.default()
+    soa = [torch.empty((3, n), dtype=torch.float32, device=device) for _ in range(3)]
+    tris = np.ascontiguousarray(scene.tris)
+    mats = np.ascontiguousarray(scene.materials())
+    le = np.ascontiguousarray(scene.light_le, np.float32)
+    n_emit = C.c_int64()
+    call("pmx_trace_photons", ctx.handle, ptr(tris), len(tris), ptr(mats), None, len(mats),
+         ptr(le), int(n), int(max_depth), int(seed), ptr(soa[0]), ptr(soa[1]), ptr(soa[2]),
+         C.byref(n_emit))
+    return tuple(a.t() for a in soa) + (int(n_emit.value),)
+
+
 def camera_queries(scene: Scene, width: int, height: int, spp: int, depth: int,
                    seed: int = 0, device: str = "cuda", want_fg_mat: bool = False):
     """Packed (m, 10) query records on the device (see make_queries), m =

@@ -2,9 +2,12 @@
 """Benchmark: kNN photon-gather queries/s (k = 100, 1,000,000 photons).
 
 Workload (the GPU configuration of the box scene): 1,000,000 stored photons
-(surrogate photon array over the box scene, seed 0), k = 100, and the
-final-gathering secondary-hit queries of a 512 x 512 camera at 64 samples per
-pixel, i.e. 16,777,216 gather queries per step.  A step is one
+traced from the box scene's light (paths in index order, seed 0, photons of
+the paths concatenated until exactly 1,000,000 are stored; the number of
+paths emitted is the estimate's divisor), k = 100, and the final-gathering
+secondary-hit queries of a 512 x 512 camera at 64 samples per pixel, i.e.
+16,777,216 gather queries per step.  --photon-source surrogate uses an
+area-uniform surrogate array of the same size instead.  A step is one
 pm_gather_radiance call over all queries with inputs resident in HBM: Morton
 keys + radix sort of the queries, group anchors, and the persistent gather
 kernel.  Prints one JSON line.
@@ -189,6 +192,7 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=512)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--leaf", type=int, default=0)
+    ap.add_argument("--photon-source", choices=["traced", "surrogate"], default="traced")
     ap.add_argument("--split", type=float, default=0.0,
                     help="query-run split factor (0: library default)")
     ap.add_argument("--waves", type=int, default=0, help="persistent waves per CU (0: default)")
@@ -223,12 +227,30 @@ def main() -> int:
 
     s = sc.box_scene()
     t0 = time.perf_counter()
-    pos, wi, flux = sc.surrogate_photons(s, args.photons, seed=0)
+    if args.photon_source == "traced":
+        pos, wi, flux, n_paths = sc.trace_photons(s, args.photons, seed=0)
+        n_emit = float(n_paths)
+    else:
+        pos, wi, flux = sc.surrogate_photons(s, args.photons, seed=0)
+        n_emit = float(args.photons)
+    t_photons = time.perf_counter() - t0
+    tracing = None
+    if args.photon_source == "traced":
+        # photons traced per second: the same 1M-photon map again, warm
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sc.trace_photons(s, args.photons, seed=0)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        tracing = {"ms": 1e3 * dt, "stored_photons_per_s": args.photons / dt,
+                   "paths_per_s": n_emit / dt, "emitted_paths": int(n_emit),
+                   "note": "two passes (count, write) over the same counter-based paths; "
+                           "brute-force closest hit over the 36 triangles"}
     q = sc.camera_queries(s, args.width, args.height, args.spp, depth=1, seed=0)
     torch.cuda.synchronize()
-    log(f"inputs: {args.photons} photons, {q.shape[0]} queries in {time.perf_counter() - t0:.2f} s")
+    log(f"inputs: {args.photons} {args.photon_source} photons (divisor {n_emit:.0f}) in "
+        f"{t_photons:.2f} s, {q.shape[0]} queries in {time.perf_counter() - t0:.2f} s")
     mats = torch.from_numpy(s.materials()).to(dev)
-    n_emit = float(args.photons)
     m = q.shape[0]
     k = args.k
 
@@ -381,10 +403,12 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": (f"box scene, {args.photons} stored photons (surrogate array), k={k}, "
+            "workload": (f"box scene, {args.photons} stored photons ({args.photon_source}), k={k}, "
                          f"final-gathering secondary-hit queries {args.width}x{args.height}x"
                          f"{args.spp} spp"),
             "photons": args.photons,
+            "photon_source": args.photon_source,
+            "emitted_paths": int(n_emit) if args.photon_source == "traced" else None,
             "queries": m,
             "k": k,
             "leaf_size": args.leaf or 8,
@@ -403,7 +427,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "detail": {"sort_ms": sort_ms, "build_ms": build_ms, **work, **(adj or {}),
-                   "batched": batched, "autotune": tuned},
+                   "batched": batched, "autotune": tuned, "tracing": tracing},
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -100,11 +100,11 @@ def batched_bench(ctx, dev, n_inst=64, frames=3, seed=0):
     """Config-4 batching workload: n_inst box-scene instances of uneven size
     (wall albedos uniform in [0.1, 0.9]; stored photons log-uniform in
     [1e4, 4e6]; k from {32, 64, 100, 200}; square resolution from {128, 256,
-    512, 1024}; 16 spp final-gathering queries), timed three ways per frame:
-    one plain pm_gather_radiance launch per instance, one pm_gather_batched
-    launch, and the prepared plan replayed as a captured HIP graph.  The
-    spheres of the config are left out (the query producer casts rays by
-    brute force over the box's triangles)."""
+    512, 1024}; 0-3 mirror or glass spheres; photons traced from the light
+    (the emitted paths are the divisor); 16 spp final-gathering queries),
+    timed three ways per frame: one plain pm_gather_radiance launch per
+    instance, one pm_gather_batched launch, and the prepared plan replayed as
+    a captured HIP graph."""
     import numpy as np
     import torch
 
@@ -114,17 +114,18 @@ def batched_bench(ctx, dev, n_inst=64, frames=3, seed=0):
     rng = np.random.default_rng(seed)
     inst, sizes = [], []
     for i in range(n_inst):
-        s = sc.box_scene_instance(rng)
+        s = sc.batch_scene_instance(rng)
         n = int(round(math.exp(rng.uniform(math.log(1e4), math.log(4e6)))))
         k = int(rng.choice([32, 64, 100, 200]))
         res = int(rng.choice([128, 256, 512, 1024]))
-        pos, wi, flux = sc.surrogate_photons(s, n, seed=1000 + i)
-        q = sc.camera_queries(s, res, res, 16, depth=1, seed=1000 + i)
+        with sc.NativeScene(s, ctx) as ns:
+            pos, wi, flux, n_paths = sc.trace_photons(ns, n, seed=1000 + i)
+            q = sc.camera_queries(ns, res, res, 16, depth=1, seed=1000 + i)
         pmap = pm.PhotonMap(ctx)
         pmap.set_photons_soa(pos, wi, flux)
         pmap.build()
         del pos, wi, flux
-        inst.append((pmap, q, torch.from_numpy(s.materials()).to(dev), k, float(n)))
+        inst.append((pmap, q, torch.from_numpy(s.materials()).to(dev), k, float(n_paths)))
         sizes.append((n, k, res))
     total_q = sum(int(q.shape[0]) for _, q, *_ in inst)
     torch.cuda.synchronize()
@@ -181,6 +182,61 @@ def batched_bench(ctx, dev, n_inst=64, frames=3, seed=0):
     }
 
 
+def ray_bench(ctx, dev, seed=0):
+    """Closest-hit rays/s of the BVH (pmx_scene_intersect over 4M rays from
+    uniform points in the unit box, isotropic directions; HIP events on the
+    null stream) and caustic photons traced/s (1M stored), per scene: the box
+    (36 triangles), the caustics scene (two 5120-triangle icospheres) and the
+    water height field (131,084 triangles)."""
+    import torch
+
+    from photon_mapping_amd import _lib
+    from photon_mapping_amd import scene as sc
+
+    out = {}
+    n = 1 << 22
+    g = torch.Generator(device=dev).manual_seed(seed)
+    org = torch.rand((3, n), device=dev, generator=g)
+    d = torch.randn((3, n), device=dev, generator=g)
+    d /= d.norm(dim=0, keepdim=True)
+    tri = torch.empty((n,), dtype=torch.int32, device=dev)
+    t = torch.empty((n,), dtype=torch.float32, device=dev)
+    for name, s in (("box", sc.box_scene()), ("caustics", sc.caustics_scene()),
+                    ("water", sc.water_scene(0))):
+        with sc.NativeScene(s, ctx) as ns:
+            info = ns.info()
+
+            def shoot():
+                _lib.call("pmx_scene_intersect", ns.handle, n, _lib.ptr(org), _lib.ptr(d), 0,
+                          _lib.ptr(tri), _lib.ptr(t), None, None, None)
+
+            shoot()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 5
+            e0.record()
+            for _ in range(reps):
+                shoot()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            hit = float((tri >= 0).float().mean())
+            row = {"triangles": info["n_tri"], "bvh_nodes": info["bvh_nodes"],
+                   "bvh_depth": info["bvh_depth"], "sah_cost": round(info["sah_cost"], 2),
+                   "mrays_per_s": n / ms / 1e3, "hit_fraction": hit}
+            if name != "box":
+                sc.trace_photons(ns, 1 << 16, seed=0, kind="caustics")
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                _, _, _, paths = sc.trace_photons(ns, 1_000_000, seed=0, kind="caustics")
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                row["caustic_photons_per_s"] = 1e6 / dt
+                row["caustic_paths_per_s"] = paths / dt
+            out[name] = row
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +254,8 @@ def main() -> int:
     ap.add_argument("--waves", type=int, default=0, help="persistent waves per CU (0: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--no-rays", action="store_true",
+                    help="skip the BVH ray and caustic-tracing rates")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 64-instance batching measurement")
     ap.add_argument("--no-autotune", action="store_true",
@@ -227,8 +285,9 @@ def main() -> int:
 
     s = sc.box_scene()
     t0 = time.perf_counter()
+    ns = sc.NativeScene(s)
     if args.photon_source == "traced":
-        pos, wi, flux, n_paths = sc.trace_photons(s, args.photons, seed=0)
+        pos, wi, flux, n_paths = sc.trace_photons(ns, args.photons, seed=0)
         n_emit = float(n_paths)
     else:
         pos, wi, flux = sc.surrogate_photons(s, args.photons, seed=0)
@@ -239,14 +298,14 @@ def main() -> int:
         # photons traced per second: the same 1M-photon map again, warm
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        sc.trace_photons(s, args.photons, seed=0)
+        sc.trace_photons(ns, args.photons, seed=0)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t1
         tracing = {"ms": 1e3 * dt, "stored_photons_per_s": args.photons / dt,
                    "paths_per_s": n_emit / dt, "emitted_paths": int(n_emit),
                    "note": "two passes (count, write) over the same counter-based paths; "
-                           "brute-force closest hit over the 36 triangles"}
-    q = sc.camera_queries(s, args.width, args.height, args.spp, depth=1, seed=0)
+                           "BVH closest hit"}
+    q = sc.camera_queries(ns, args.width, args.height, args.spp, depth=1, seed=0)
     torch.cuda.synchronize()
     log(f"inputs: {args.photons} {args.photon_source} photons (divisor {n_emit:.0f}) in "
         f"{t_photons:.2f} s, {q.shape[0]} queries in {time.perf_counter() - t0:.2f} s")
@@ -356,6 +415,10 @@ def main() -> int:
     if not args.no_batched:
         batched = batched_bench(ctx, dev)
         log(f"batched: {batched}")
+    rays = None
+    if not args.no_rays:
+        rays = ray_bench(ctx, dev)
+        log(f"rays: {rays}")
 
     tuned = None
     if not args.no_autotune:
@@ -427,7 +490,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "detail": {"sort_ms": sort_ms, "build_ms": build_ms, **work, **(adj or {}),
-                   "batched": batched, "autotune": tuned, "tracing": tracing},
+                   "batched": batched, "autotune": tuned, "tracing": tracing, "rays": rays},
     }
     print(json.dumps(out), flush=True)
     return 0

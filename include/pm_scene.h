@@ -1,13 +1,13 @@
 /*
  * pm_scene.h -- synthetic-workload producers of libpm_gather (gfx950).
  *
- * These entry points generate the inputs of the gather on the device: the
- * query points of a camera pass over a small triangle scene, and a surrogate
- * photon array.  They stand in for the reference's Embree ray casting and
- * photon tracing (include/scene.h:307-348, include/integrator.h:393-469) for
- * benchmark and test workloads; they use a counter-based hash RNG, not the
- * reference's PCG32 stream, so they reproduce its distributions, not its
- * exact samples.
+ * These entry points build a triangle scene with a BVH on the device and
+ * generate the inputs of the gather from it: traced photon maps and the
+ * query points of a camera pass, plus a surrogate photon array.  They stand
+ * in for the reference's Embree ray casting and photon tracing
+ * (include/scene.h:275-348, include/integrator.h:393-563); they use a
+ * counter-based hash RNG, not the reference's PCG32 stream, so they
+ * reproduce its distributions, not its exact samples.
  */
 #ifndef PM_SCENE_H
 #define PM_SCENE_H
@@ -40,38 +40,93 @@ typedef struct pmx_camera {
   float focal;
 } pmx_camera;
 
+/* ---- scene: triangles, materials, lights and a BVH ----------------------- */
+
+/* A triangle scene on the device with its bounding-volume hierarchy.
+ * Replaces Scene::build / Scene::intersect (include/scene.h:275-348): the
+ * reference hands its triangles to Embree; here the library builds a binned
+ * surface-area-heuristic BVH on the host and traverses it in HIP kernels. */
+typedef struct pmx_scene pmx_scene;
+
+/* Copies the scene to the device and builds its BVH.  tris (n_tri records),
+ * mats (n_mat records, rho and type: include/material.h Lambert / Mirror /
+ * Glass), mat_ior (n_mat floats, may be NULL: 1.5 for every glass material)
+ * and light_le (3 floats: the radiance of every triangle with is_light = 1,
+ * include/light.h:17-45) are host memory.  Triangle materials outside
+ * [0, n_mat) read as material 0.  Synchronous. */
+int pmx_scene_create(pm_context* ctx, const pmx_triangle* tris, int n_tri,
+                     const pm_material* mats, const float* mat_ior, int n_mat,
+                     const float* light_le, pmx_scene** out_scene);
+int pmx_scene_destroy(pmx_scene* scene);
+
+typedef struct pmx_scene_info {
+  int32_t n_tri;
+  int32_t n_lights;
+  int32_t bvh_nodes;   /* internal nodes */
+  int32_t bvh_leaves;
+  int32_t bvh_depth;   /* deepest internal node, root = 0 */
+  int32_t reserved;
+  double sah_cost;     /* expected node + triangle tests per ray */
+  double light_area;   /* summed area of the light triangles */
+} pmx_scene_info;
+
+int pmx_scene_get_info(const pmx_scene* scene, pmx_scene_info* out);
+
+#define PMX_HIT_BVH 0          /* traverse the BVH */
+#define PMX_HIT_BRUTE_FORCE 1  /* test every triangle (same results, for checks) */
+
+/* Closest hits of n rays (Scene::intersect, include/scene.h:307-348): for
+ * ray i with origin org_soa[{0,n,2n}+i] and direction dir_soa[...], the
+ * triangle with the smallest t in (1e-5, inf) of a two-sided
+ * Moller-Trumbore test, ties to the lower triangle index; out_tri = -1 and
+ * out_t = inf on a miss.  out_u / out_v are the barycentrics of vertices 1
+ * and 2.  All arrays are device memory; asynchronous on the stream. */
+int pmx_scene_intersect(const pmx_scene* scene, int64_t n, const float* org_soa,
+                        const float* dir_soa, int method, int32_t* out_tri, float* out_t,
+                        float* out_u, float* out_v, void* stream);
+
+/* ---- workload producers over a scene -------------------------------------- */
+
 /* Generates width*height*spp gather queries, pixel-major then sample.
  * depth 0: the primary diffuse hit (include/integrator.h:305-308);
  * depth 1: the final-gathering secondary hit of a cosine-sampled bounce from
  * the primary hit (include/integrator.h:215-244).  Camera samples that miss,
- * or hit a light at the primary vertex, and bounces that escape, are redrawn.
- * tris and cam are host memory; out (m pm_query records) and fg_mat_opt
- * (m int32: material of the primary hit, may be NULL) are device memory.
- * Runs on the null stream and synchronises. */
-int pmx_fg_queries(pm_context* ctx, const pmx_triangle* tris, int n_tri,
-                   const pmx_camera* cam, int width, int height, int spp,
-                   int depth, uint64_t seed, pm_query* out, int32_t* fg_mat_opt);
+ * or hit a light or a non-Lambert surface at the primary vertex, and bounces
+ * that escape, are redrawn.  cam is host memory; out (m pm_query records)
+ * and fg_mat_opt (m int32: material of the primary hit, may be NULL) are
+ * device memory.  Runs on the null stream and synchronises. */
+int pmx_fg_queries(const pmx_scene* scene, const pmx_camera* cam, int width, int height,
+                   int spp, int depth, uint64_t seed, pm_query* out, int32_t* fg_mat_opt);
 
-/* Photon tracing of include/integrator.h:393-469 over the triangle scene:
- * each path leaves a light triangle (uniform choice, uniform point, cosine
- * direction; initial flux Le * n_lights * area * pi), stores a photon
- * (flux, position, -direction) at every Lambert hit, continues with Russian
- * roulette from its second bounce (p = min(max flux component, 1)) and
- * bounces as its material samples it (Lambert: cosine; mirror; glass with
- * Schlick Fresnel, include/material.h:73-245), with the light-leak cosine
- * term of include/integrator.h:25-46.  Paths are numbered from 0 with a
- * counter-based RNG and their photons concatenated in (path, bounce) order
- * until n_store photons are stored; *n_emit receives the number of paths
- * emitted (the estimate's divisor).  Triangles with is_light = 1 are the
- * lights, all with radiance light_le.  mats (n_mat records) and mat_ior
- * (n_mat floats, may be NULL: 1.5) are host memory; the SoA outputs (3 *
- * n_store floats each) are device memory.  Runs on the null stream and
+#define PMX_PHOTONS_GLOBAL 0    /* a photon at every Lambert hit */
+#define PMX_PHOTONS_CAUSTICS 1  /* only at the first Lambert hit after specular bounces */
+
+/* Photon tracing of include/integrator.h:393-563.  Each path leaves a light
+ * triangle (uniform choice, uniform point, cosine direction; initial flux
+ * Le * n_lights * area * pi), continues with Russian roulette from its
+ * second bounce (p = min(max flux component, 1)) and bounces as its
+ * material samples it (Lambert: cosine; mirror; glass with Schlick Fresnel,
+ * include/material.h:73-245), with the light-leak cosine term of
+ * include/integrator.h:25-46.  GLOBAL stores a photon (flux, position,
+ * -direction) at every Lambert hit (:431-438); CAUSTICS ends a path at its
+ * first Lambert hit and stores a photon there only if every earlier hit was
+ * specular (:496-512).  Paths are numbered from 0 with a counter-based RNG
+ * (the kind is part of the stream) and their photons concatenated in
+ * (path, bounce) order until n_store photons are stored; *n_emit receives
+ * the number of paths emitted (the estimate's divisor).  The SoA outputs
+ * (3 * n_store floats each) are device memory.  Runs on the null stream and
  * synchronises. */
-int pmx_trace_photons(pm_context* ctx, const pmx_triangle* tris, int n_tri,
-                      const pm_material* mats, const float* mat_ior, int n_mat,
-                      const float* light_le, int64_t n_store, int max_depth,
+int pmx_trace_photons(const pmx_scene* scene, int kind, int64_t n_store, int max_depth,
                       uint64_t seed, float* pos_soa, float* wi_soa, float* flux_soa,
                       int64_t* n_emit);
+
+/* The reference's own count semantics (PhotonMapping::build,
+ * include/integrator.h:393-563): exactly n_paths paths are emitted and every
+ * photon they store goes into a new photon map of the scene's context, in
+ * (path, bounce) order; *n_stored receives the number stored (possibly 0).
+ * The map is not built yet (pm_photonmap_build).  Synchronous. */
+int pmx_trace_photon_map(const pmx_scene* scene, int kind, int64_t n_paths, int max_depth,
+                         uint64_t seed, pm_photonmap** out_map, int64_t* n_stored);
 
 /* Surrogate photon array: n photons uniform by area over the triangles,
  * wi cosine-distributed about the face normal, flux = power * albedo of the

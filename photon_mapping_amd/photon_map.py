@@ -130,6 +130,14 @@ class PhotonMap:
         self._n = 0
         self._built = False
 
+    @classmethod
+    def _adopt(cls, ctx: Context, handle: C.c_void_p, n: int) -> "PhotonMap":
+        """Wraps a device photon map the library created (not built yet)."""
+        pmap = cls(ctx)
+        pmap._handle = handle
+        pmap._n = int(n)
+        return pmap
+
     # ---- reference-named API -------------------------------------------------
     def getNPhotons(self) -> int:
         return self._n if self._handle is not None else len(self._host)

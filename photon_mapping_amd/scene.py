@@ -14,8 +14,12 @@ Camera: the reference's pinhole (include/camera.h:21-47) with a 40 degree
 vertical field of view, pinhole at (0.5, 0.5, 2.3) looking down -z, so every
 camera ray enters the box through its open face.
 
-Surrogate photons (used until photon tracing runs on the device): uniform by
-area over all 36 triangles, wi cosine-distributed about the face normal,
+Also the caustics scene (two spheres), the water scene (a 131,072-triangle
+height field) and the batching instances with spheres.  Every scene is traced
+through NativeScene, the device copy with its BVH.
+
+Surrogate photons (a stand-in array for tests): uniform by area over all
+triangles, wi cosine-distributed about the face normal,
 flux = (Le * A_light * pi) * albedo * (0.5 + 0.5 u).
 """
 
@@ -40,6 +44,15 @@ class pmx_camera(C.Structure):
                 ("right", C.c_float * 3), ("up", C.c_float * 3), ("focal", C.c_float)]
 
 
+class pmx_scene_info(C.Structure):
+    _fields_ = [("n_tri", C.c_int32), ("n_lights", C.c_int32), ("bvh_nodes", C.c_int32),
+                ("bvh_leaves", C.c_int32), ("bvh_depth", C.c_int32), ("reserved", C.c_int32),
+                ("sah_cost", C.c_double), ("light_area", C.c_double)]
+
+
+assert C.sizeof(pmx_scene_info) == 40
+
+
 @dataclass
 class Scene:
     tris: np.ndarray                   # TRI_DTYPE records
@@ -48,6 +61,7 @@ class Scene:
     light_le: np.ndarray               # (3,)
     light_area: float
     camera: dict = field(default_factory=dict)
+    ior: np.ndarray | None = None     # (n_mat,) float32 for glass; None: 1.5
 
     @property
     def light_power(self) -> np.ndarray:
@@ -144,6 +158,146 @@ def box_scene_instance(rng) -> Scene:
     return s
 
 
+def _box_shell(with_inner_boxes: bool = True):
+    """Walls, light and (optionally) the two inner boxes of the box scene,
+    with materials 0 grey, 1 left, 2 right, 3 light."""
+    s = box_scene()
+    if not with_inner_boxes:
+        s.tris = s.tris[:12].copy()
+    return s
+
+
+def icosphere(center, radius, subdivisions, mat):
+    """Triangles of an icosphere (20 * 4^subdivisions faces), outward
+    winding, smooth per-vertex normals (the unit direction from the centre),
+    so its shading and geometric normals differ (include/integrator.h:25-46)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+             (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.asarray(v, np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                mid[key] = len(verts) - 1
+            return mid[key]
+
+        nxt = []
+        for a, b, c in faces:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nxt += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nxt
+    v = np.asarray(verts)
+    f = np.asarray(faces)
+    out = np.zeros(len(f), TRI_DTYPE)
+    pts = np.asarray(center, np.float64) + radius * v[f]           # (F, 3, 3)
+    nrm = v[f]
+    flip = np.einsum("fi,fi->f", np.cross(pts[:, 1] - pts[:, 0], pts[:, 2] - pts[:, 0]),
+                     nrm.mean(1)) < 0
+    pts[flip] = pts[flip][:, [0, 2, 1]]
+    nrm[flip] = nrm[flip][:, [0, 2, 1]]
+    out["v"] = pts.astype(np.float32)
+    out["n"] = nrm.astype(np.float32)
+    out["material"] = mat
+    return out
+
+
+def caustics_scene() -> Scene:
+    """Caustics configuration: the box scene's walls and light (the inner
+    boxes replaced by two spheres), a mirror sphere (rho 1) at (0.3, 0.18,
+    0.4) and a glass sphere (rho 1, IOR 1.5) at (0.7, 0.18, 0.6), radius
+    0.18, icospheres subdivided 4 times (5120 triangles each)."""
+    s = _box_shell(False)
+    mirror, glass = len(s.albedo), len(s.albedo) + 1
+    sph = [icosphere((0.3, 0.18, 0.4), 0.18, 4, mirror),
+           icosphere((0.7, 0.18, 0.6), 0.18, 4, glass)]
+    s.tris = np.concatenate([s.tris] + sph)
+    s.albedo = np.concatenate([s.albedo, np.ones((2, 3), np.float32)])
+    s.mat_types = np.concatenate([s.mat_types, np.array([_lib.PM_MATERIAL_MIRROR,
+                                                         _lib.PM_MATERIAL_GLASS], np.int32)])
+    s.ior = np.concatenate([np.full(len(s.albedo) - 1, 1.5, np.float32),
+                            np.array([1.5], np.float32)])
+    return s
+
+
+def water_scene(seed: int = 0, grid: int = 256) -> Scene:
+    """Differentiation configuration: the box scene's walls and light over a
+    water surface, a height field on a grid x grid lattice over the floor
+    (2 grid^2 triangles, 131,072 at 256) at y = 0.4 with heights the sum of 8
+    sinusoids (wavevector direction uniform, magnitude uniform in [4, 40]
+    rad/unit, amplitude uniform in [0, 0.02], phase uniform; numpy seed),
+    glass of rho 1 and IOR 1.33, smooth normals from the analytic gradient."""
+    rng = np.random.default_rng(seed)
+    ang = rng.uniform(0, 2 * math.pi, 8)
+    mag = rng.uniform(4.0, 40.0, 8)
+    amp = rng.uniform(0.0, 0.02, 8)
+    phase = rng.uniform(0, 2 * math.pi, 8)
+    kx, kz = mag * np.cos(ang), mag * np.sin(ang)
+    g = np.linspace(0.0, 1.0, grid + 1)
+    X, Z = np.meshgrid(g, g, indexing="ij")
+    arg = X[..., None] * kx + Z[..., None] * kz + phase
+    H = 0.4 + (amp * np.sin(arg)).sum(-1)
+    dHx = (amp * kx * np.cos(arg)).sum(-1)
+    dHz = (amp * kz * np.cos(arg)).sum(-1)
+    P = np.stack([X, H, Z], -1)
+    N = np.stack([-dHx, np.ones_like(H), -dHz], -1)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    i, j = np.meshgrid(np.arange(grid), np.arange(grid), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    # two triangles per cell, wound so that their normals point up (+y)
+    a, b, c, d = (i, j), (i + 1, j), (i + 1, j + 1), (i, j + 1)
+    tri_idx = [(a, d, c), (a, c, b)]
+    s = _box_shell(False)
+    water = len(s.albedo)
+    recs = []
+    for p0, p1, p2 in tri_idx:
+        r = np.zeros(len(i), TRI_DTYPE)
+        r["v"] = np.stack([P[p0], P[p1], P[p2]], 1).astype(np.float32)
+        r["n"] = np.stack([N[p0], N[p1], N[p2]], 1).astype(np.float32)
+        r["material"] = water
+        recs.append(r)
+    hf = np.stack(recs, 1).reshape(-1)
+    s.tris = np.concatenate([s.tris, hf])
+    s.albedo = np.concatenate([s.albedo, np.ones((1, 3), np.float32)])
+    s.mat_types = np.concatenate([s.mat_types, np.array([_lib.PM_MATERIAL_GLASS], np.int32)])
+    s.ior = np.concatenate([np.full(len(s.albedo) - 1, 1.5, np.float32),
+                            np.array([1.33], np.float32)])
+    return s
+
+
+def batch_scene_instance(rng) -> Scene:
+    """Batching configuration instance: the box scene with the five walls'
+    albedos uniform in [0.1, 0.9] (box_scene_instance) plus 0-3 spheres
+    (radius uniform in [0.08, 0.15], resting on the floor at a uniform
+    position inside the walls; icospheres subdivided 3 times), each a mirror
+    or IOR-1.5 glass with probability 1/2."""
+    s = box_scene_instance(rng)
+    n_sph = int(rng.integers(0, 4))
+    parts = [s.tris]
+    albedo, types = [s.albedo], [s.mat_types]
+    for _ in range(n_sph):
+        r = float(rng.uniform(0.08, 0.15))
+        x, z = rng.uniform(r, 1.0 - r, 2)
+        glass = bool(rng.integers(0, 2))
+        mat = sum(len(a) for a in albedo)
+        parts.append(icosphere((x, r, z), r, 3, mat))
+        albedo.append(np.ones((1, 3), np.float32))
+        types.append(np.array([_lib.PM_MATERIAL_GLASS if glass else _lib.PM_MATERIAL_MIRROR],
+                              np.int32))
+    s.tris = np.concatenate(parts)
+    s.albedo = np.concatenate(albedo)
+    s.mat_types = np.concatenate(types)
+    s.ior = np.full(len(s.albedo), 1.5, np.float32)
+    return s
+
+
 def _camera_struct(cam: dict) -> pmx_camera:
     c = pmx_camera()
     for k in ("position", "forward", "right", "up"):
@@ -167,41 +321,126 @@ def surrogate_photons(scene: Scene, n: int, seed: int = 0, device: str = "cuda")
     return tuple(a.t() for a in soa)
 
 
-def trace_photons(scene: Scene, n: int, seed: int = 0, max_depth: int = 100,
+class NativeScene:
+    """The scene on the device with its BVH (pmx_scene_create): triangles,
+    materials, IORs and the light radiance are copied at creation."""
+
+    def __init__(self, scene: Scene, ctx=None):
+        from .photon_map import Context
+
+        self.ctx = ctx or Context.default()
+        self.scene = scene
+        tris = np.ascontiguousarray(scene.tris)
+        mats = np.ascontiguousarray(scene.materials())
+        ior = None if scene.ior is None else np.ascontiguousarray(scene.ior, np.float32)
+        le = np.ascontiguousarray(scene.light_le, np.float32)
+        h = C.c_void_p()
+        call("pmx_scene_create", self.ctx.handle, ptr(tris), len(tris), ptr(mats), ptr(ior),
+             len(mats), ptr(le), C.byref(h))
+        self.handle = h
+
+    def info(self) -> dict:
+        out = pmx_scene_info()
+        call("pmx_scene_get_info", self.handle, C.byref(out))
+        return {k: getattr(out, k) for k, _ in pmx_scene_info._fields_ if k != "reserved"}
+
+    def close(self) -> None:
+        if self.handle is not None:
+            call("pmx_scene_destroy", self.handle)
+            self.handle = None
+
+    def __enter__(self) -> "NativeScene":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Native:
+    """A NativeScene for the call: the one given, or a temporary one."""
+
+    def __init__(self, scene):
+        self.own = not isinstance(scene, NativeScene)
+        self.ns = NativeScene(scene) if self.own else scene
+
+    def __enter__(self) -> NativeScene:
+        return self.ns
+
+    def __exit__(self, *exc) -> None:
+        if self.own:
+            self.ns.close()
+
+
+PHOTON_KINDS = {"global": 0, "caustics": 1}
+
+
+def intersect(scene, org, dirs, brute_force: bool = False):
+    """Closest hits of rays (pmx_scene_intersect): org and dirs (n, 3) torch
+    float32 on the device -> (tri int32 (-1: miss), t, u, v) of shape (n,)."""
+    import torch
+
+    n = int(org.shape[0])
+    o = org.to(torch.float32).t().contiguous()
+    d = dirs.to(torch.float32).t().contiguous()
+    tri = torch.empty((n,), dtype=torch.int32, device=org.device)
+    t, u, v = (torch.empty((n,), dtype=torch.float32, device=org.device) for _ in range(3))
+    with _Native(scene) as ns:
+        call("pmx_scene_intersect", ns.handle, n, ptr(o), ptr(d), int(bool(brute_force)),
+             ptr(tri), ptr(t), ptr(u), ptr(v), None)
+        torch.cuda.synchronize(org.device)
+    return tri, t, u, v
+
+
+def trace_photons(scene, n: int, seed: int = 0, max_depth: int = 100, kind: str = "global",
                   device: str = "cuda"):
-    """Traced photon map (pmx_trace_photons): exactly n photons stored in
+    """Traced photon array (pmx_trace_photons): exactly n photons stored in
     (path, bounce) order.  Returns (pos, wi, flux) as (n, 3) float32 torch
     tensors on the device and the number of emitted paths (the estimate's
-    divisor)."""
+    divisor).  scene: a Scene or a NativeScene."""
     import torch
-    from .photon_map import Context
 
-    ctx = Context.default()
     soa = [torch.empty((3, n), dtype=torch.float32, device=device) for _ in range(3)]
-    tris = np.ascontiguousarray(scene.tris)
-    mats = np.ascontiguousarray(scene.materials())
-    le = np.ascontiguousarray(scene.light_le, np.float32)
     n_emit = C.c_int64()
-    call("pmx_trace_photons", ctx.handle, ptr(tris), len(tris), ptr(mats), None, len(mats),
-         ptr(le), int(n), int(max_depth), int(seed), ptr(soa[0]), ptr(soa[1]), ptr(soa[2]),
-         C.byref(n_emit))
+    with _Native(scene) as ns:
+        call("pmx_trace_photons", ns.handle, PHOTON_KINDS[kind], int(n), int(max_depth),
+             int(seed), ptr(soa[0]), ptr(soa[1]), ptr(soa[2]), C.byref(n_emit))
     return tuple(a.t() for a in soa) + (int(n_emit.value),)
 
 
-def camera_queries(scene: Scene, width: int, height: int, spp: int, depth: int,
+def trace_photon_map(scene, n_paths: int, seed: int = 0, max_depth: int = 100,
+                     kind: str = "global"):
+    """The reference's photon-map construction (pmx_trace_photon_map):
+    n_paths paths emitted, every stored photon kept.  Returns (PhotonMap, not
+    built yet, in the scene's context; number of photons stored)."""
+    from .photon_map import PhotonMap
+
+    h = C.c_void_p()
+    n_stored = C.c_int64()
+    with _Native(scene) as ns:
+        call("pmx_trace_photon_map", ns.handle, PHOTON_KINDS[kind], int(n_paths),
+             int(max_depth), int(seed), C.byref(h), C.byref(n_stored))
+        ctx = ns.ctx
+    return PhotonMap._adopt(ctx, h, n_stored.value), int(n_stored.value)
+
+
+def camera_queries(scene, width: int, height: int, spp: int, depth: int,
                    seed: int = 0, device: str = "cuda", want_fg_mat: bool = False):
     """Packed (m, 10) query records on the device (see make_queries), m =
     width * height * spp; with want_fg_mat also the primary-hit material of
-    each final-gathering query."""
+    each final-gathering query.  scene: a Scene or a NativeScene."""
     import torch
-    from .photon_map import Context
 
-    ctx = Context.default()
     m = width * height * spp
     out = torch.empty((m, 10), dtype=torch.float32, device=device)
     fg = torch.empty((m,), dtype=torch.int32, device=device) if want_fg_mat else None
-    tris = np.ascontiguousarray(scene.tris)
-    cam = _camera_struct(scene.camera)
-    call("pmx_fg_queries", ctx.handle, ptr(tris), len(tris), C.byref(cam), width, height, spp,
-         depth, int(seed), ptr(out), ptr(fg))
+    with _Native(scene) as ns:
+        cam = _camera_struct(ns.scene.camera)
+        call("pmx_fg_queries", ns.handle, C.byref(cam), width, height, spp, depth, int(seed),
+             ptr(out), ptr(fg))
     return (out, fg) if want_fg_mat else out

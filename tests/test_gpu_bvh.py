@@ -1,0 +1,142 @@
+"""Scene BVH (pmx_scene_create / pmx_scene_intersect) against brute force
+and the numpy oracle.
+
+BVH traversal and the brute-force loop call the same float32 triangle test
+on the same operands, and the BVH's boxes are padded so that no reported
+hit can be culled, so both must return identical (triangle, t, u, v) for
+every ray, bit for bit.  The brute-force kernel in turn must equal the numpy
+restatement (oracle/scene_oracle.py).  Scenes: the box, the caustics scene
+(two 5120-triangle icospheres), a batching instance with spheres, and the
+131,072-triangle water height field."""
+
+import numpy as np
+import pytest
+
+from oracle import scene_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(name):
+    from photon_mapping_amd import scene as sc
+
+    if name == "box":
+        return sc.box_scene()
+    if name == "caustics":
+        return sc.caustics_scene()
+    if name == "water":
+        return sc.water_scene(0)
+    return sc.batch_scene_instance(np.random.default_rng(7))
+
+
+def _rays(n, seed):
+    """Origins in and around the unit box, isotropic directions, plus a
+    quarter of the rays aimed from the camera side at points in the box."""
+    rng = np.random.default_rng(seed)
+    org = rng.uniform(-0.2, 1.2, (n, 3))
+    d = rng.normal(size=(n, 3))
+    q = n // 4
+    org[:q] = np.array([0.5, 0.5, 3.0]) + rng.uniform(-0.3, 0.3, (q, 3))
+    d[:q] = rng.uniform(0.0, 1.0, (q, 3)) - org[:q]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return org.astype(np.float32), d.astype(np.float32)
+
+
+def _hits(ns, org, d, brute):
+    import torch
+
+    from photon_mapping_amd import scene as sc
+
+    o = torch.from_numpy(org).cuda()
+    dd = torch.from_numpy(d).cuda()
+    return [x.cpu().numpy() for x in sc.intersect(ns, o, dd, brute_force=brute)]
+
+
+def _same(a, b):
+    for x, y in zip(a, b):
+        assert x.dtype == y.dtype
+        assert np.array_equal(x.view(np.uint32) if x.dtype == np.float32 else x,
+                              y.view(np.uint32) if y.dtype == np.float32 else y)
+
+
+@pytest.mark.parametrize("name,n", [("box", 65536), ("caustics", 32768), ("batch", 32768),
+                                    ("water", 8192)])
+def test_bvh_equals_brute_force(gpu, name, n):
+    from photon_mapping_amd import scene as sc
+
+    print(f"bvh vs brute force: {name}, {n} rays", flush=True)
+    with sc.NativeScene(_scene(name)) as ns:
+        org, d = _rays(n, seed=len(name))
+        bvh = _hits(ns, org, d, False)
+        brute = _hits(ns, org, d, True)
+    _same(bvh, brute)
+    assert (bvh[0] >= 0).mean() > 0.5
+
+
+def test_axis_aligned_and_grazing_rays(gpu):
+    """Directions with zero components (infinite slab reciprocals), rays in
+    the planes of walls, and rays through edges and corners."""
+    from photon_mapping_amd import scene as sc
+
+    org, d = [], []
+    axes = np.eye(3)
+    for p in [(0.5, 0.5, 0.5), (0.5, 0.0, 0.5), (0.0, 0.5, 0.5), (0.5, 1.0, 0.5),
+              (0.375, 0.999, 0.5), (0.5, 0.5, 1.0), (0.0, 0.0, 0.0), (1.0, 1.0, 0.0),
+              (0.25, 0.3, 0.3), (0.65, 0.3, 0.62)]:
+        for ax in axes:
+            for sgn in (1.0, -1.0):
+                org.append(p)
+                d.append(sgn * ax)
+        for dd in [(1, 1, 0), (1, 0, 1), (0, 1, 1), (1, -1, 0), (-1, 0, -1), (1, 1, 1)]:
+            org.append(p)
+            d.append(np.asarray(dd, np.float64) / np.linalg.norm(dd))
+    org = np.asarray(org, np.float32)
+    d = np.asarray(d, np.float32)
+    for name in ("box", "caustics"):
+        with sc.NativeScene(_scene(name)) as ns:
+            bvh = _hits(ns, org, d, False)
+            brute = _hits(ns, org, d, True)
+        _same(bvh, brute)
+
+
+@pytest.mark.parametrize("name,n", [("box", 4096), ("caustics", 256)])
+def test_brute_force_equals_numpy_oracle(gpu, name, n):
+    from photon_mapping_amd import scene as sc
+
+    s = _scene(name)
+    org, d = _rays(n, seed=3)
+    with sc.NativeScene(s) as ns:
+        got = _hits(ns, org, d, True)
+    want = so.closest_hits(s.tris["v"], org, d)
+    _same(got, list(want))
+
+
+def test_scene_info(gpu):
+    from photon_mapping_amd import scene as sc
+
+    with sc.NativeScene(_scene("box")) as ns:
+        info = ns.info()
+    assert info["n_tri"] == 36 and info["n_lights"] == 2
+    assert info["light_area"] == pytest.approx(0.0625, rel=1e-6)
+    assert 1 <= info["bvh_nodes"] < 36 and info["bvh_leaves"] == info["bvh_nodes"] + 1
+    with sc.NativeScene(_scene("water")) as ns:
+        info = ns.info()
+    assert info["n_tri"] == 131084
+    assert info["bvh_depth"] <= 38           # the device stack holds 40 entries
+    assert info["bvh_leaves"] == info["bvh_nodes"] + 1
+    assert 1.0 < info["sah_cost"] < 200.0
+
+
+def test_single_triangle_scene(gpu):
+    from photon_mapping_amd import scene as sc
+
+    s = sc.box_scene()
+    s.tris = s.tris[10:11].copy()            # one light triangle
+    org = np.array([[0.45, 0.5, 0.45], [0.9, 0.5, 0.9]], np.float32)
+    d = np.array([[0, 1, 0], [0, 1, 0]], np.float32)
+    with sc.NativeScene(s) as ns:
+        bvh = _hits(ns, org, d, False)
+        brute = _hits(ns, org, d, True)
+    _same(bvh, brute)
+    want = so.closest_hits(s.tris["v"], org, d)
+    _same(bvh, list(want))

@@ -113,7 +113,9 @@ int pmx_fg_queries(const pmx_scene* scene, const pmx_camera* cam, int width, int
  * specular (:496-512).  Paths are numbered from 0 with a counter-based RNG
  * (the kind is part of the stream) and their photons concatenated in
  * (path, bounce) order until n_store photons are stored; *n_emit receives
- * the number of paths emitted (the estimate's divisor).  The SoA outputs
+ * the number of paths emitted (the estimate's divisor).  Fails with
+ * PM_ERR_INVALID_ARG when 2^26 paths store less than 1e-4 of n_store (for
+ * example caustics in a scene without specular surfaces).  The SoA outputs
  * (3 * n_store floats each) are device memory.  Runs on the null stream and
  * synchronises. */
 int pmx_trace_photons(const pmx_scene* scene, int kind, int64_t n_store, int max_depth,
@@ -127,6 +129,62 @@ int pmx_trace_photons(const pmx_scene* scene, int kind, int64_t n_store, int max
  * The map is not built yet (pm_photonmap_build).  Synchronous. */
 int pmx_trace_photon_map(const pmx_scene* scene, int kind, int64_t n_paths, int max_depth,
                          uint64_t seed, pm_photonmap** out_map, int64_t* n_stored);
+
+/* ---- rendering --------------------------------------------------------------- */
+
+/* Parameters of a render (include/integrator.h:49-130, 290-392;
+ * examples/main.cpp:22-70). */
+typedef struct pmx_render_params {
+  int32_t width, height, spp;
+  int32_t max_depth;              /* eye-path depth limit (maxDepth) */
+  int32_t final_gathering_depth;  /* photon mapping: diffuse hits at a smaller depth get
+                                     direct light, caustics and a final-gathering ray */
+  int32_t k;                      /* nEstimationGlobal, used for both maps
+                                     (include/integrator.h:139, 161) */
+  double n_emit_global;           /* paths emitted for the global map (its divisor) */
+  double n_emit_caustics;         /* paths emitted for the caustics map */
+  uint64_t seed;
+  int32_t batch_pixels;           /* pixels per wavefront batch; 0: library default */
+  int32_t reserved;
+} pmx_render_params;
+
+typedef struct pmx_render_stats {
+  int64_t samples;
+  int64_t dropped_samples;        /* NaN, negative or beyond 2^30 (not accumulated) */
+  int64_t rays;                   /* closest-hit queries: camera, bounce and shadow rays */
+  int64_t gathers_global;         /* photon-density estimates on each map */
+  int64_t gathers_caustics;
+  int32_t batches;
+  int32_t iterations;             /* wavefront shading rounds, summed over batches */
+  double ms;                      /* wall time of the call */
+} pmx_render_stats;
+
+/* PhotonMapping::integrate for every pixel sample
+ * (include/integrator.h:290-377), the reference's render loop
+ * (examples/main.cpp:33-70): sample (i, j, s) shoots the camera ray of
+ * sensor point u = (2 (j + r0) - width) / height, v = (2 (i + r1) - height) /
+ * height; a sample whose radiance is NaN or negative is dropped, and every
+ * pixel is divided by spp.  Wavefront: eye paths are shaded in rounds
+ * (light hits, next-event estimation with shadow rays, specular splits at
+ * depth < 3, final-gathering rays), the photon-density estimates of each
+ * round run as pm_gather_radiance calls on global_map (the radiance at
+ * diffuse hits at depth >= final_gathering_depth and at final-gathering
+ * hits) and caustics_map (may be NULL when final_gathering_depth is 0), and
+ * sample radiances accumulate in 32.32 fixed point, so the image does not
+ * depend on scheduling.  Both maps must be built.  out_mean (height x width
+ * x 3 floats, row i = sensor v) and out_var_opt (the variance of each pixel
+ * mean, same shape, may be NULL) are device memory.  Synchronous. */
+int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
+                              pm_photonmap* global_map, pm_photonmap* caustics_map,
+                              const pmx_render_params* params, float* out_mean,
+                              float* out_var_opt, pmx_render_stats* stats_opt);
+
+/* PathTracing::integrate (include/integrator.h:49-101) for every pixel
+ * sample, the same camera, accumulation and outputs as above (k, maps and
+ * final_gathering_depth unused).  Synchronous. */
+int pmx_render_path_tracing(const pmx_scene* scene, const pmx_camera* cam,
+                            const pmx_render_params* params, float* out_mean,
+                            float* out_var_opt, pmx_render_stats* stats_opt);
 
 /* Surrogate photon array: n photons uniform by area over the triangles,
  * wi cosine-distributed about the face normal, flux = power * albedo of the

@@ -24,8 +24,8 @@ def load() -> C.CDLL:
     global _lib
     if _lib is not None:
         return _lib
-    src = HERE / "pm_oracle.cpp"
-    if not LIB.exists() or LIB.stat().st_mtime < src.stat().st_mtime:
+    srcs = [HERE / "pm_oracle.cpp", HERE / "render_oracle.cpp", HERE / "oracle_core.hpp"]
+    if not LIB.exists() or LIB.stat().st_mtime < max(s.stat().st_mtime for s in srcs):
         subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
     lib = C.CDLL(str(LIB))
     VP, I64, I32, DBL = C.c_void_p, C.c_int64, C.c_int, C.c_double
@@ -40,6 +40,9 @@ def load() -> C.CDLL:
         "orc_gather_ref32": (I32, [VP, VP, VP, I64, VP, VP, I32, I32, DBL, VP, VP, I32]),
         "orc_adjoint_fp64": (I32, [I64, VP, VP, I64, VP, VP, I32, I32, VP, VP, DBL, VP, VP,
                                    VP]),
+        "orc_render": (I64, [I32, VP, I32, VP, VP, I32, VP, VP, I32, I32, I32, I32, I32, I32,
+                             I64, VP, VP, VP, C.c_float, I64, VP, VP, VP, C.c_float,
+                             C.c_uint64, I32, VP, VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -155,3 +158,38 @@ def adjoint_fp64(wi, flux, queries, mats, idx, r2, n_emit, dl_drgb):
                                  _p(r2), float(n_emit), _p(g), _p(dflux), _p(drho))
     assert rc == 0
     return dflux, drho
+
+
+def render(mode, tris, mats, ior, le, cam, width, height, spp, max_depth=100, fg_depth=0, k=100,
+           global_map=None, n_emit_global=1.0, caustics_map=None, n_emit_caustics=1.0, seed=0,
+           threads: int = 0):
+    """CPU restatement of the reference's render loop (render_oracle.cpp).
+    mode "pt" (PathTracing) or "pm" (PhotonMapping).  tris: 80-byte
+    triangle records; mats (n, 4) material records; ior (n,) or None; cam:
+    13 floats (position, forward, right, up, focal); maps: (pos, wi, flux)
+    arrays of shape (n, 3).  Returns (mean, variance of the mean) as
+    (height, width, 3) float32 arrays and the number of dropped samples."""
+    t = np.ascontiguousarray(tris)
+    mt = np.ascontiguousarray(mats, np.float32)
+    io = None if ior is None else np.ascontiguousarray(ior, np.float32)
+    l = np.ascontiguousarray(le, np.float32)
+    c = np.ascontiguousarray(cam, np.float32)
+    assert c.shape == (13,)
+
+    def soa3(m):
+        if m is None:
+            z = np.zeros((3, 0), np.float32)
+            return 0, z, z, z
+        pos, wi, flux = (_soa(a) for a in m)
+        return pos.shape[1], pos, wi, flux
+
+    ng, gp, gw, gf = soa3(global_map)
+    nc, cp, cw, cf = soa3(caustics_map)
+    mean = np.empty((height, width, 3), np.float32)
+    var = np.empty((height, width, 3), np.float32)
+    dropped = load().orc_render(0 if mode == "pt" else 1, _p(t), len(t), _p(mt), _p(io),
+                                mt.shape[0], _p(l), _p(c), width, height, spp, max_depth,
+                                fg_depth, k, ng, _p(gp), _p(gw), _p(gf), float(n_emit_global),
+                                nc, _p(cp), _p(cw), _p(cf), float(n_emit_caustics), int(seed),
+                                _threads(threads), _p(mean), _p(var))
+    return mean, var, int(dropped)

@@ -120,6 +120,8 @@ SIGNATURES = {
     "pmx_trace_photon_map": (None, [VP, I32, I64, I32, C.c_uint64, C.POINTER(VP), VP]),
     "pmx_surrogate_photons": (None, [VP, VP, I32, VP, I32, VP, I64, C.c_uint64, VP, VP,
                                      VP]),
+    "pmx_render_photon_mapping": (None, [VP, VP, VP, VP, VP, VP, VP, VP]),
+    "pmx_render_path_tracing": (None, [VP, VP, VP, VP, VP, VP]),
 }
 
 # Symbols declared in include/*.h (checked by the CPU tests).

@@ -24,7 +24,7 @@ LIB_PATH = PKG_DIR / "libpm_gather.so"
 ORACLE_DIR = REPO_DIR / "oracle"
 
 HIP_SOURCES = ["pm_capi.hip", "pm_build.hip", "pm_sort.hip", "pm_gather.hip",
-               "pm_scene.hip", "pm_bvh.hip"]
+               "pm_scene.hip", "pm_bvh.hip", "pm_render.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

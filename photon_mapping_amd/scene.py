@@ -53,6 +53,24 @@ class pmx_scene_info(C.Structure):
 assert C.sizeof(pmx_scene_info) == 40
 
 
+class pmx_render_params(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("max_depth", C.c_int32), ("final_gathering_depth", C.c_int32),
+                ("k", C.c_int32), ("n_emit_global", C.c_double),
+                ("n_emit_caustics", C.c_double), ("seed", C.c_uint64),
+                ("batch_pixels", C.c_int32), ("reserved", C.c_int32)]
+
+
+class pmx_render_stats(C.Structure):
+    _fields_ = [("samples", C.c_int64), ("dropped_samples", C.c_int64), ("rays", C.c_int64),
+                ("gathers_global", C.c_int64), ("gathers_caustics", C.c_int64),
+                ("batches", C.c_int32), ("iterations", C.c_int32), ("ms", C.c_double)]
+
+
+assert C.sizeof(pmx_render_params) == 56
+assert C.sizeof(pmx_render_stats) == 56
+
+
 @dataclass
 class Scene:
     tris: np.ndarray                   # TRI_DTYPE records
@@ -444,3 +462,57 @@ def camera_queries(scene, width: int, height: int, spp: int, depth: int,
         call("pmx_fg_queries", ns.handle, C.byref(cam), width, height, spp, depth, int(seed),
              ptr(out), ptr(fg))
     return (out, fg) if want_fg_mat else out
+
+
+def _render_params(width, height, spp, max_depth, fg_depth, k, n_emit_global, n_emit_caustics,
+                   seed, batch_pixels):
+    p = pmx_render_params()
+    p.width, p.height, p.spp = int(width), int(height), int(spp)
+    p.max_depth, p.final_gathering_depth, p.k = int(max_depth), int(fg_depth), int(k)
+    p.n_emit_global, p.n_emit_caustics = float(n_emit_global), float(n_emit_caustics)
+    p.seed, p.batch_pixels = int(seed), int(batch_pixels)
+    return p
+
+
+def _stats_dict(st: pmx_render_stats) -> dict:
+    return {k: getattr(st, k) for k, _ in pmx_render_stats._fields_}
+
+
+def render_photon_mapping(scene, global_map, n_emit_global: float, width: int, height: int,
+                          spp: int, k: int = 100, caustics_map=None, n_emit_caustics: float = 1.0,
+                          final_gathering_depth: int = 0, max_depth: int = 100, seed: int = 0,
+                          batch_pixels: int = 0, device: str = "cuda"):
+    """PhotonMapping::integrate over every pixel sample
+    (pmx_render_photon_mapping).  The maps are built PhotonMaps.  Returns
+    (mean, variance of the mean) as (height, width, 3) float32 device
+    tensors and the render statistics."""
+    import torch
+
+    mean = torch.empty((height, width, 3), dtype=torch.float32, device=device)
+    var = torch.empty_like(mean)
+    p = _render_params(width, height, spp, max_depth, final_gathering_depth, k, n_emit_global,
+                       n_emit_caustics, seed, batch_pixels)
+    st = pmx_render_stats()
+    with _Native(scene) as ns:
+        cam = _camera_struct(ns.scene.camera)
+        call("pmx_render_photon_mapping", ns.handle, C.byref(cam), global_map._handle,
+             caustics_map._handle if caustics_map is not None else None, C.byref(p), ptr(mean),
+             ptr(var), C.byref(st))
+    return mean, var, _stats_dict(st)
+
+
+def render_path_tracing(scene, width: int, height: int, spp: int, max_depth: int = 100,
+                        seed: int = 0, batch_pixels: int = 0, device: str = "cuda"):
+    """PathTracing::integrate over every pixel sample
+    (pmx_render_path_tracing): (mean, variance of the mean, statistics)."""
+    import torch
+
+    mean = torch.empty((height, width, 3), dtype=torch.float32, device=device)
+    var = torch.empty_like(mean)
+    p = _render_params(width, height, spp, max_depth, 0, 1, 1.0, 1.0, seed, batch_pixels)
+    st = pmx_render_stats()
+    with _Native(scene) as ns:
+        cam = _camera_struct(ns.scene.camera)
+        call("pmx_render_path_tracing", ns.handle, C.byref(cam), C.byref(p), ptr(mean),
+             ptr(var), C.byref(st))
+    return mean, var, _stats_dict(st)

@@ -237,6 +237,46 @@ def ray_bench(ctx, dev, seed=0):
     return out
 
 
+def render_bench(ns, pmap, n_emit, width, height, spp, k, seed=0):
+    """Frames of the box scene at the benchmark's resolution: photon mapping
+    with final gathering (pmx_render_photon_mapping: camera rays, direct light
+    with shadow rays, final-gathering rays, the global-map gathers through the
+    gather kernel; the box holds no specular surface, so its caustics map is
+    empty) and the path tracer (pmx_render_path_tracing), one warm frame
+    each, wall time of the call."""
+    import photon_mapping_amd as pm
+    import torch
+
+    from photon_mapping_amd import scene as sc
+
+    empty = pm.PhotonMap(pmap.ctx)
+    z = torch.zeros((0, 3), dtype=torch.float32, device="cuda")
+    empty.set_photons_soa(z, z, z)
+    empty.build()
+    out = {}
+    kw = dict(k=k, caustics_map=empty, n_emit_caustics=1.0, final_gathering_depth=1, seed=seed)
+    sc.render_photon_mapping(ns, pmap, n_emit, 64, 64, 4, **kw)     # warm
+    torch.cuda.synchronize()
+    img, _, st = sc.render_photon_mapping(ns, pmap, n_emit, width, height, spp, **kw)
+    torch.cuda.synchronize()
+    out["photon_mapping"] = {
+        "resolution": f"{width}x{height}", "spp": spp, "k": k, "ms": st["ms"],
+        "frames_per_s": 1e3 / st["ms"], "mrays_per_s": st["rays"] / st["ms"] / 1e3,
+        "gathers": st["gathers_global"], "gathers_per_s": st["gathers_global"] / st["ms"] * 1e3,
+        "rounds": st["iterations"], "batches": st["batches"],
+        "dropped_samples": st["dropped_samples"],
+        "image_mean": [float(x) for x in img.double().mean((0, 1)).cpu()]}
+    sc.render_path_tracing(ns, 64, 64, 4, seed=seed)
+    torch.cuda.synchronize()
+    img, _, st = sc.render_path_tracing(ns, width, height, spp, seed=seed)
+    torch.cuda.synchronize()
+    out["path_tracing"] = {
+        "resolution": f"{width}x{height}", "spp": spp, "ms": st["ms"],
+        "frames_per_s": 1e3 / st["ms"], "mrays_per_s": st["rays"] / st["ms"] / 1e3,
+        "image_mean": [float(x) for x in img.double().mean((0, 1)).cpu()]}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +294,8 @@ def main() -> int:
     ap.add_argument("--waves", type=int, default=0, help="persistent waves per CU (0: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--no-render", action="store_true",
+                    help="skip the photon-mapping and path-tracing frame timings")
     ap.add_argument("--no-rays", action="store_true",
                     help="skip the BVH ray and caustic-tracing rates")
     ap.add_argument("--no-batched", action="store_true",
@@ -419,6 +461,10 @@ def main() -> int:
     if not args.no_rays:
         rays = ray_bench(ctx, dev)
         log(f"rays: {rays}")
+    render = None
+    if not args.no_render:
+        render = render_bench(ns, pmap, n_emit, args.width, args.height, args.spp, k)
+        log(f"render: {render}")
 
     tuned = None
     if not args.no_autotune:
@@ -490,7 +536,8 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "detail": {"sort_ms": sort_ms, "build_ms": build_ms, **work, **(adj or {}),
-                   "batched": batched, "autotune": tuned, "tracing": tracing, "rays": rays},
+                   "batched": batched, "autotune": tuned, "tracing": tracing, "rays": rays,
+                   "render": render},
     }
     print(json.dumps(out), flush=True)
     return 0

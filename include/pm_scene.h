@@ -51,12 +51,18 @@ typedef struct pmx_scene pmx_scene;
 /* Copies the scene to the device and builds its BVH.  tris (n_tri records),
  * mats (n_mat records, rho and type: include/material.h Lambert / Mirror /
  * Glass), mat_ior (n_mat floats, may be NULL: 1.5 for every glass material)
- * and light_le (3 floats: the radiance of every triangle with is_light = 1,
- * include/light.h:17-45) are host memory.  Triangle materials outside
- * [0, n_mat) read as material 0.  Synchronous. */
+ * are host memory.  Emission (include/light.h:17-45), host memory too:
+ *   mat_le (n_mat x 3 floats, may be NULL): a triangle whose material has a
+ *     positive emission component is a light of that radiance, as the
+ *     reference makes an AreaLight of every face with a positive MTL Ke
+ *     (include/scene.h:41-52);
+ *   light_le (3 floats, may be NULL): the radiance of every other triangle
+ *     with is_light = 1.
+ * At least one of them must be given.  Triangle materials outside [0, n_mat)
+ * read as material 0.  Synchronous. */
 int pmx_scene_create(pm_context* ctx, const pmx_triangle* tris, int n_tri,
                      const pm_material* mats, const float* mat_ior, int n_mat,
-                     const float* light_le, pmx_scene** out_scene);
+                     const float* light_le, const float* mat_le, pmx_scene** out_scene);
 int pmx_scene_destroy(pmx_scene* scene);
 
 typedef struct pmx_scene_info {

@@ -40,7 +40,7 @@ def load() -> C.CDLL:
         "orc_gather_ref32": (I32, [VP, VP, VP, I64, VP, VP, I32, I32, DBL, VP, VP, I32]),
         "orc_adjoint_fp64": (I32, [I64, VP, VP, I64, VP, VP, I32, I32, VP, VP, DBL, VP, VP,
                                    VP]),
-        "orc_render": (I64, [I32, VP, I32, VP, VP, I32, VP, VP, I32, I32, I32, I32, I32, I32,
+        "orc_render": (I64, [I32, VP, I32, VP, VP, I32, VP, VP, VP, I32, I32, I32, I32, I32, I32,
                              I64, VP, VP, VP, C.c_float, I64, VP, VP, VP, C.c_float,
                              C.c_uint64, I32, VP, VP]),
     }
@@ -162,7 +162,7 @@ def adjoint_fp64(wi, flux, queries, mats, idx, r2, n_emit, dl_drgb):
 
 def render(mode, tris, mats, ior, le, cam, width, height, spp, max_depth=100, fg_depth=0, k=100,
            global_map=None, n_emit_global=1.0, caustics_map=None, n_emit_caustics=1.0, seed=0,
-           threads: int = 0):
+           threads: int = 0, mat_le=None):
     """CPU restatement of the reference's render loop (render_oracle.cpp).
     mode "pt" (PathTracing) or "pm" (PhotonMapping).  tris: 80-byte
     triangle records; mats (n, 4) material records; ior (n,) or None; cam:
@@ -172,7 +172,8 @@ def render(mode, tris, mats, ior, le, cam, width, height, spp, max_depth=100, fg
     t = np.ascontiguousarray(tris)
     mt = np.ascontiguousarray(mats, np.float32)
     io = None if ior is None else np.ascontiguousarray(ior, np.float32)
-    l = np.ascontiguousarray(le, np.float32)
+    l = None if le is None else np.ascontiguousarray(le, np.float32)
+    ml = None if mat_le is None else np.ascontiguousarray(mat_le, np.float32)
     c = np.ascontiguousarray(cam, np.float32)
     assert c.shape == (13,)
 
@@ -188,7 +189,7 @@ def render(mode, tris, mats, ior, le, cam, width, height, spp, max_depth=100, fg
     mean = np.empty((height, width, 3), np.float32)
     var = np.empty((height, width, 3), np.float32)
     dropped = load().orc_render(0 if mode == "pt" else 1, _p(t), len(t), _p(mt), _p(io),
-                                mt.shape[0], _p(l), _p(c), width, height, spp, max_depth,
+                                mt.shape[0], _p(l), _p(ml), _p(c), width, height, spp, max_depth,
                                 fg_depth, k, ng, _p(gp), _p(gw), _p(gf), float(n_emit_global),
                                 nc, _p(cp), _p(cw), _p(cf), float(n_emit_caustics), int(seed),
                                 _threads(threads), _p(mean), _p(var))

@@ -66,8 +66,9 @@ struct Scene {
   const MaterialRec* mats;
   const float* ior;
   int nm;
-  V le;
+  std::vector<V> tri_le;   // emission per triangle (zero: not a light)
   std::vector<int> lights;
+  bool is_light(int t) const { return tri_le[t].x > 0 || tri_le[t].y > 0 || tri_le[t].z > 0; }
   int mat_of(int t) const {
     const int m = tris[t].material;
     return (m >= 0 && m < nm) ? m : 0;
@@ -256,12 +257,13 @@ struct PM {
     const float r = std::sqrt(dot(lp - s.pos, lp - s.pos));
     const float pdf_dir = pdf_pos * r * r / std::fabs(dot(wi * -1.0f, lns));
     if (closest(S, s.pos, wi, r - kRayEps).tri >= 0) return {0, 0, 0};
+    const V le = S.tri_le[S.lights[li]];
     const MaterialRec& m = S.mats[s.mat];
     const V wol = to_local(wo, s), wil = to_local(wi, s);
     V f{0, 0, 0};
     if (m.type == 0 && !(wol.y < 0 || wil.y < 0)) f = ld(m.rho) / kPiF;
     const float c = std::fabs(dot(wi, s.ns));
-    return mul(f * c, S.le) / (pdf_choose * pdf_dir);
+    return mul(f * c, le) / (pdf_choose * pdf_dir);
   }
 
   V indirect(V wo, const Surf& s, Rng& rng, int depth) const {
@@ -282,7 +284,7 @@ struct PM {
     if (depth >= max_depth) return {0, 0, 0};
     const Hit h = closest(S, o, d, std::numeric_limits<float>::max());
     if (h.tri < 0) return {0, 0, 0};
-    if (S.tris[h.tri].is_light) return S.le;
+    if (S.is_light(h.tri)) return S.tri_le[h.tri];
     const Surf s = surface(S, o, d, h);
     const V wo = d * -1.0f;
     const MaterialRec& m = S.mats[s.mat];
@@ -352,7 +354,7 @@ V path_trace(const Scene& S, V o, V d, Rng& rng, int max_depth) {
       if (rng.next() >= p) break;
       tp = tp / p;
     }
-    if (S.tris[h.tri].is_light) L = L + mul(tp, S.le);
+    if (S.is_light(h.tri)) L = L + mul(tp, S.tri_le[h.tri]);
     const Surf s = surface(S, o, d, h);
     V dir;
     float pdf;
@@ -374,16 +376,25 @@ extern "C" {
 // up[3], focal.  Outputs (height x width x 3, row-major): the pixel means and
 // the variances of the means.  Returns the number of dropped samples.
 int64_t orc_render(int mode, const void* tris, int n_tri, const void* mats, const float* ior,
-                   int n_mat, const float* le, const float* cam, int width, int height, int spp,
+                   int n_mat, const float* le, const float* mat_le, const float* cam, int width, int height, int spp,
                    int max_depth, int fg_depth, int k, int64_t n_global, const float* g_pos,
                    const float* g_wi, const float* g_flux, float n_emit_global,
                    int64_t n_caustics, const float* c_pos, const float* c_wi,
                    const float* c_flux, float n_emit_caustics, uint64_t seed, int nthreads,
                    float* out_mean, float* out_var) {
   Scene S{static_cast<const Tri*>(tris), n_tri, static_cast<const MaterialRec*>(mats), ior,
-          n_mat, V{le[0], le[1], le[2]}, {}};
-  for (int i = 0; i < n_tri; ++i)
-    if (S.tris[i].is_light) S.lights.push_back(i);
+          n_mat, {}, {}};
+  // a face whose material emits is a light of that emission
+  // (include/scene.h:41-52); otherwise the is_light flag with le
+  S.tri_le.assign(n_tri, V{0, 0, 0});
+  for (int i = 0; i < n_tri; ++i) {
+    const int m = S.mat_of(i);
+    if (mat_le && (mat_le[3 * m] > 0 || mat_le[3 * m + 1] > 0 || mat_le[3 * m + 2] > 0))
+      S.tri_le[i] = V{mat_le[3 * m], mat_le[3 * m + 1], mat_le[3 * m + 2]};
+    else if (le && S.tris[i].is_light)
+      S.tri_le[i] = V{le[0], le[1], le[2]};
+    if (S.is_light(i)) S.lights.push_back(i);
+  }
   KdTreeOracle gt, ct;
   auto build = [](KdTreeOracle& t, int64_t n, const float* p) {
     t.n = static_cast<int>(n);

@@ -111,7 +111,7 @@ SIGNATURES = {
     "pm_plan_launch": (None, [VP, VP]),
     "pm_plan_destroy": (None, [VP]),
     "pm_autotune": (None, [VP, I64, VP, VP, I32, I32, C.c_double, VP, VP]),
-    "pmx_scene_create": (None, [VP, VP, I32, VP, VP, I32, VP, C.POINTER(VP)]),
+    "pmx_scene_create": (None, [VP, VP, I32, VP, VP, I32, VP, VP, C.POINTER(VP)]),
     "pmx_scene_destroy": (None, [VP]),
     "pmx_scene_get_info": (None, [VP, VP]),
     "pmx_scene_intersect": (None, [VP, I64, VP, VP, I32, VP, VP, VP, VP, VP]),

@@ -214,7 +214,8 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
     if (kind == kKindCamera) {
       if (tri.is_light) {
         // a light hit returns its emission (include/integrator.h:296-299)
-        add_term(O.acc, O.bad, sg.sample, w.x * S.le[0], w.y * S.le[1], w.z * S.le[2]);
+        const V3 le = emission(S, tri);
+        add_term(O.acc, O.bad, sg.sample, w.x * le.x, w.y * le.y, w.z * le.z);
       } else if (type == PM_MATERIAL_LAMBERT) {
         if (static_cast<int>(depth) >= R.fg_depth) {
           push_query(O.qg, O.wg, O.n_qg, O.q_cap, x, ns_raw, wo, tri.material, w, sg.sample,
@@ -242,8 +243,9 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
               const float f = lambert_on(wo, wi, ns);
               const float c = f * fabsf(dot(wi, ns)) /
                               ((1.0f / static_cast<float>(S.n_lights)) * pdf_dir);
-              add_term(O.acc, O.bad, sg.sample, w.x * rho.x * c * S.le[0],
-                       w.y * rho.y * c * S.le[1], w.z * rho.z * c * S.le[2]);
+              const V3 le = emission(S, lt);
+              add_term(O.acc, O.bad, sg.sample, w.x * rho.x * c * le.x, w.y * rho.y * c * le.y,
+                       w.z * rho.z * c * le.z);
             }
           }
           // caustics by the caustics map (include/integrator.h:155-175)
@@ -401,7 +403,7 @@ __global__ __launch_bounds__(kThreads) void path_trace_kernel(RenderDev R, pmx_c
       tp = tp * (1.0f / p);
     }
     const pmx_triangle& tri = S.tris[h.tri];
-    if (tri.is_light) L = L + v3(tp.x * S.le[0], tp.y * S.le[1], tp.z * S.le[2]);
+    if (tri.is_light) L = L + mul(tp, emission(S, tri));
     const int m = material_of(S, tri);
     const V3 x = o + d * h.t;
     const V3 ns = normalize(shading_normal(tri, h.u, h.v));

@@ -178,7 +178,8 @@ __global__ __launch_bounds__(kThreads) void trace_kernel(
   dim += 2;
   // Le / (choose pdf * point pdf * direction pdf) * cos = Le * n_lights * area * pi
   const float w0 = static_cast<float>(S.n_lights) * area * kPiF;
-  float tr_r = S.le[0] * w0, tr_g = S.le[1] * w0, tr_b = S.le[2] * w0;
+  const V3 le = emission(S, lt);
+  float tr_r = le.x * w0, tr_g = le.y * w0, tr_b = le.z * w0;
   uint32_t stored = 0;
   for (int k = 0; k < max_depth; ++k) {
     if (!(tr_r >= 0.0f && tr_g >= 0.0f && tr_b >= 0.0f)) break;  // NaN or negative
@@ -323,9 +324,9 @@ extern "C" {
 
 int pmx_scene_create(pm_context* h, const pmx_triangle* tris, int n_tri, const pm_material* mats,
                      const float* mat_ior, int n_mat, const float* light_le,
-                     pmx_scene** out_scene) {
-  if (!h || !tris || !mats || !light_le || !out_scene)
-    return fail(PM_ERR_INVALID_ARG, "null argument");
+                     const float* mat_le, pmx_scene** out_scene) {
+  if (!h || !tris || !mats || !out_scene) return fail(PM_ERR_INVALID_ARG, "null argument");
+  if (!light_le && !mat_le) return fail(PM_ERR_INVALID_ARG, "no emission given (light_le or mat_le)");
   *out_scene = nullptr;
   if (n_tri < 1 || n_mat < 1) return fail(PM_ERR_INVALID_ARG, "a scene needs triangles and materials");
   for (int m = 0; m < n_mat; ++m) {
@@ -338,10 +339,25 @@ int pmx_scene_create(pm_context* h, const pmx_triangle* tris, int n_tri, const p
   PM_HIP(hipSetDevice(ctx->device));
   BvhHost bvh;
   if (int rc = build_bvh(tris, n_tri, &bvh)) return rc;
+  // emission table: slot 0 = light_le, slot 1 + m = material m's emission;
+  // the device copy of each triangle carries its slot + 1 in is_light
+  std::vector<float> le_table(3 * (1 + static_cast<size_t>(n_mat)), 0.0f);
+  if (light_le)
+    for (int a = 0; a < 3; ++a) le_table[a] = light_le[a];
+  std::vector<pmx_triangle> dev_tris(tris, tris + n_tri);
   std::vector<int32_t> lights;
   double light_area = 0.0;
   for (int i = 0; i < n_tri; ++i) {
-    if (!tris[i].is_light) continue;
+    int slot = -1;
+    const int m = (tris[i].material >= 0 && tris[i].material < n_mat) ? tris[i].material : 0;
+    if (mat_le && (mat_le[3 * m] > 0.0f || mat_le[3 * m + 1] > 0.0f || mat_le[3 * m + 2] > 0.0f)) {
+      slot = 1 + m;
+      for (int a = 0; a < 3; ++a) le_table[3 * slot + a] = mat_le[3 * m + a];
+    } else if (light_le && tris[i].is_light) {
+      slot = 0;
+    }
+    dev_tris[i].is_light = slot + 1;
+    if (slot < 0) continue;
     lights.push_back(i);
     const float* a = tris[i].v[0];
     const float* b = tris[i].v[1];
@@ -364,7 +380,6 @@ int pmx_scene_create(pm_context* h, const pmx_triangle* tris, int n_tri, const p
   s->bvh_depth = bvh.depth;
   s->sah_cost = bvh.sah_cost;
   s->light_area = light_area;
-  for (int a = 0; a < 3; ++a) s->le[a] = light_le[a];
   s->has_ior = mat_ior != nullptr;
   const auto upload = [](DevBuf& buf, const void* src, size_t bytes) -> int {
     if (bytes == 0) return PM_OK;
@@ -372,7 +387,8 @@ int pmx_scene_create(pm_context* h, const pmx_triangle* tris, int n_tri, const p
     PM_HIP(hipMemcpy(buf.ptr, src, bytes, hipMemcpyHostToDevice));
     return PM_OK;
   };
-  int rc = upload(s->tris, tris, n_tri * sizeof(pmx_triangle));
+  int rc = upload(s->tris, dev_tris.data(), n_tri * sizeof(pmx_triangle));
+  if (!rc) rc = upload(s->le, le_table.data(), le_table.size() * sizeof(float));
   if (!rc) rc = upload(s->tri4, bvh.tri4.data(), bvh.tri4.size() * sizeof(float4));
   if (!rc) rc = upload(s->nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(BvhNode));
   if (!rc) rc = upload(s->mats, mats, n_mat * sizeof(pm_material));

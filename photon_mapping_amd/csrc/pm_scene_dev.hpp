@@ -75,7 +75,7 @@ struct SceneDev {
   int n_mat;
   const int32_t* lights;     // light triangle indices
   int n_lights;
-  float le[3];
+  const float* le;           // emission table, 3 floats per slot
 };
 
 struct Hit {
@@ -186,6 +186,13 @@ __device__ __forceinline__ V3 geometric_normal(const pmx_triangle& tr) {
   return normalize(cross(ld3(tr.v[1]) - p0, ld3(tr.v[2]) - p0));
 }
 
+// Emission of a light triangle: the device copy's is_light holds its slot in
+// the emission table plus one (0: not a light).
+__device__ __forceinline__ V3 emission(const SceneDev& S, const pmx_triangle& tr) {
+  const float* e = S.le + 3 * (tr.is_light - 1);
+  return v3(e[0], e[1], e[2]);
+}
+
 __device__ __forceinline__ int material_of(const SceneDev& S, const pmx_triangle& tr) {
   return (tr.material >= 0 && tr.material < S.n_mat) ? tr.material : 0;
 }
@@ -256,15 +263,14 @@ struct SceneImpl {
   int n_tri = 0, n_mat = 0, n_lights = 0;
   int bvh_nodes = 0, bvh_leaves = 0, bvh_depth = 0;
   double sah_cost = 0.0, light_area = 0.0;
-  float le[3] = {0.f, 0.f, 0.f};
   bool has_ior = false;
-  DevBuf tris, tri4, nodes, mats, ior, lights;
+  DevBuf tris, tri4, nodes, mats, ior, lights, le;
 
   scn::SceneDev dev(int brute) const {
     return scn::SceneDev{tris.as<pmx_triangle>(), tri4.as<float4>(), nodes.as<BvhNode>(),
                     n_tri, brute, mats.as<pm_material>(),
                     has_ior ? ior.as<float>() : nullptr, n_mat, lights.as<int32_t>(), n_lights,
-                    {le[0], le[1], le[2]}};
+                    le.as<float>()};
   }
 };
 

@@ -80,6 +80,7 @@ class Scene:
     light_area: float
     camera: dict = field(default_factory=dict)
     ior: np.ndarray | None = None     # (n_mat,) float32 for glass; None: 1.5
+    mat_le: np.ndarray | None = None  # (n_mat, 3) emission per material (MTL Ke), or None
 
     @property
     def light_power(self) -> np.ndarray:
@@ -351,10 +352,11 @@ class NativeScene:
         tris = np.ascontiguousarray(scene.tris)
         mats = np.ascontiguousarray(scene.materials())
         ior = None if scene.ior is None else np.ascontiguousarray(scene.ior, np.float32)
-        le = np.ascontiguousarray(scene.light_le, np.float32)
+        le = None if scene.light_le is None else np.ascontiguousarray(scene.light_le, np.float32)
+        mle = None if scene.mat_le is None else np.ascontiguousarray(scene.mat_le, np.float32)
         h = C.c_void_p()
         call("pmx_scene_create", self.ctx.handle, ptr(tris), len(tris), ptr(mats), ptr(ior),
-             len(mats), ptr(le), C.byref(h))
+             len(mats), ptr(le), ptr(mle), C.byref(h))
         self.handle = h
 
     def info(self) -> dict:
@@ -516,3 +518,182 @@ def render_path_tracing(scene, width: int, height: int, spp: int, max_depth: int
         call("pmx_render_path_tracing", ns.handle, C.byref(cam), C.byref(p), ptr(mean),
              ptr(var), C.byref(st))
     return mean, var, _stats_dict(st)
+
+
+# ---- scene I/O (include/scene.h:15-52, 109-270; include/image.h:55-90) ------------
+
+def _mtl_parse(path) -> dict:
+    mats, cur = {}, None
+    with open(path) as fh:
+        for line in fh:
+            tok = line.split()
+            if not tok or tok[0].startswith("#"):
+                continue
+            if tok[0] == "newmtl":
+                cur = " ".join(tok[1:])
+                mats[cur] = {"Kd": (0.0, 0.0, 0.0), "Ke": (0.0, 0.0, 0.0), "illum": 0, "Ni": 1.0}
+            elif cur is not None and tok[0] in ("Kd", "Ke"):
+                mats[cur][tok[0]] = tuple(float(x) for x in tok[1:4])
+            elif cur is not None and tok[0] == "illum":
+                mats[cur]["illum"] = int(tok[1])
+            elif cur is not None and tok[0] == "Ni":
+                mats[cur]["Ni"] = float(tok[1])
+    return mats
+
+
+def load_obj(path, camera: dict | None = None) -> Scene:
+    """Scene::loadModel (include/scene.h:109-270) for Wavefront OBJ + MTL.
+    Polygons are fan-triangulated; every face gets its own three vertices.
+    Normals: the face's vertex normals when all three have one, else the
+    winding normal normalize(cross(normalize(v1 - v0), normalize(v2 - v0)))
+    for all three (include/scene.h:182-189).  Materials (include/scene.h:15-38):
+    illum 5 is a mirror of albedo 1, illum 7 glass of albedo 1 and IOR Ni,
+    anything else Lambert with albedo Kd; faces without a material are
+    Lambert 0.9.  A face whose material has a positive Ke is an area light
+    of radiance Ke (include/scene.h:41-52).  MTL files are looked up next to
+    the OBJ file."""
+    import os
+
+    base = os.path.dirname(os.path.abspath(path))
+    V, N = [], []
+    mtl = {}
+    names = []            # material names in first-use order
+    faces = []            # (vertex idx[3], normal idx[3] or None, material name)
+    cur = None
+    with open(path) as fh:
+        for line in fh:
+            tok = line.split()
+            if not tok or tok[0].startswith("#"):
+                continue
+            if tok[0] == "v":
+                V.append(tuple(float(x) for x in tok[1:4]))
+            elif tok[0] == "vn":
+                N.append(tuple(float(x) for x in tok[1:4]))
+            elif tok[0] == "mtllib":
+                for f in tok[1:]:
+                    mtl.update(_mtl_parse(os.path.join(base, f)))
+            elif tok[0] == "usemtl":
+                cur = " ".join(tok[1:])
+            elif tok[0] == "f":
+                vi, ni = [], []
+                for ref in tok[1:]:
+                    parts = ref.split("/")
+                    a = int(parts[0])
+                    vi.append(a - 1 if a > 0 else len(V) + a)
+                    if len(parts) >= 3 and parts[2]:
+                        b = int(parts[2])
+                        ni.append(b - 1 if b > 0 else len(N) + b)
+                    else:
+                        ni.append(None)
+                for j in range(1, len(vi) - 1):
+                    idx = (vi[0], vi[j], vi[j + 1])
+                    nidx = (ni[0], ni[j], ni[j + 1])
+                    faces.append((idx, None if None in nidx else nidx, cur))
+                    if cur is not None and cur not in names:
+                        names.append(cur)
+    # material table: the used MTL materials, then the default Lambert 0.9
+    albedo, types, ior, mat_le = [], [], [], []
+    for nm in names:
+        m = mtl.get(nm, {"Kd": (0.0, 0.0, 0.0), "Ke": (0.0, 0.0, 0.0), "illum": 0, "Ni": 1.0})
+        if m["illum"] == 5:
+            albedo.append((1.0, 1.0, 1.0)); types.append(_lib.PM_MATERIAL_MIRROR)
+        elif m["illum"] == 7:
+            albedo.append((1.0, 1.0, 1.0)); types.append(_lib.PM_MATERIAL_GLASS)
+        else:
+            albedo.append(m["Kd"]); types.append(_lib.PM_MATERIAL_LAMBERT)
+        ior.append(m["Ni"])
+        mat_le.append(m["Ke"])
+    default = len(names)
+    albedo.append((0.9, 0.9, 0.9)); types.append(_lib.PM_MATERIAL_LAMBERT)
+    ior.append(1.0); mat_le.append((0.0, 0.0, 0.0))
+    Vv = np.asarray(V, np.float32)
+    Nn = np.asarray(N, np.float32) if N else np.zeros((0, 3), np.float32)
+    tris = np.zeros(len(faces), TRI_DTYPE)
+    for t, (idx, nidx, name) in enumerate(faces):
+        p = Vv[list(idx)]
+        tris[t]["v"] = p
+        if nidx is not None:
+            tris[t]["n"] = Nn[list(nidx)]
+        else:
+            e1 = p[1] - p[0]
+            e2 = p[2] - p[0]
+            e1 = e1 / np.linalg.norm(e1)
+            e2 = e2 / np.linalg.norm(e2)
+            n = np.cross(e1, e2)
+            tris[t]["n"] = np.tile(n / np.linalg.norm(n), (3, 1))
+        tris[t]["material"] = names.index(name) if name is not None else default
+    le = np.asarray(mat_le, np.float32)
+    emitting = (le > 0).any(1)
+    light_area = 0.0
+    for t in range(len(tris)):
+        if emitting[tris[t]["material"]]:
+            v = tris[t]["v"].astype(np.float64)
+            light_area += 0.5 * float(np.linalg.norm(np.cross(v[1] - v[0], v[2] - v[0])))
+    return Scene(tris, np.asarray(albedo, np.float32), np.asarray(types, np.int32), None,
+                 light_area, camera or {}, np.asarray(ior, np.float32), le)
+
+
+def write_obj(scene: Scene, path) -> None:
+    """Writes the scene as OBJ + MTL (next to it, same stem) that load_obj
+    reads back to the same triangles, normals and materials.  Emission: the
+    material's mat_le, or light_le for the materials of is_light triangles."""
+    import os
+
+    stem = os.path.splitext(os.path.basename(path))[0]
+    mtl_path = os.path.join(os.path.dirname(os.path.abspath(path)), stem + ".mtl")
+    n_mat = len(scene.albedo)
+    le = (np.zeros((n_mat, 3), np.float32) if scene.mat_le is None
+          else np.asarray(scene.mat_le, np.float32).copy())
+    if scene.light_le is not None:
+        for t in scene.tris:
+            if t["is_light"] and not (le[t["material"]] > 0).any():
+                le[t["material"]] = scene.light_le
+    ior = np.full(n_mat, 1.5, np.float32) if scene.ior is None else scene.ior
+    with open(mtl_path, "w") as fh:
+        for m in range(n_mat):
+            illum = {_lib.PM_MATERIAL_MIRROR: 5, _lib.PM_MATERIAL_GLASS: 7}.get(
+                int(scene.mat_types[m]), 2)
+            kd = " ".join(repr(float(x)) for x in scene.albedo[m])
+            ke = " ".join(repr(float(x)) for x in le[m])
+            fh.write(f"newmtl m{m}\nKd {kd}\nKe {ke}\nillum {illum}\nNi {float(ior[m])!r}\n\n")
+    with open(path, "w") as fh:
+        fh.write(f"mtllib {stem}.mtl\n")
+        cur = None
+        for t, tri in enumerate(scene.tris):
+            for v in tri["v"]:
+                fh.write(f"v {float(v[0])!r} {float(v[1])!r} {float(v[2])!r}\n")
+            for n in tri["n"]:
+                fh.write(f"vn {float(n[0])!r} {float(n[1])!r} {float(n[2])!r}\n")
+            m = int(tri["material"])
+            if m != cur:
+                fh.write(f"usemtl m{m}\n")
+                cur = m
+            b = 3 * t
+            fh.write(f"f {b + 1}//{b + 1} {b + 2}//{b + 2} {b + 3}//{b + 3}\n")
+
+
+def write_ppm(path, image, gamma: float = 2.2) -> None:
+    """Image::gammaCorrection + writePPM (include/image.h:55-90): each
+    channel c -> c^(1/gamma), then uint(255 c) clamped to [0, 255], written
+    as ASCII P3, row 0 first.  image: (height, width, 3), numpy or torch."""
+    img = image.detach().cpu().numpy() if hasattr(image, "detach") else np.asarray(image)
+    img = np.asarray(img, np.float32)
+    h, w, _ = img.shape
+    g = np.power(np.maximum(img, 0.0), np.float32(1.0 / gamma)).astype(np.float32)
+    q = np.clip(np.floor(np.float32(255.0) * g), 0, 255).astype(np.int64)
+    with open(path, "w") as fh:
+        fh.write(f"P3\n{w} {h}\n255\n")
+        for row in q:
+            fh.write("\n".join(f"{r} {g_} {b}" for r, g_, b in row))
+            fh.write("\n")
+
+
+def write_pfm(path, image) -> None:
+    """Linear float image (height, width, 3) as a little-endian PFM, row 0 at
+    the top of the picture (PFM stores bottom-up, so rows are reversed)."""
+    img = image.detach().cpu().numpy() if hasattr(image, "detach") else np.asarray(image)
+    img = np.asarray(img, np.float32)
+    h, w, _ = img.shape
+    with open(path, "wb") as fh:
+        fh.write(f"PF\n{w} {h}\n-1.0\n".encode())
+        fh.write(np.ascontiguousarray(img[::-1]).astype("<f4").tobytes())

@@ -62,7 +62,7 @@ def _oracle_render(s, mode, w, h, spp, **kw):
     import pm_oracle
 
     return pm_oracle.render(mode, s.tris, s.materials(), s.ior, s.light_le,
-                            _cam_array(s.camera), w, h, spp, **kw)
+                            _cam_array(s.camera), w, h, spp, mat_le=s.mat_le, **kw)
 
 
 def _maps(s, n_global, n_caustics):

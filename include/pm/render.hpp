@@ -1,0 +1,477 @@
+// pm/render.hpp -- header-only C++ drop-in for the reference's Scene
+// (include/scene.h), Camera (include/camera.h), PathTracing and
+// PhotonMapping integrators (include/integrator.h) and Image
+// (include/image.h), backed by libpm_gather.so.
+//
+// Same class names, method names and argument meaning, with one change of
+// shape: the reference renders by calling Integrator::integrate(ray, scene,
+// sampler) for every pixel sample from an OpenMP loop (examples/main.cpp:
+// 33-70); here Integrator::render(scene, camera, width, height, n_samples,
+// seed) runs that whole loop on the GPU and returns the averaged Image
+// (dropped NaN / negative samples and the division by n_samples as in the
+// reference's loop).  Scene::loadModel reads OBJ + MTL with the reference's
+// material semantics; Scene::build uploads the scene and builds its BVH.
+// PhotonMapping::build traces nPhotonsGlobal (and nPhotonsGlobal x
+// multiplier caustic) paths and builds both maps on the GPU.
+//
+// The vector type is PM_VEC3F_TYPE (default pm::Vec3f), as in
+// pm/photon_map.hpp.  Errors throw pm::Error.
+#ifndef PM_RENDER_HPP
+#define PM_RENDER_HPP
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "pm/photon_map.hpp"
+#include "pm_scene.h"
+
+namespace pm {
+
+constexpr float kPi = 3.14159265359f;  // include/core.h:7
+
+namespace detail {
+inline void cross3(const float* a, const float* b, float* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+inline void normalize3(float* a) {
+  const float l = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  for (int i = 0; i < 3; ++i) a[i] = a[i] / l;
+}
+}  // namespace detail
+
+template <typename V>
+struct BasicRay {
+  V origin;
+  V direction;
+};
+
+// Pinhole camera (include/camera.h:10-47): right = normalize(forward x y),
+// up = normalize(right x forward), focal length 1 / tan(FOV / 2); the sensor
+// centre is `position` and the pinhole sits at position + focal * forward.
+template <typename V>
+class BasicCamera {
+ public:
+  BasicCamera(const V& position, const V& forward, float FOV = 0.5f * kPi) {
+    for (int i = 0; i < 3; ++i) {
+      cam.position[i] = position[i];
+      cam.forward[i] = forward[i];
+    }
+    const float y[3] = {0.f, 1.f, 0.f};
+    detail::cross3(cam.forward, y, cam.right);
+    detail::normalize3(cam.right);
+    detail::cross3(cam.right, cam.forward, cam.up);
+    detail::normalize3(cam.up);
+    cam.focal = 1.0f / std::tan(0.5f * FOV);
+  }
+
+  // uv in [-1, 1]^2 on the sensor; pdf 1 (include/camera.h:41-47).
+  bool sampleRay(const float uv[2], BasicRay<V>& ray, float& pdf) const {
+    float s[3], d[3];
+    for (int i = 0; i < 3; ++i) {
+      s[i] = cam.position[i] + uv[0] * cam.right[i] + uv[1] * cam.up[i];
+      d[i] = cam.position[i] + cam.focal * cam.forward[i] - s[i];
+    }
+    detail::normalize3(d);
+    for (int i = 0; i < 3; ++i) {
+      ray.origin[i] = s[i];
+      ray.direction[i] = d[i];
+    }
+    pdf = 1.0f;
+    return true;
+  }
+
+  const pmx_camera& desc() const { return cam; }
+
+ private:
+  pmx_camera cam{};
+};
+
+// Triangle scene (include/scene.h:54-356).
+class Scene {
+ public:
+  Scene() = default;
+  Scene(const Scene&) = delete;
+  Scene& operator=(const Scene&) = delete;
+  ~Scene() { release(); }
+
+  // Scene::loadModel (include/scene.h:109-270): OBJ + MTL, polygons
+  // fan-triangulated, vertex normals when a face has them for every vertex,
+  // else the winding normal; illum 5 = mirror (albedo 1), illum 7 = glass
+  // (albedo 1, IOR Ni), otherwise Lambert Kd; faces without a material are
+  // Lambert 0.9; a positive Ke makes the face an area light.  MTL files are
+  // read from the OBJ file's directory.
+  void loadModel(const std::string& filepath) {
+    clear();
+    std::ifstream in(filepath);
+    if (!in) throw Error(PM_ERR_INVALID_ARG, "Scene::loadModel: cannot open " + filepath);
+    const std::string dir = filepath.substr(0, filepath.find_last_of("/\\") + 1);
+    std::vector<float> v, vn;
+    std::map<std::string, Mtl> mtl;
+    std::vector<std::string> used;
+    std::vector<int> face_mat;  // index into used, -1: none
+    std::string cur;
+    bool has_cur = false;
+    std::string line;
+    while (std::getline(in, line)) {
+      std::istringstream ls(line);
+      std::string tag;
+      if (!(ls >> tag) || tag[0] == '#') continue;
+      if (tag == "v" || tag == "vn") {
+        float x, y, z;
+        ls >> x >> y >> z;
+        auto& dst = tag == "v" ? v : vn;
+        dst.insert(dst.end(), {x, y, z});
+      } else if (tag == "mtllib") {
+        std::string f;
+        while (ls >> f) read_mtl(dir + f, mtl);
+      } else if (tag == "usemtl") {
+        std::getline(ls >> std::ws, cur);
+        has_cur = true;
+      } else if (tag == "f") {
+        std::vector<int> vi, ni;
+        std::string ref;
+        while (ls >> ref) {
+          const size_t s1 = ref.find('/');
+          const int a = std::stoi(ref.substr(0, s1));
+          vi.push_back(a > 0 ? a - 1 : static_cast<int>(v.size() / 3) + a);
+          int b = -1;
+          if (s1 != std::string::npos) {
+            const size_t s2 = ref.find('/', s1 + 1);
+            if (s2 != std::string::npos && s2 + 1 < ref.size()) {
+              const int bb = std::stoi(ref.substr(s2 + 1));
+              b = bb > 0 ? bb - 1 : static_cast<int>(vn.size() / 3) + bb;
+            }
+          }
+          ni.push_back(b);
+        }
+        int mi = -1;
+        if (has_cur) {
+          auto it = std::find(used.begin(), used.end(), cur);
+          if (it == used.end()) {
+            used.push_back(cur);
+            mi = static_cast<int>(used.size()) - 1;
+          } else {
+            mi = static_cast<int>(it - used.begin());
+          }
+        }
+        for (size_t j = 1; j + 1 < vi.size(); ++j) {
+          const int idx[3] = {vi[0], vi[j], vi[j + 1]};
+          const int nidx[3] = {ni[0], ni[j], ni[j + 1]};
+          pmx_triangle t{};
+          for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a) t.v[c][a] = v[3 * idx[c] + a];
+          if (nidx[0] >= 0 && nidx[1] >= 0 && nidx[2] >= 0) {
+            for (int c = 0; c < 3; ++c)
+              for (int a = 0; a < 3; ++a) t.n[c][a] = vn[3 * nidx[c] + a];
+          } else {
+            float e1[3], e2[3], n[3];
+            for (int a = 0; a < 3; ++a) {
+              e1[a] = t.v[1][a] - t.v[0][a];
+              e2[a] = t.v[2][a] - t.v[0][a];
+            }
+            detail::normalize3(e1);
+            detail::normalize3(e2);
+            detail::cross3(e1, e2, n);
+            detail::normalize3(n);
+            for (int c = 0; c < 3; ++c)
+              for (int a = 0; a < 3; ++a) t.n[c][a] = n[a];
+          }
+          tris.push_back(t);
+          face_mat.push_back(mi);
+        }
+      }
+    }
+    // materials: the used MTL materials in first-use order, then Lambert 0.9
+    for (const std::string& name : used) {
+      const auto it = mtl.find(name);
+      const Mtl m = it == mtl.end() ? Mtl{} : it->second;
+      pm_material pmat{};
+      if (m.illum == 5 || m.illum == 7) {
+        pmat.rho[0] = pmat.rho[1] = pmat.rho[2] = 1.0f;
+        pmat.type = m.illum == 5 ? PM_MATERIAL_MIRROR : PM_MATERIAL_GLASS;
+      } else {
+        for (int a = 0; a < 3; ++a) pmat.rho[a] = m.kd[a];
+        pmat.type = PM_MATERIAL_LAMBERT;
+      }
+      mats.push_back(pmat);
+      ior.push_back(m.ni);
+      for (int a = 0; a < 3; ++a) mat_le.push_back(m.ke[a]);
+    }
+    const int dflt = static_cast<int>(used.size());
+    mats.push_back(pm_material{{0.9f, 0.9f, 0.9f}, PM_MATERIAL_LAMBERT});
+    ior.push_back(1.0f);
+    mat_le.insert(mat_le.end(), {0.f, 0.f, 0.f});
+    for (size_t f = 0; f < tris.size(); ++f) tris[f].material = face_mat[f] < 0 ? dflt : face_mat[f];
+  }
+
+  // Scene::build (include/scene.h:275-305): device copy and BVH.
+  void build() {
+    release();
+    if (tris.empty()) throw Error(PM_ERR_INVALID_ARG, "Scene::build: no triangles");
+    check(pmx_scene_create(default_context(), tris.data(), static_cast<int>(tris.size()),
+                           mats.data(), ior.data(), static_cast<int>(mats.size()), nullptr,
+                           mat_le.data(), &scene),
+          "pmx_scene_create");
+  }
+
+  uint32_t nVertices() const { return static_cast<uint32_t>(3 * tris.size()); }
+  uint32_t nFaces() const { return static_cast<uint32_t>(tris.size()); }
+  const std::vector<pmx_triangle>& triangles() const { return tris; }
+  const std::vector<pm_material>& materials() const { return mats; }
+  pmx_scene* handle() const {
+    if (!scene) throw Error(PM_ERR_NOT_BUILT, "Scene used before build()");
+    return scene;
+  }
+
+ private:
+  struct Mtl {
+    float kd[3] = {0.f, 0.f, 0.f};
+    float ke[3] = {0.f, 0.f, 0.f};
+    int illum = 0;
+    float ni = 1.0f;
+  };
+
+  static void read_mtl(const std::string& path, std::map<std::string, Mtl>& out) {
+    std::ifstream in(path);
+    if (!in) throw Error(PM_ERR_INVALID_ARG, "Scene::loadModel: cannot open " + path);
+    std::string line, name;
+    Mtl* cur = nullptr;
+    while (std::getline(in, line)) {
+      std::istringstream ls(line);
+      std::string tag;
+      if (!(ls >> tag) || tag[0] == '#') continue;
+      if (tag == "newmtl") {
+        std::getline(ls >> std::ws, name);
+        cur = &out[name];
+        *cur = Mtl{};
+      } else if (cur && tag == "Kd") {
+        ls >> cur->kd[0] >> cur->kd[1] >> cur->kd[2];
+      } else if (cur && tag == "Ke") {
+        ls >> cur->ke[0] >> cur->ke[1] >> cur->ke[2];
+      } else if (cur && tag == "illum") {
+        ls >> cur->illum;
+      } else if (cur && tag == "Ni") {
+        ls >> cur->ni;
+      }
+    }
+  }
+
+  void clear() {
+    release();
+    tris.clear();
+    mats.clear();
+    ior.clear();
+    mat_le.clear();
+  }
+  void release() {
+    if (scene) pmx_scene_destroy(scene);
+    scene = nullptr;
+  }
+
+  std::vector<pmx_triangle> tris;
+  std::vector<pm_material> mats;
+  std::vector<float> ior;
+  std::vector<float> mat_le;
+  pmx_scene* scene = nullptr;
+};
+
+// Image (include/image.h): row-major RGB floats.
+template <typename V>
+class BasicImage {
+ public:
+  BasicImage(unsigned int width, unsigned int height)
+      : w(width), h(height), pixels(3 * static_cast<size_t>(width) * height, 0.0f) {}
+  unsigned int getWidth() const { return w; }
+  unsigned int getHeight() const { return h; }
+  V getPixel(unsigned int i, unsigned int j) const {
+    const float* p = &pixels[3 * (static_cast<size_t>(i) * w + j)];
+    return V(p[0], p[1], p[2]);
+  }
+  void setPixel(unsigned int i, unsigned int j, const V& c) {
+    float* p = &pixels[3 * (static_cast<size_t>(i) * w + j)];
+    for (int a = 0; a < 3; ++a) p[a] = c[a];
+  }
+  void addPixel(unsigned int i, unsigned int j, const V& c) {
+    float* p = &pixels[3 * (static_cast<size_t>(i) * w + j)];
+    for (int a = 0; a < 3; ++a) p[a] += c[a];
+  }
+  void divide(float k) {
+    for (float& x : pixels) x /= k;
+  }
+  void gammaCorrection(float gamma) {
+    for (float& x : pixels) x = std::pow(x, 1.0f / gamma);
+  }
+  // ASCII P3, uint(255 c) clamped to [0, 255] (include/image.h:71-90)
+  void writePPM(const std::string& filename) const {
+    std::ofstream file(filename);
+    file << "P3\n" << w << " " << h << "\n255\n";
+    for (size_t p = 0; p < static_cast<size_t>(w) * h; ++p) {
+      unsigned int c[3];
+      for (int a = 0; a < 3; ++a) {
+        const float x = 255.0f * pixels[3 * p + a];
+        c[a] = x <= 0.0f ? 0u : (x >= 255.0f ? 255u : static_cast<unsigned int>(x));
+      }
+      file << c[0] << " " << c[1] << " " << c[2] << "\n";
+    }
+  }
+  std::vector<float>& data() { return pixels; }
+  const std::vector<float>& data() const { return pixels; }
+
+ private:
+  unsigned int w, h;
+  std::vector<float> pixels;
+};
+
+// The seed source of an integrator's random streams (the reference's
+// UniformSampler, include/sampler.h:61-80).
+class UniformSampler {
+ public:
+  explicit UniformSampler(uint64_t seed = 0) : seed(seed) {}
+  uint64_t getSeed() const { return seed; }
+  void setSeed(uint64_t s) { seed = s; }
+
+ private:
+  uint64_t seed;
+};
+
+template <typename V>
+class BasicIntegrator {
+ public:
+  using Image = BasicImage<V>;
+  using Camera = BasicCamera<V>;
+  virtual ~BasicIntegrator() = default;
+  virtual void build(const Scene& scene, UniformSampler& sampler) = 0;
+  // every pixel sample of the reference's render loop, averaged
+  virtual Image render(const Scene& scene, const Camera& camera, int width, int height,
+                       int n_samples, uint64_t seed = 0) const = 0;
+
+ protected:
+  static pmx_render_params params(int width, int height, int n_samples, int max_depth,
+                                  uint64_t seed) {
+    pmx_render_params p{};
+    p.width = width;
+    p.height = height;
+    p.spp = n_samples;
+    p.max_depth = max_depth;
+    p.seed = seed;
+    p.out_mem = PM_MEM_HOST;
+    return p;
+  }
+};
+
+// PathTracing (include/integrator.h:49-101).
+template <typename V>
+class BasicPathTracing : public BasicIntegrator<V> {
+ public:
+  using typename BasicIntegrator<V>::Image;
+  using typename BasicIntegrator<V>::Camera;
+  explicit BasicPathTracing(int maxDepth = 100) : maxDepth(maxDepth) {}
+  void build(const Scene&, UniformSampler&) override {}
+  Image render(const Scene& scene, const Camera& camera, int width, int height, int n_samples,
+               uint64_t seed = 0) const override {
+    Image img(width, height);
+    const pmx_render_params p = this->params(width, height, n_samples, maxDepth, seed);
+    check(pmx_render_path_tracing(scene.handle(), &camera.desc(), &p, img.data().data(), nullptr,
+                                  nullptr),
+          "pmx_render_path_tracing");
+    return img;
+  }
+
+ private:
+  const int maxDepth;
+};
+
+// PhotonMapping (include/integrator.h:104-563).
+template <typename V>
+class BasicPhotonMapping : public BasicIntegrator<V> {
+ public:
+  using typename BasicIntegrator<V>::Image;
+  using typename BasicIntegrator<V>::Camera;
+  BasicPhotonMapping(int nPhotonsGlobal, int nEstimationGlobal, float nPhotonsCausticsMultiplier,
+                     int nEstimationCaustics, int strictCalcDepth, int maxDepth)
+      : nPhotonsGlobal(nPhotonsGlobal),
+        nEstimationGlobal(nEstimationGlobal),
+        nPhotonsCaustics(static_cast<int>(nPhotonsGlobal * nPhotonsCausticsMultiplier)),
+        nEstimationCaustics(nEstimationCaustics),
+        finalGatheringDepth(strictCalcDepth),
+        maxDepth(maxDepth) {}
+  BasicPhotonMapping(const BasicPhotonMapping&) = delete;
+  BasicPhotonMapping& operator=(const BasicPhotonMapping&) = delete;
+  ~BasicPhotonMapping() override { release(); }
+
+  // photon tracing of both maps and their builds (include/integrator.h:393-563)
+  void build(const Scene& scene, UniformSampler& sampler) override {
+    release();
+    check(pmx_trace_photon_map(scene.handle(), PMX_PHOTONS_GLOBAL, nPhotonsGlobal, maxDepth,
+                               sampler.getSeed(), &global, &nGlobalStored),
+          "pmx_trace_photon_map");
+    check(pm_photonmap_build(global, 0, nullptr), "pm_photonmap_build");
+    if (finalGatheringDepth > 0) {
+      check(pmx_trace_photon_map(scene.handle(), PMX_PHOTONS_CAUSTICS, nPhotonsCaustics, maxDepth,
+                                 sampler.getSeed(), &caustics, &nCausticsStored),
+            "pmx_trace_photon_map");
+      check(pm_photonmap_build(caustics, 0, nullptr), "pm_photonmap_build");
+    }
+  }
+
+  Image render(const Scene& scene, const Camera& camera, int width, int height, int n_samples,
+               uint64_t seed = 0) const override {
+    if (!global) throw Error(PM_ERR_NOT_BUILT, "PhotonMapping used before build()");
+    Image img(width, height);
+    pmx_render_params p = this->params(width, height, n_samples, maxDepth, seed);
+    p.final_gathering_depth = finalGatheringDepth;
+    p.k = nEstimationGlobal;  // both maps, as include/integrator.h:139, 161 do
+    p.n_emit_global = nPhotonsGlobal;
+    p.n_emit_caustics = nPhotonsCaustics;
+    check(pmx_render_photon_mapping(scene.handle(), &camera.desc(), global, caustics, &p,
+                                    img.data().data(), nullptr, nullptr),
+          "pmx_render_photon_mapping");
+    return img;
+  }
+
+  // the device photon maps (the reference's getPhotonMapPtr returns the global one)
+  pm_photonmap* getPhotonMapPtr() const { return global; }
+  pm_photonmap* getCausticsPhotonMapPtr() const { return caustics; }
+  int64_t nGlobalPhotonsStored() const { return nGlobalStored; }
+  int64_t nCausticsPhotonsStored() const { return nCausticsStored; }
+
+ private:
+  void release() {
+    if (global) pm_photonmap_destroy(global);
+    if (caustics) pm_photonmap_destroy(caustics);
+    global = caustics = nullptr;
+  }
+
+  const int nPhotonsGlobal;
+  const int nEstimationGlobal;
+  const int nPhotonsCaustics;
+  const int nEstimationCaustics;  // read by nothing, as in the reference
+  const int finalGatheringDepth;
+  const int maxDepth;
+  pm_photonmap* global = nullptr;
+  pm_photonmap* caustics = nullptr;
+  int64_t nGlobalStored = 0, nCausticsStored = 0;
+};
+
+}  // namespace pm
+
+#ifndef PM_NO_GLOBAL_NAMES
+using Ray = pm::BasicRay<PM_VEC3F_TYPE>;
+using Camera = pm::BasicCamera<PM_VEC3F_TYPE>;
+using Scene = pm::Scene;
+using Image = pm::BasicImage<PM_VEC3F_TYPE>;
+using UniformSampler = pm::UniformSampler;
+using Integrator = pm::BasicIntegrator<PM_VEC3F_TYPE>;
+using PathTracing = pm::BasicPathTracing<PM_VEC3F_TYPE>;
+using PhotonMapping = pm::BasicPhotonMapping<PM_VEC3F_TYPE>;
+#endif
+
+#endif  // PM_RENDER_HPP

@@ -151,7 +151,8 @@ typedef struct pmx_render_params {
   double n_emit_caustics;         /* paths emitted for the caustics map */
   uint64_t seed;
   int32_t batch_pixels;           /* pixels per wavefront batch; 0: library default */
-  int32_t reserved;
+  int32_t out_mem;                /* PM_MEM_DEVICE (0) or PM_MEM_HOST: where out_mean and
+                                     out_var_opt live */
 } pmx_render_params;
 
 typedef struct pmx_render_stats {
@@ -179,7 +180,8 @@ typedef struct pmx_render_stats {
  * sample radiances accumulate in 32.32 fixed point, so the image does not
  * depend on scheduling.  Both maps must be built.  out_mean (height x width
  * x 3 floats, row i = sensor v) and out_var_opt (the variance of each pixel
- * mean, same shape, may be NULL) are device memory.  Synchronous. */
+ * mean, same shape, may be NULL) are device or host memory as
+ * params->out_mem says.  Synchronous. */
 int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
                               pm_photonmap* global_map, pm_photonmap* caustics_map,
                               const pmx_render_params* params, float* out_mean,

@@ -483,6 +483,8 @@ unsigned grid_of(uint64_t n) { return static_cast<unsigned>((n + kThreads - 1) /
 int check_params(const pmx_scene* scene, const pmx_camera* cam, const pmx_render_params* p,
                  float* out_mean) {
   if (!scene || !cam || !p || !out_mean) return fail(PM_ERR_INVALID_ARG, "null argument");
+  if (p->out_mem != PM_MEM_DEVICE && p->out_mem != PM_MEM_HOST)
+    return fail(PM_ERR_INVALID_ARG, "bad out_mem");
   if (p->width < 1 || p->height < 1 || p->spp < 1 || p->max_depth < 0)
     return fail(PM_ERR_INVALID_ARG, "bad image size, spp or depth");
   if (static_cast<int64_t>(p->width) * p->height * p->spp > (int64_t{1} << 40))
@@ -490,6 +492,35 @@ int check_params(const pmx_scene* scene, const pmx_camera* cam, const pmx_render
   if (p->batch_pixels < 0) return fail(PM_ERR_INVALID_ARG, "negative batch size");
   return PM_OK;
 }
+
+// Device images for a render whose outputs are host memory; copies back.
+struct ImageOut {
+  DevBuf mean, var;
+  float* d_mean = nullptr;
+  float* d_var = nullptr;
+  int begin(const pmx_render_params* p, float* out_mean, float* out_var) {
+    if (p->out_mem == PM_MEM_DEVICE) {
+      d_mean = out_mean;
+      d_var = out_var;
+      return PM_OK;
+    }
+    const size_t bytes = static_cast<size_t>(p->width) * p->height * 3 * sizeof(float);
+    if (int rc = mean.reserve(bytes)) return rc;
+    d_mean = mean.as<float>();
+    if (out_var) {
+      if (int rc = var.reserve(bytes)) return rc;
+      d_var = var.as<float>();
+    }
+    return PM_OK;
+  }
+  int end(const pmx_render_params* p, float* out_mean, float* out_var) {
+    if (p->out_mem == PM_MEM_DEVICE) return PM_OK;
+    const size_t bytes = static_cast<size_t>(p->width) * p->height * 3 * sizeof(float);
+    PM_HIP(hipMemcpy(out_mean, d_mean, bytes, hipMemcpyDeviceToHost));
+    if (out_var) PM_HIP(hipMemcpy(out_var, d_var, bytes, hipMemcpyDeviceToHost));
+    return PM_OK;
+  }
+};
 
 // pixels per batch: whole pixels, at most 2^21 samples (or the caller's choice)
 int64_t batch_pixels_of(const pmx_render_params* p) {
@@ -516,6 +547,8 @@ int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
   const auto t_start = std::chrono::steady_clock::now();
   const auto* s = reinterpret_cast<const SceneImpl*>(scene);
   PM_HIP(hipSetDevice(s->ctx->device));
+  ImageOut img;
+  if (int rc = img.begin(p, out_mean, out_var)) return rc;
   RenderDev R{s->dev(0), p->width, p->height, p->spp, p->max_depth, p->final_gathering_depth,
               p->final_gathering_depth > 0 ? 1 : 0, p->seed};
   const int64_t n_pix = static_cast<int64_t>(p->width) * p->height;
@@ -621,7 +654,7 @@ int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
     }
     hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(np)), dim3(kThreads), 0, nullptr,
                        acc.as<unsigned long long>(), bad.as<uint32_t>(), p->spp,
-                       static_cast<uint32_t>(np), pix0, out_mean, out_var, dropped);
+                       static_cast<uint32_t>(np), pix0, img.d_mean, img.d_var, dropped);
     PM_HIP(hipGetLastError());
   }
   unsigned long long tail[2];
@@ -632,6 +665,7 @@ int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
   st.dropped_samples = static_cast<int64_t>(tail[1]);
   st.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
               .count();
+  if (int rc = img.end(p, out_mean, out_var)) return rc;
   if (stats) *stats = st;
   return PM_OK;
 }
@@ -643,6 +677,8 @@ int pmx_render_path_tracing(const pmx_scene* scene, const pmx_camera* cam,
   const auto t_start = std::chrono::steady_clock::now();
   const auto* s = reinterpret_cast<const SceneImpl*>(scene);
   PM_HIP(hipSetDevice(s->ctx->device));
+  ImageOut img;
+  if (int rc = img.begin(p, out_mean, out_var)) return rc;
   RenderDev R{s->dev(0), p->width, p->height, p->spp, p->max_depth, 0, 0, p->seed};
   const int64_t n_pix = static_cast<int64_t>(p->width) * p->height;
   const int64_t bp = std::min(batch_pixels_of(p), n_pix);
@@ -665,7 +701,7 @@ int pmx_render_path_tracing(const pmx_scene* scene, const pmx_camera* cam,
     PM_HIP(hipGetLastError());
     hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(np)), dim3(kThreads), 0, nullptr,
                        acc.as<unsigned long long>(), bad.as<uint32_t>(), p->spp,
-                       static_cast<uint32_t>(np), pix0, out_mean, out_var, rays + 1);
+                       static_cast<uint32_t>(np), pix0, img.d_mean, img.d_var, rays + 1);
     PM_HIP(hipGetLastError());
     ++st.batches;
     ++st.iterations;
@@ -678,6 +714,7 @@ int pmx_render_path_tracing(const pmx_scene* scene, const pmx_camera* cam,
   st.dropped_samples = static_cast<int64_t>(tail[1]);
   st.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
               .count();
+  if (int rc = img.end(p, out_mean, out_var)) return rc;
   if (stats) *stats = st;
   return PM_OK;
 }

@@ -58,7 +58,7 @@ class pmx_render_params(C.Structure):
                 ("max_depth", C.c_int32), ("final_gathering_depth", C.c_int32),
                 ("k", C.c_int32), ("n_emit_global", C.c_double),
                 ("n_emit_caustics", C.c_double), ("seed", C.c_uint64),
-                ("batch_pixels", C.c_int32), ("reserved", C.c_int32)]
+                ("batch_pixels", C.c_int32), ("out_mem", C.c_int32)]
 
 
 class pmx_render_stats(C.Structure):

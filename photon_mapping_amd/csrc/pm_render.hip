@@ -522,10 +522,12 @@ struct ImageOut {
   }
 };
 
-// pixels per batch: whole pixels, at most 2^21 samples (or the caller's choice)
+// pixels per batch: whole pixels, at most 2^24 samples (or the caller's
+// choice).  Large batches matter: the gather's cost per query falls as the
+// queries of a launch get denser.
 int64_t batch_pixels_of(const pmx_render_params* p) {
   if (p->batch_pixels > 0) return p->batch_pixels;
-  return std::max<int64_t>(1, (int64_t{1} << 21) / p->spp);
+  return std::max<int64_t>(1, (int64_t{1} << 24) / p->spp);
 }
 
 }  // namespace
@@ -556,34 +558,37 @@ int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
   const int64_t bsamples = bp * p->spp;
   if (bsamples > (int64_t{1} << (32 - kLineageBits)))
     return fail(PM_ERR_INVALID_ARG, "batch too large (at most 2^24 samples)");
-  // every segment spawns at most two below the split depth and one after it
-  const uint64_t seg_cap = static_cast<uint64_t>(bsamples) << kSplitDepth;
-  const uint64_t q_cap = seg_cap;
   int sample_bits = 1;
   while ((int64_t{1} << sample_bits) < bsamples) ++sample_bits;
   const int key_bits = sample_bits + kLineageBits;
-  DevBuf seg_a, seg_b, qg, wg, qc, wc, rgb, r2, acc, bad, counters;
+  // Buffers grow to each round's needs: a segment spawns at most two
+  // segments and one query per photon map.
+  DevBuf seg[2], qg, wg, qc, wc, rgb, r2, acc, bad, counters;
   DevBuf keys_a, keys_b, vals_a, vals_b, q_sorted, w_sorted;
   Workspace ws;
-  int rc = seg_a.reserve(seg_cap * sizeof(Segment));
-  if (!rc) rc = seg_b.reserve(seg_cap * sizeof(Segment));
-  if (!rc) rc = qg.reserve(q_cap * sizeof(pm_query));
-  if (!rc) rc = wg.reserve(q_cap * sizeof(Weighted));
-  if (!rc && R.caustics) rc = qc.reserve(q_cap * sizeof(pm_query));
-  if (!rc && R.caustics) rc = wc.reserve(q_cap * sizeof(Weighted));
-  if (!rc) rc = rgb.reserve(q_cap * 3 * sizeof(float));
-  if (!rc) rc = r2.reserve(q_cap * sizeof(float));
-  if (!rc) rc = acc.reserve(static_cast<size_t>(bsamples) * 3 * sizeof(unsigned long long));
+  int rc = acc.reserve(static_cast<size_t>(bsamples) * 3 * sizeof(unsigned long long));
   if (!rc) rc = bad.reserve(static_cast<size_t>(bsamples) * sizeof(uint32_t));
   if (!rc) rc = counters.reserve(16 * sizeof(unsigned long long));
-  if (!rc) rc = keys_a.reserve(q_cap * sizeof(uint32_t));
-  if (!rc) rc = keys_b.reserve(q_cap * sizeof(uint32_t));
-  if (!rc) rc = vals_a.reserve(q_cap * sizeof(uint32_t));
-  if (!rc) rc = vals_b.reserve(q_cap * sizeof(uint32_t));
-  if (!rc) rc = q_sorted.reserve(q_cap * sizeof(pm_query));
-  if (!rc) rc = w_sorted.reserve(q_cap * sizeof(Weighted));
-  if (!rc) rc = sort_reserve(&ws, static_cast<int64_t>(q_cap));
+  if (!rc) rc = seg[0].reserve(static_cast<size_t>(bsamples) * sizeof(Segment));
   if (rc) return rc;
+  const auto reserve_round = [&](size_t n_in, DevBuf& out) -> int {
+    const size_t q = n_in;
+    int e = out.reserve(2 * n_in * sizeof(Segment));
+    if (!e) e = qg.reserve(q * sizeof(pm_query));
+    if (!e) e = wg.reserve(q * sizeof(Weighted));
+    if (!e && R.caustics) e = qc.reserve(q * sizeof(pm_query));
+    if (!e && R.caustics) e = wc.reserve(q * sizeof(Weighted));
+    if (!e) e = rgb.reserve(q * 3 * sizeof(float));
+    if (!e) e = r2.reserve(q * sizeof(float));
+    if (!e) e = keys_a.reserve(q * sizeof(uint32_t));
+    if (!e) e = keys_b.reserve(q * sizeof(uint32_t));
+    if (!e) e = vals_a.reserve(q * sizeof(uint32_t));
+    if (!e) e = vals_b.reserve(q * sizeof(uint32_t));
+    if (!e) e = q_sorted.reserve(q * sizeof(pm_query));
+    if (!e) e = w_sorted.reserve(q * sizeof(Weighted));
+    if (!e) e = sort_reserve(&ws, static_cast<int64_t>(q));
+    return e;
+  };
   // counter words (uint32): 0 segments out, 1 global queries, 2 caustics
   // queries, 3 overflow flag; uint64 at words 8 (rays) and 10 (dropped samples)
   auto* c32 = counters.as<uint32_t>();
@@ -597,25 +602,29 @@ int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
     if (int e = zero_words(acc.ptr, static_cast<size_t>(ns) * 3 * sizeof(unsigned long long)))
       return e;
     if (int e = zero_words(bad.ptr, static_cast<size_t>(ns) * sizeof(uint32_t))) return e;
+    int cur_i = 0;
+    if (int e = seg[cur_i].reserve(static_cast<size_t>(ns) * sizeof(Segment))) return e;
     hipLaunchKernelGGL(camera_kernel, dim3(grid_of(ns)), dim3(kThreads), 0, nullptr, R, *cam,
-                       pix0 * p->spp, ns, seg_a.as<Segment>());
+                       pix0 * p->spp, ns, seg[cur_i].as<Segment>());
     PM_HIP(hipGetLastError());
-    Segment* cur = seg_a.as<Segment>();
-    Segment* nxt = seg_b.as<Segment>();
     uint32_t n_cur = ns;
     ++st.batches;
     while (n_cur > 0) {
+      if (2 * static_cast<uint64_t>(n_cur) > 0xffffffffull)
+        return fail(PM_ERR_UNSUPPORTED, "render round too large");
+      if (int e = reserve_round(n_cur, seg[cur_i ^ 1])) return e;
+      Segment* cur = seg[cur_i].as<Segment>();
+      Segment* nxt = seg[cur_i ^ 1].as<Segment>();
       if (int e = zero_words(c32, 4 * sizeof(uint32_t))) return e;
-      ShadeOut O{nxt, c32 + 0, static_cast<uint32_t>(seg_cap), qg.as<pm_query>(),
-                 wg.as<Weighted>(), c32 + 1, qc.as<pm_query>(), wc.as<Weighted>(), c32 + 2,
-                 static_cast<uint32_t>(q_cap), acc.as<unsigned long long>(), bad.as<uint32_t>(),
-                 c32 + 3, rays};
+      ShadeOut O{nxt, c32 + 0, 2 * n_cur, qg.as<pm_query>(), wg.as<Weighted>(), c32 + 1,
+                 qc.as<pm_query>(), wc.as<Weighted>(), c32 + 2, n_cur,
+                 acc.as<unsigned long long>(), bad.as<uint32_t>(), c32 + 3, rays};
       hipLaunchKernelGGL(shade_kernel, dim3(grid_of(n_cur)), dim3(kThreads), 0, nullptr, R, cur,
                          n_cur, O);
       PM_HIP(hipGetLastError());
       uint32_t cnt[4];
       PM_HIP(hipMemcpy(cnt, c32, sizeof(cnt), hipMemcpyDeviceToHost));
-      // the capacities bound the worst case, so this is a guard, not a limit
+      // the capacities bound what a round can produce, so this is a guard
       if (cnt[3]) return fail(PM_ERR_UNSUPPORTED, "render segment or query list overflow");
       ++st.iterations;
       for (int map_i = 0; map_i < 2; ++map_i) {
@@ -649,7 +658,7 @@ int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
         PM_HIP(hipGetLastError());
         (map_i == 0 ? st.gathers_global : st.gathers_caustics) += nq;
       }
-      std::swap(cur, nxt);
+      cur_i ^= 1;
       n_cur = cnt[0];
     }
     hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(np)), dim3(kThreads), 0, nullptr,

@@ -122,31 +122,52 @@ __global__ __launch_bounds__(kSortThreads) void scan_down_kernel(
   }
 }
 
+// Scatter of one digit pass.  The tile's elements are first ranked (stable:
+// wave ballots find equal-digit peers, per-wave digit counts in LDS order
+// the waves) and staged in LDS in digit order; the staged tile is then
+// written out with consecutive lanes on consecutive elements, so each digit's
+// run of the tile lands in consecutive addresses instead of one scattered
+// element per lane.
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift,
     const uint32_t* __restrict__ offsets, int64_t n_tiles) {
-  __shared__ uint32_t run[256];
+  __shared__ uint32_t glob[256];   // first output position of each digit (this tile)
+  __shared__ uint32_t loc[256];    // first staged position of each digit
+  __shared__ uint32_t seen[256];   // elements of each digit staged so far
   __shared__ uint32_t wcnt[kSortThreads / 64][256];
+  __shared__ uint32_t wsum[kSortThreads / 64];
+  __shared__ K skey[kSortTile];
+  __shared__ uint32_t sval[kSortTile];
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
-  run[t] = offsets[static_cast<int64_t>(t) * n_tiles + blockIdx.x];
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+  const int tile_n = static_cast<int>(std::min<int64_t>(kSortTile, n - base));
+  // this tile's count of digit t: the difference of consecutive entries of
+  // the digit-major exclusive scan (the last entry closes at n)
+  const int64_t at = static_cast<int64_t>(t) * n_tiles + blockIdx.x;
+  const uint32_t g = offsets[at];
+  const uint32_t next = (at + 1 < 256 * n_tiles) ? offsets[at + 1] : static_cast<uint32_t>(n);
+  uint32_t tot;
+  const uint32_t l0 = block_exclusive_scan(next - g, wsum, &tot);
+  glob[t] = g;
+  loc[t] = l0;
+  seen[t] = 0;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int r = 0; r < kSortItems; ++r) {
 #pragma unroll
     for (int i = 0; i < kSortThreads / 64; ++i) wcnt[i][t] = 0;
     __syncthreads();
-    const int64_t idx = base + static_cast<int64_t>(r) * kSortThreads + t;
-    const bool valid = idx < n;
+    const int li = r * kSortThreads + t;
+    const bool valid = li < tile_n;
     K key = 0;
     uint32_t val = 0;
     uint32_t d = 0;
     if (valid) {
-      key = kin[idx];
-      val = vin[idx];
+      key = kin[base + li];
+      val = vin[base + li];
       d = static_cast<uint32_t>(key >> shift) & 255u;
     }
     const uint64_t vmask = __ballot(valid);
@@ -161,19 +182,26 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     if (valid && rank == 0) wcnt[w][d] = static_cast<uint32_t>(__popcll(peers));
     __syncthreads();
     if (valid) {
-      uint32_t pos = run[d] + rank;
+      uint32_t pos = loc[d] + seen[d] + rank;
       for (int i = 0; i < w; ++i) pos += wcnt[i][d];
-      kout[pos] = key;
-      vout[pos] = val;
+      skey[pos] = key;
+      sval[pos] = val;
     }
     __syncthreads();
     {
       uint32_t add = 0;
 #pragma unroll
       for (int i = 0; i < kSortThreads / 64; ++i) add += wcnt[i][t];
-      run[t] += add;
+      seen[t] += add;
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  for (int j = t; j < tile_n; j += kSortThreads) {
+    const K key = skey[j];
+    const uint32_t d = static_cast<uint32_t>(key >> shift) & 255u;
+    const uint32_t pos = glob[d] + static_cast<uint32_t>(j) - loc[d];
+    kout[pos] = key;
+    vout[pos] = sval[j];
   }
 }
 

@@ -83,11 +83,35 @@ __device__ __forceinline__ void add_term(unsigned long long* acc, uint32_t* bad,
   if (b > 0.0f) atomicAdd(&acc[3 * s + 2], __double2ull_rn(static_cast<double>(b) * kFixScale));
 }
 
+// Slot for each lane active here in an append-only list: one atomic per wave
+// (the lowest active lane adds the active count), ranks by lane order.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter) {
+  const uint64_t active = __ballot(1);
+  const int leader = __builtin_ctzll(active);
+  const int lane = static_cast<int>(__lane_id());
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(active)));
+  base = __shfl(base, leader);
+  return base + static_cast<uint32_t>(__popcll(active & ((1ull << lane) - 1ull)));
+}
+
+// Adds v (< 2^kBits) of every lane active here to a 64-bit counter, one
+// atomic per wave: the sum is assembled bit plane by bit plane from ballots.
+template <int kBits>
+__device__ __forceinline__ void wave_count_add(unsigned long long* counter, uint32_t v) {
+  const uint64_t active = __ballot(1);
+  unsigned long long sum = 0;
+#pragma unroll
+  for (int b = 0; b < kBits; ++b)
+    sum += static_cast<unsigned long long>(__popcll(__ballot((v >> b) & 1u))) << b;
+  if (static_cast<int>(__lane_id()) == __builtin_ctzll(active)) atomicAdd(counter, sum);
+}
+
 __device__ __forceinline__ void push_segment(Segment* out, uint32_t* n_out, uint32_t cap, V3 o,
                                              V3 d, V3 w, uint32_t sample, uint32_t dim,
                                              uint32_t depth, uint32_t kind, uint32_t lineage,
                                              uint64_t stream, uint32_t* overflow) {
-  const uint32_t at = atomicAdd(n_out, 1u);
+  const uint32_t at = wave_append(n_out);
   if (at >= cap) {
     atomicOr(overflow, 1u);
     return;
@@ -105,7 +129,7 @@ __device__ __forceinline__ void push_segment(Segment* out, uint32_t* n_out, uint
 __device__ __forceinline__ void push_query(pm_query* q, Weighted* wq, uint32_t* n_q, uint32_t cap,
                                            V3 x, V3 ns, V3 wo, int mat, V3 w, uint32_t sample,
                                            uint32_t lineage, uint32_t* overflow) {
-  const uint32_t at = atomicAdd(n_q, 1u);
+  const uint32_t at = wave_append(n_q);
   if (at >= cap) {
     atomicOr(overflow, 1u);
     return;
@@ -328,7 +352,7 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
       }
     }
   }
-  atomicAdd(O.rays, static_cast<unsigned long long>(n_rays));
+  wave_count_add<2>(O.rays, n_rays);
 }
 
 // weight * estimated radiance into the samples
@@ -425,7 +449,7 @@ __global__ __launch_bounds__(kThreads) void path_trace_kernel(RenderDev R, pmx_c
     acc[3 * i + 1] = __double2ull_rn(static_cast<double>(L.y) * kFixScale);
     acc[3 * i + 2] = __double2ull_rn(static_cast<double>(L.z) * kFixScale);
   }
-  atomicAdd(rays, static_cast<unsigned long long>(n_rays));
+  wave_count_add<16>(rays, n_rays);
 }
 
 // Pixel means and variances of the means over the batch's samples; dropped
@@ -485,8 +509,8 @@ int check_params(const pmx_scene* scene, const pmx_camera* cam, const pmx_render
   if (!scene || !cam || !p || !out_mean) return fail(PM_ERR_INVALID_ARG, "null argument");
   if (p->out_mem != PM_MEM_DEVICE && p->out_mem != PM_MEM_HOST)
     return fail(PM_ERR_INVALID_ARG, "bad out_mem");
-  if (p->width < 1 || p->height < 1 || p->spp < 1 || p->max_depth < 0)
-    return fail(PM_ERR_INVALID_ARG, "bad image size, spp or depth");
+  if (p->width < 1 || p->height < 1 || p->spp < 1 || p->max_depth < 0 || p->max_depth > 65535)
+    return fail(PM_ERR_INVALID_ARG, "bad image size, spp or depth (max_depth <= 65535)");
   if (static_cast<int64_t>(p->width) * p->height * p->spp > (int64_t{1} << 40))
     return fail(PM_ERR_INVALID_ARG, "too many samples");
   if (p->batch_pixels < 0) return fail(PM_ERR_INVALID_ARG, "negative batch size");

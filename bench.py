@@ -152,6 +152,85 @@ def plain_config_bench(ctx, with_cpu, budget_s=10.0):
     return out
 
 
+def water_diff_bench(ctx, dev, k=100, reps=3):
+    """The differentiation configuration's gather on the water scene (131,084
+    triangles, glass IOR 1.33): 1,000,000 traced global photons (seed 0),
+    10,000,000 emitted caustic paths (their map built and sized), and the
+    final-gathering queries of 512 x 512 x 64 spp with the primary hit's
+    material (the final-gathering albedo factor of the adjoint).  Times the
+    forward estimate and the adjoint in both modes (keys reused, kNN
+    recomputed) with HIP events, like the main measurement."""
+    import numpy as np
+    import torch
+
+    import photon_mapping_amd as pm
+    from photon_mapping_amd import _lib
+    from photon_mapping_amd import scene as sc
+
+    s = sc.water_scene(0)
+    out = {"workload": "water scene (131,084 triangles), 1M global photons, 10M caustic paths, "
+                       f"k={k}, 512x512x64 final-gathering queries"}
+    with sc.NativeScene(s, ctx) as ns:
+        t0 = time.perf_counter()
+        pos, wi, flux, n_emit = sc.trace_photons(ns, 1_000_000, seed=0)
+        torch.cuda.synchronize()
+        out["global_trace_ms"] = 1e3 * (time.perf_counter() - t0)
+        out["global_paths"] = int(n_emit)
+        t0 = time.perf_counter()
+        cmap, n_c = sc.trace_photon_map(ns, 10_000_000, seed=0, kind="caustics")
+        cmap.build()
+        torch.cuda.synchronize()
+        out["caustics_trace_build_ms"] = 1e3 * (time.perf_counter() - t0)
+        out["caustic_photons"] = int(n_c)
+        q, fg = sc.camera_queries(ns, 512, 512, 64, depth=1, seed=0, want_fg_mat=True)
+    pmap = pm.PhotonMap(ctx)
+    pmap.set_photons_soa(pos, wi, flux)
+    pmap.build()
+    mats = torch.from_numpy(s.materials()).to(dev)
+    m = int(q.shape[0])
+    stream = torch.cuda.current_stream(dev)
+    sptr = _lib.C.c_void_p(stream.cuda_stream)
+    rgb = torch.empty((m, 3), dtype=torch.float32, device=dev)
+    r2 = torch.empty((m,), dtype=torch.float32, device=dev)
+    key = torch.empty((m,), dtype=torch.int64, device=dev)
+    g = torch.randn((m, 3), dtype=torch.float32, device=dev,
+                    generator=torch.Generator(device=dev).manual_seed(0))
+    dflux = torch.zeros((3, 1_000_000), dtype=torch.float32, device=dev)
+    dalb = torch.zeros((mats.shape[0], 3), dtype=torch.float32, device=dev)
+
+    def fwd():
+        _lib.call("pm_gather_radiance", pmap.handle, m, _lib.ptr(q), _lib.ptr(mats),
+                  mats.shape[0], k, float(n_emit), _lib.ptr(rgb), _lib.ptr(r2), _lib.ptr(key),
+                  _lib.PM_MEM_DEVICE, sptr)
+
+    def adj(kk):
+        _lib.call("pm_gather_adjoint", pmap.handle, m, _lib.ptr(q), _lib.ptr(mats),
+                  mats.shape[0], k, float(n_emit), _lib.ptr(g), _lib.ptr(kk), _lib.ptr(fg),
+                  _lib.ptr(dflux), _lib.ptr(dalb), _lib.PM_MEM_DEVICE, sptr)
+
+    fwd()
+    adj(key)
+    torch.cuda.synchronize()
+    ctx.set_option(_lib.PM_OPT_TIMING, 1)
+    times = {"forward": [], "reuse": [], "recompute": []}
+    for _ in range(reps):
+        fwd()
+        st = ctx.stats()
+        times["forward"].append(st["sort_ms"] + st["gather_ms"] + st["post_ms"])
+        for name, kk in (("reuse", key), ("recompute", None)):
+            adj(kk)
+            st = ctx.stats()
+            times[name].append(st["sort_ms"] + st["gather_ms"] + st["post_ms"])
+    ctx.set_option(_lib.PM_OPT_TIMING, 0)
+    f = float(np.mean(times["forward"]))
+    out.update({"forward_ms": f, "forward_queries_per_s": m / f * 1e3,
+                "adjoint_reuse_ms": float(np.mean(times["reuse"])),
+                "adjoint_recompute_ms": float(np.mean(times["recompute"]))})
+    out["fwd_plus_adj_over_fwd_reuse"] = (f + out["adjoint_reuse_ms"]) / f
+    out["fwd_plus_adj_over_fwd_recompute"] = (f + out["adjoint_recompute_ms"]) / f
+    return out
+
+
 def batched_bench(ctx, dev, n_inst=64, frames=3, seed=0):
     """Config-4 batching workload: n_inst box-scene instances of uneven size
     (wall albedos uniform in [0.1, 0.9]; stored photons log-uniform in
@@ -508,6 +587,8 @@ def main() -> int:
         adj["fwd_plus_adj_over_fwd_reuse"] = (fwd_ms + adj["adjoint_reuse_ms"]) / fwd_ms
         adj["fwd_plus_adj_over_fwd_recompute"] = (fwd_ms + adj["adjoint_recompute_ms"]) / fwd_ms
         log(f"adjoint: {adj}")
+        adj["water_scene"] = water_diff_bench(ctx, dev, k)
+        log(f"adjoint (water scene): {adj['water_scene']}")
 
     batched = None
     if not args.no_batched:

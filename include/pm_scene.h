@@ -98,7 +98,10 @@ int pmx_scene_intersect(const pmx_scene* scene, int64_t n, const float* org_soa,
  * depth 1: the final-gathering secondary hit of a cosine-sampled bounce from
  * the primary hit (include/integrator.h:215-244).  Camera samples that miss,
  * or hit a light or a non-Lambert surface at the primary vertex, and bounces
- * that escape, are redrawn.  cam is host memory; out (m pm_query records)
+ * that escape, are redrawn; after 64 draws without a query point the record
+ * is a placeholder with material -1 (its estimate is zero).  Final-gathering
+ * rays are followed through up to 16 specular bounces, each sampled as the
+ * material does for camera paths.  cam is host memory; out (m pm_query records)
  * and fg_mat_opt (m int32: material of the primary hit, may be NULL) are
  * device memory.  Runs on the null stream and synchronises. */
 int pmx_fg_queries(const pmx_scene* scene, const pmx_camera* cam, int width, int height,

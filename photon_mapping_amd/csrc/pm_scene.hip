@@ -56,7 +56,9 @@ __global__ __launch_bounds__(kThreads) void fg_queries_kernel(
            up = ld3(cam.up);
   const V3 pinhole = pos + fwd * cam.focal;
   V3 qp = v3(0.f, 0.f, 0.f), qn = v3(0.f, 1.f, 0.f), qwo = v3(0.f, 1.f, 0.f);
-  int qmat = 0, pmat = -1;
+  // a sample that finds no query point in 64 draws keeps this placeholder:
+  // material -1, whose estimate is zero
+  int qmat = -1, pmat = -1;
   uint32_t dim = 0;
   for (int attempt = 0; attempt < 64; ++attempt) {
     // camera sample (examples/main.cpp:40-41, include/camera.h:41-47)

@@ -140,3 +140,22 @@ def test_single_triangle_scene(gpu):
     _same(bvh, brute)
     want = so.closest_hits(s.tris["v"], org, d)
     _same(bvh, list(want))
+
+
+def test_camera_queries_land_on_lambert_faces(gpu):
+    """Final-gathering queries of the caustics scene lie on Lambert faces
+    (specular hits are followed, not queried); a scene the camera sees only
+    as a mirror yields placeholders of material -1."""
+    from photon_mapping_amd import _lib
+    from photon_mapping_amd import scene as sc
+
+    s = sc.caustics_scene()
+    q = sc.camera_queries(s, 32, 32, 4, depth=1, seed=1).cpu().numpy()
+    mat = q[:, 9].view(np.int32)
+    assert (mat >= 0).all()
+    assert (s.mat_types[mat] == _lib.PM_MATERIAL_LAMBERT).all()
+    m = sc.box_scene()
+    m.tris = m.tris[:12].copy()                     # walls and light only
+    m.mat_types = np.full(len(m.albedo), _lib.PM_MATERIAL_MIRROR, np.int32)
+    q = sc.camera_queries(m, 8, 8, 1, depth=0, seed=1).cpu().numpy()
+    assert (q[:, 9].view(np.int32) == -1).all()

@@ -152,8 +152,10 @@ def test_camera_queries_land_on_lambert_faces(gpu):
     s = sc.caustics_scene()
     q = sc.camera_queries(s, 32, 32, 4, depth=1, seed=1).cpu().numpy()
     mat = q[:, 9].view(np.int32)
-    assert (mat >= 0).all()
-    assert (s.mat_types[mat] == _lib.PM_MATERIAL_LAMBERT).all()
+    # samples of pixels that see only a sphere redraw in vain: placeholders
+    found = mat >= 0
+    assert found.mean() > 0.7 and (mat[~found] == -1).all()
+    assert (s.mat_types[mat[found]] == _lib.PM_MATERIAL_LAMBERT).all()
     m = sc.box_scene()
     m.tris = m.tris[:12].copy()                     # walls and light only
     m.mat_types = np.full(len(m.albedo), _lib.PM_MATERIAL_MIRROR, np.int32)

@@ -117,7 +117,9 @@ def test_gather_box_scene_workload(gpu, oracle):
     torch.cuda.synchronize()
     pos, wi, flux = (t.cpu().numpy() for t in (pos_t, wi_t, flux_t))
     q = q_t.cpu().numpy()
-    assert np.isfinite(q).all() and np.isfinite(pos).all()
+    # columns 0-8 are floats; column 9 holds the material as int32 bits (-1:
+    # a camera sample that never found the box, whose estimate is zero)
+    assert np.isfinite(q[:, :9]).all() and np.isfinite(pos).all()
     mats = s.materials()
     pmap = gpu.PhotonMap()
     pmap.set_photons_soa(pos_t, wi_t, flux_t)

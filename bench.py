@@ -343,7 +343,7 @@ def ray_bench(ctx, dev, seed=0):
 
             def shoot():
                 _lib.call("pmx_scene_intersect", ns.handle, n, _lib.ptr(org), _lib.ptr(d), 0,
-                          _lib.ptr(tri), _lib.ptr(t), None, None, None)
+                          _lib.ptr(tri), _lib.ptr(t), None, None, _lib.PM_MEM_DEVICE, None)
 
             shoot()
             torch.cuda.synchronize()

@@ -94,6 +94,18 @@ class BasicCamera {
   pmx_camera cam{};
 };
 
+// Hit record of Scene::intersect (include/core.h SurfaceInfo / IntersectInfo,
+// without the primitive pointer: primID indexes Scene::triangles()).
+template <typename V>
+struct BasicIntersectInfo {
+  float t = 0.0f;
+  int primID = -1;
+  V position;
+  V geometricNormal;
+  V shadingNormal;   // interpolated, not renormalised (include/triangle.h:64-71)
+  float barycentric[2] = {0.0f, 0.0f};
+};
+
 // Triangle scene (include/scene.h:54-356).
 class Scene {
  public:
@@ -220,6 +232,56 @@ class Scene {
                            mats.data(), ior.data(), static_cast<int>(mats.size()), nullptr,
                            mat_le.data(), &scene),
           "pmx_scene_create");
+  }
+
+  // Scene::intersect (include/scene.h:307-348) for one ray: the closest hit
+  // in (1e-5, inf) over both faces.  Stages through the device and
+  // synchronises; batches of rays go through intersectBatch.
+  template <typename V>
+  bool intersect(const BasicRay<V>& ray, BasicIntersectInfo<V>& info) const {
+    std::vector<int32_t> prim;
+    std::vector<float> t, u, v;
+    intersectBatch(std::vector<BasicRay<V>>{ray}, prim, t, u, v);
+    if (prim[0] < 0) return false;
+    const pmx_triangle& tr = tris[prim[0]];
+    info.t = t[0];
+    info.primID = prim[0];
+    info.barycentric[0] = u[0];
+    info.barycentric[1] = v[0];
+    float e1[3], e2[3], ng[3];
+    for (int a = 0; a < 3; ++a) {
+      info.position[a] = ray.origin[a] + t[0] * ray.direction[a];
+      info.shadingNormal[a] =
+          tr.n[0][a] * (1.0f - u[0] - v[0]) + tr.n[1][a] * u[0] + tr.n[2][a] * v[0];
+      e1[a] = tr.v[1][a] - tr.v[0][a];
+      e2[a] = tr.v[2][a] - tr.v[0][a];
+    }
+    detail::cross3(e1, e2, ng);
+    detail::normalize3(ng);
+    for (int a = 0; a < 3; ++a) info.geometricNormal[a] = ng[a];
+    return true;
+  }
+
+  // Closest hits of many rays: triangle index (-1: miss), t, barycentrics.
+  template <typename V>
+  void intersectBatch(const std::vector<BasicRay<V>>& rays, std::vector<int32_t>& prim,
+                      std::vector<float>& t, std::vector<float>& u,
+                      std::vector<float>& v) const {
+    const size_t n = rays.size();
+    std::vector<float> org(3 * n), dir(3 * n);
+    for (size_t i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) {
+        org[a * n + i] = rays[i].origin[a];
+        dir[a * n + i] = rays[i].direction[a];
+      }
+    prim.assign(n, -1);
+    t.assign(n, 0.0f);
+    u.assign(n, 0.0f);
+    v.assign(n, 0.0f);
+    check(pmx_scene_intersect(handle(), static_cast<int64_t>(n), org.data(), dir.data(),
+                              PMX_HIT_BVH, prim.data(), t.data(), u.data(), v.data(),
+                              PM_MEM_HOST, nullptr),
+          "pmx_scene_intersect");
   }
 
   uint32_t nVertices() const { return static_cast<uint32_t>(3 * tris.size()); }
@@ -465,6 +527,7 @@ class BasicPhotonMapping : public BasicIntegrator<V> {
 
 #ifndef PM_NO_GLOBAL_NAMES
 using Ray = pm::BasicRay<PM_VEC3F_TYPE>;
+using IntersectInfo = pm::BasicIntersectInfo<PM_VEC3F_TYPE>;
 using Camera = pm::BasicCamera<PM_VEC3F_TYPE>;
 using Scene = pm::Scene;
 using Image = pm::BasicImage<PM_VEC3F_TYPE>;

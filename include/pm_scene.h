@@ -85,11 +85,13 @@ int pmx_scene_get_info(const pmx_scene* scene, pmx_scene_info* out);
  * ray i with origin org_soa[{0,n,2n}+i] and direction dir_soa[...], the
  * triangle with the smallest t in (1e-5, inf) of a two-sided
  * Moller-Trumbore test, ties to the lower triangle index; out_tri = -1 and
- * out_t = inf on a miss.  out_u / out_v are the barycentrics of vertices 1
- * and 2.  All arrays are device memory; asynchronous on the stream. */
+ * out_t = inf on a miss.  out_u / out_v (may be NULL) are the barycentrics
+ * of vertices 1 and 2.  mem PM_MEM_DEVICE: every array is device memory and
+ * the call is asynchronous on the stream; PM_MEM_HOST: host arrays, staged
+ * through device memory, synchronous. */
 int pmx_scene_intersect(const pmx_scene* scene, int64_t n, const float* org_soa,
                         const float* dir_soa, int method, int32_t* out_tri, float* out_t,
-                        float* out_u, float* out_v, void* stream);
+                        float* out_u, float* out_v, int mem, void* stream);
 
 /* ---- workload producers over a scene -------------------------------------- */
 

@@ -114,7 +114,7 @@ SIGNATURES = {
     "pmx_scene_create": (None, [VP, VP, I32, VP, VP, I32, VP, VP, C.POINTER(VP)]),
     "pmx_scene_destroy": (None, [VP]),
     "pmx_scene_get_info": (None, [VP, VP]),
-    "pmx_scene_intersect": (None, [VP, I64, VP, VP, I32, VP, VP, VP, VP, VP]),
+    "pmx_scene_intersect": (None, [VP, I64, VP, VP, I32, VP, VP, VP, VP, I32, VP]),
     "pmx_fg_queries": (None, [VP, VP, I32, I32, I32, I32, C.c_uint64, VP, VP]),
     "pmx_trace_photons": (None, [VP, I32, I64, I32, C.c_uint64, VP, VP, VP, VP]),
     "pmx_trace_photon_map": (None, [VP, I32, I64, I32, C.c_uint64, C.POINTER(VP), VP]),

@@ -419,19 +419,46 @@ int pmx_scene_get_info(const pmx_scene* scene, pmx_scene_info* out) {
 
 int pmx_scene_intersect(const pmx_scene* scene, int64_t n, const float* org_soa,
                         const float* dir_soa, int method, int32_t* out_tri, float* out_t,
-                        float* out_u, float* out_v, void* stream) {
+                        float* out_u, float* out_v, int mem, void* stream) {
   if (!scene) return fail(PM_ERR_INVALID_ARG, "null scene");
   if (n < 0) return fail(PM_ERR_INVALID_ARG, "negative ray count");
   if (method != PMX_HIT_BVH && method != PMX_HIT_BRUTE_FORCE)
     return fail(PM_ERR_INVALID_ARG, "unknown hit method");
+  if (mem != PM_MEM_DEVICE && mem != PM_MEM_HOST) return fail(PM_ERR_INVALID_ARG, "bad mem");
   if (n == 0) return PM_OK;
   if (!org_soa || !dir_soa || !out_tri || !out_t) return fail(PM_ERR_INVALID_ARG, "null array");
   const auto* s = reinterpret_cast<const SceneImpl*>(scene);
   PM_HIP(hipSetDevice(s->ctx->device));
-  hipLaunchKernelGGL(intersect_kernel, dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)),
-                     dim3(kThreads), 0, static_cast<hipStream_t>(stream), s->dev(method), n,
-                     org_soa, dir_soa, out_tri, out_t, out_u, out_v);
+  const auto hs = static_cast<hipStream_t>(stream);
+  const dim3 grid(static_cast<unsigned>((n + kThreads - 1) / kThreads));
+  if (mem == PM_MEM_DEVICE) {
+    hipLaunchKernelGGL(intersect_kernel, grid, dim3(kThreads), 0, hs, s->dev(method), n,
+                       org_soa, dir_soa, out_tri, out_t, out_u, out_v);
+    PM_HIP(hipGetLastError());
+    return PM_OK;
+  }
+  // host arrays: stage through device memory and synchronise
+  const size_t v3 = 3 * static_cast<size_t>(n) * sizeof(float);
+  const size_t v1 = static_cast<size_t>(n) * sizeof(float);
+  DevBuf d_org, d_dir, d_tri, d_t, d_u, d_v;
+  int rc = d_org.reserve(v3);
+  if (!rc) rc = d_dir.reserve(v3);
+  if (!rc) rc = d_tri.reserve(v1);
+  if (!rc) rc = d_t.reserve(v1);
+  if (!rc && out_u) rc = d_u.reserve(v1);
+  if (!rc && out_v) rc = d_v.reserve(v1);
+  if (rc) return rc;
+  PM_HIP(hipMemcpyAsync(d_org.ptr, org_soa, v3, hipMemcpyHostToDevice, hs));
+  PM_HIP(hipMemcpyAsync(d_dir.ptr, dir_soa, v3, hipMemcpyHostToDevice, hs));
+  hipLaunchKernelGGL(intersect_kernel, grid, dim3(kThreads), 0, hs, s->dev(method), n,
+                     d_org.as<float>(), d_dir.as<float>(), d_tri.as<int32_t>(), d_t.as<float>(),
+                     out_u ? d_u.as<float>() : nullptr, out_v ? d_v.as<float>() : nullptr);
   PM_HIP(hipGetLastError());
+  PM_HIP(hipMemcpyAsync(out_tri, d_tri.ptr, v1, hipMemcpyDeviceToHost, hs));
+  PM_HIP(hipMemcpyAsync(out_t, d_t.ptr, v1, hipMemcpyDeviceToHost, hs));
+  if (out_u) PM_HIP(hipMemcpyAsync(out_u, d_u.ptr, v1, hipMemcpyDeviceToHost, hs));
+  if (out_v) PM_HIP(hipMemcpyAsync(out_v, d_v.ptr, v1, hipMemcpyDeviceToHost, hs));
+  PM_HIP(hipStreamSynchronize(hs));
   return PM_OK;
 }
 

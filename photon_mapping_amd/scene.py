@@ -412,7 +412,7 @@ def intersect(scene, org, dirs, brute_force: bool = False):
     t, u, v = (torch.empty((n,), dtype=torch.float32, device=org.device) for _ in range(3))
     with _Native(scene) as ns:
         call("pmx_scene_intersect", ns.handle, n, ptr(o), ptr(d), int(bool(brute_force)),
-             ptr(tri), ptr(t), ptr(u), ptr(v), None)
+             ptr(tri), ptr(t), ptr(u), ptr(v), _lib.PM_MEM_DEVICE, None)
         torch.cuda.synchronize(org.device)
     return tri, t, u, v
 

@@ -30,6 +30,16 @@ int main(int argc, char** argv) {
     for (const float* f : {c.position, c.forward, c.right, c.up})
       std::printf("%a %a %a ", f[0], f[1], f[2]);
     std::printf("%a\n", c.focal);
+    // a few closest hits through Scene::intersect, printed as "prim t"
+    for (int i = 0; i < 4; ++i) {
+      float uv[2] = {-0.75f + 0.5f * i, 0.1f * i}, pdf;
+      Ray ray;
+      camera.sampleRay(uv, ray, pdf);
+      IntersectInfo info;
+      if (scene.intersect(ray, info)) std::printf("%d %a ", info.primID, info.t);
+      else std::printf("-1 0 ");
+    }
+    std::printf("\n");
     UniformSampler sampler(seed);
     if (mode == "pt") {
       PathTracing integrator(100);

@@ -99,6 +99,8 @@ def test_dropin_render_matches_python_binding(tmp_path, mode):
     assert out.returncode == 0, out.stderr
     lines = out.stdout.strip().splitlines()
     cam = [float.fromhex(x) for x in lines[0].split()]
+    hits = lines[1].split()
+    lines = lines[:1] + lines[2:]
     img = np.array([float.fromhex(x) for x in lines[1:1 + 3 * w * h]],
                    np.float32).reshape(h, w, 3)
     s = sc.load_obj(obj, camera=dict(position=cam[0:3], forward=cam[3:6], right=cam[6:9],
@@ -119,3 +121,21 @@ def test_dropin_render_matches_python_binding(tmp_path, mode):
             assert (tmp_path / "dropin_render.ppm").read_text().startswith(f"P3\n{w} {h}\n255\n")
     assert np.array_equal(img, ref.cpu().numpy())
     assert img.mean() > 0
+    # Scene::intersect on the camera rays matches the batched query
+    import torch
+
+    pos = np.asarray(cam[0:3], np.float32)
+    fwd, rgt, up = (np.asarray(cam[a:a + 3], np.float32) for a in (3, 6, 9))
+    org, dirs = [], []
+    for i in range(4):
+        uv = (np.float32(-0.75 + 0.5 * i), np.float32(0.1 * i))
+        sensor = pos + uv[0] * rgt + uv[1] * up
+        d = pos + np.float32(cam[12]) * fwd - sensor
+        org.append(sensor)
+        dirs.append(d / np.sqrt(np.float32((d * d).sum())))
+    tri, t, _, _ = sc.intersect(s, torch.tensor(np.array(org)).cuda(),
+                                torch.tensor(np.array(dirs)).cuda())
+    got = [(int(hits[2 * i]), float.fromhex(hits[2 * i + 1])) for i in range(4)]
+    for i in range(4):
+        assert got[i][0] == int(tri[i])
+        assert abs(got[i][1] - float(t[i])) <= 1e-5 * abs(float(t[i]))

@@ -51,6 +51,40 @@ __device__ __forceinline__ uint32_t morton3(float x, float y, float z, const flo
          expand10(quantize(z, lo[2], scale[2]));
 }
 
+// Hilbert index of the 10-bit quantized position (Skilling 2004, "Programming
+// the Hilbert curve", AIP Conf. Proc. 707: axes to transposed index, then the
+// transpose's bits interleaved like a Morton code).  Consecutive indices are
+// always face-adjacent cells, so runs of sorted queries stay compact where a
+// Morton order jumps.
+__device__ __forceinline__ uint32_t hilbert3(float x, float y, float z, const float* lo,
+                                             const float* scale) {
+  uint32_t X[3] = {quantize(x, lo[0], scale[0]), quantize(y, lo[1], scale[1]),
+                   quantize(z, lo[2], scale[2])};
+  constexpr uint32_t M = 1u << (kMortonBits - 1);
+  for (uint32_t Q = M; Q > 1; Q >>= 1) {
+    const uint32_t P = Q - 1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        const uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  X[1] ^= X[0];
+  X[2] ^= X[1];
+  uint32_t t = 0;
+  for (uint32_t Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  X[0] ^= t;
+  X[1] ^= t;
+  X[2] ^= t;
+  return (expand10(X[0]) << 2) | (expand10(X[1]) << 1) | expand10(X[2]);
+}
+
 // Per-block min/max of positions plus a count of non-finite coordinates.
 __global__ __launch_bounds__(kThreads) void bbox_partial_kernel(
     const float* __restrict__ pos, int64_t n, float* __restrict__ partial,
@@ -204,10 +238,15 @@ __device__ __forceinline__ int find_inst(const int64_t* __restrict__ prefix, int
   return lo;
 }
 
+// Sort key of every query, and its nearest radius anchor: the last anchor
+// whose first Morton code is <= the query's Morton code (upper bound, one
+// back).  Both are computed in original query order, so the position reads
+// are coalesced; q_anchor is indexed by global query index.
 template <typename K>
 __global__ __launch_bounds__(kThreads) void query_keys_kernel(
     const GatherDesc* __restrict__ descs, const int64_t* __restrict__ q_prefix,
-    int n_inst, int64_t total, K* __restrict__ keys, uint32_t* __restrict__ vals) {
+    int n_inst, int64_t total, K* __restrict__ keys, uint32_t* __restrict__ vals,
+    uint32_t* __restrict__ q_anchor) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= total) return;
   const int inst = n_inst > 1 ? find_inst(q_prefix, n_inst, i) : 0;
@@ -220,26 +259,15 @@ __global__ __launch_bounds__(kThreads) void query_keys_kernel(
   } else {
     x = D.q_soa[li]; y = D.q_soa[D.m + li]; z = D.q_soa[2 * D.m + li];
   }
-  const uint32_t code = morton3(x, y, z, D.qlo, D.qscale);
+  // queries run in Hilbert order (the photons and anchors stay in Morton order)
+  const uint32_t code = hilbert3(x, y, z, D.qlo, D.qscale);
   keys[i] = static_cast<K>((static_cast<uint64_t>(inst) << 32) | code);
   vals[i] = static_cast<uint32_t>(i);
-}
-
-// Nearest radius anchor in Morton order of every sorted query: the last
-// anchor whose first code is <= the query's code (upper bound, one back).
-template <typename K>
-__global__ __launch_bounds__(kThreads) void query_anchor_kernel(
-    const GatherDesc* __restrict__ descs, const int64_t* __restrict__ q_prefix, int n_inst,
-    int64_t total_q, const K* __restrict__ sorted_keys, uint32_t* __restrict__ q_anchor) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (i >= total_q) return;
-  const int inst = n_inst > 1 ? find_inst(q_prefix, n_inst, i) : 0;
-  const GatherDesc& D = descs[inst];
-  const uint32_t code = static_cast<uint32_t>(sorted_keys[i] & 0xffffffffu);
+  const uint32_t mc = morton3(x, y, z, D.qlo, D.qscale);
   int lo = 0, hi = D.n_anchors;
   while (lo < hi) {
     const int m = (lo + hi) >> 1;
-    if (D.anchor_code[m] <= code) lo = m + 1;
+    if (D.anchor_code[m] <= mc) lo = m + 1;
     else hi = m;
   }
   q_anchor[i] = static_cast<uint32_t>(lo > 0 ? lo - 1 : 0);
@@ -542,25 +570,19 @@ int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
     uint32_t* k0 = ws->keys.as<uint32_t>();
     uint32_t* k1 = ws->keys2.as<uint32_t>();
     hipLaunchKernelGGL(query_keys_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, n_inst, total_q, k0, v0);
+                       dev_q_prefix, n_inst, total_q, k0, v0, ws->q_anchor.as<uint32_t>());
     PM_HIP(hipGetLastError());
     if (int rc = radix_sort_u32(ws, k0, v0, k1, v1, total_q, 3 * kMortonBits, s, &which))
       return rc;
-    hipLaunchKernelGGL(query_anchor_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, n_inst, total_q, which ? k1 : k0,
-                       ws->q_anchor.as<uint32_t>());
   } else {
     uint64_t* k0 = ws->keys.as<uint64_t>();
     uint64_t* k1 = ws->keys2.as<uint64_t>();
     int ib = 1;
     while ((1 << ib) < n_inst) ++ib;
     hipLaunchKernelGGL(query_keys_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, n_inst, total_q, k0, v0);
+                       dev_q_prefix, n_inst, total_q, k0, v0, ws->q_anchor.as<uint32_t>());
     PM_HIP(hipGetLastError());
     if (int rc = radix_sort_u64(ws, k0, v0, k1, v1, total_q, 32 + ib, s, &which)) return rc;
-    hipLaunchKernelGGL(query_anchor_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, n_inst, total_q, which ? k1 : k0,
-                       ws->q_anchor.as<uint32_t>());
   }
   PM_HIP(hipGetLastError());
   *perm_out = which ? v1 : v0;

@@ -58,7 +58,7 @@ struct DevBuf {
 
 // Device scratch of one gather call: query keys and permutation (double
 // buffered for the radix sort), sort histograms and scan partials, the
-// nearest anchor of every sorted query, and the work-queue / statistics
+// nearest anchor of every query, and the work-queue / statistics
 // counters.
 struct Workspace {
   DevBuf keys, vals, keys2, vals2, counts, partials, q_anchor, counters;
@@ -80,7 +80,7 @@ struct alignas(16) GatherDesc {
   const pm_query* q;       // RADIANCE / ADJOINT: local query records
   const float* q_soa;      // KNN: 3*m positions
   const pm_material* mats;
-  const uint32_t* q_anchor;  // sorted query position -> nearest anchor (Morton order)
+  const uint32_t* q_anchor;  // local query index -> nearest anchor (Morton order)
   // outputs
   float* out_rgb;
   float* out_r2;
@@ -218,9 +218,9 @@ int update_flux(PhotonMapImpl* map, const float* flux_dev, hipStream_t s);
 int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** out);
 void fill_desc_map_part(const PhotonMapImpl* map, GatherDesc* d);
 
-// Morton keys for all queries of all instances, stable sort, and the
-// nearest radius anchor of every sorted query (ws->q_anchor, global sorted
-// order).
+// Hilbert keys for all queries of all instances, stable sort, and the
+// nearest radius anchor of every query (ws->q_anchor, global query order;
+// instance i starts at q_prefix[i], which is also its first sorted position).
 // Reserves every buffer prepare_queries and the gather launch use.
 int prepare_reserve(Workspace* ws, int n_inst, int64_t total_q, int64_t total_groups);
 int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,

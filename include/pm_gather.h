@@ -91,7 +91,7 @@ typedef struct pm_stats {
 
 /* Options for pm_context_set_option. */
 #define PM_OPT_TIMING 1         /* 0/1: record HIP events (syncs stream)    */
-#define PM_OPT_WAVES_PER_CU 2   /* persistent waves per CU (default 8)      */
+#define PM_OPT_WAVES_PER_CU 2   /* persistent waves per CU (default 12)     */
 #define PM_OPT_ANCHOR_STRIDE 3  /* photons per radius anchor (default 8)    */
 #define PM_OPT_LEAF_SIZE 4      /* default kd-tree leaf size (default 8)    */
 #define PM_OPT_XCD_AWARE 5      /* 0/1: per-XCD work queues (default 1)     */
@@ -220,7 +220,7 @@ int pm_plan_destroy(pm_plan* plan);
  * Times pm_gather_radiance on the caller's workload (device pointers) under a
  * small grid of settings, coordinate by coordinate: leaf size {8, 16, 32}
  * (the map is rebuilt), query-run split factor {0.5 .. 1.25} and persistent
- * waves per CU {4, 6, 8, 12}.  The fastest setting stays in force (map leaf
+ * waves per CU {6, 8, 10, 12, 16}.  The fastest setting stays in force (map leaf
  * size, context options); results do not depend on it.  Synchronises. */
 typedef struct pm_tune_report {
   int32_t leaf_size;    /* chosen leaf size                                  */

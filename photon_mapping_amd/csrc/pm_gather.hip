@@ -68,7 +68,7 @@ constexpr int kFront = 256;       // breadth-first frontier capacity
 constexpr int kDrhoSlots = 64;    // adjoint: materials accumulated in LDS
 constexpr int kCentreBins = 256;  // centre histogram bins
 constexpr int kSlotCap = 512;     // photons of an explicit slot list
-// A candidate list is either nr subtrees (sh.rnode / sh.rpre) or, when nr
+// A candidate list is either nr subtrees (sh.rl.rnode / sh.rl.rpre) or, when nr
 // is this value, an explicit list of photon slots (sh.cslot).
 constexpr uint32_t kSlotList = 0xffffffffu;
 
@@ -86,8 +86,21 @@ __device__ __forceinline__ float unif(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+// Subtree list: roots and the exclusive prefix of their slot counts.
+struct RangeList {
+  uint32_t rnode[kMaxRanges];
+  uint32_t rpre[kMaxRanges];
+};
+
+// One wave's LDS (one-wave workgroups).  Buffers whose lifetimes do not
+// overlap share storage, which keeps the block at 13 KB so that 12 waves
+// fit a CU's 160 KB.
 struct GatherShared {
-  uint32_t hist[(kBins + 2) * kWave];  // per-lane bins; centre bins alias the start
+  union {
+    uint32_t hist[(kBins + 2) * kWave];  // per-lane bins; centre bins alias the start
+    uint64_t stash[kStashCap * kWave];   // pass B / finish: per-lane keys, entry-major
+    uint32_t cscratch[kSlotCap];         // collect_slots output before it is accepted
+  };
   // The chunk staging buffers are dead while a candidate list is built, so
   // the breadth-first frontier shares their storage.
   union {
@@ -95,15 +108,22 @@ struct GatherShared {
       float4 pos[kChunk];
       float4 wif[kChunk];
       float2 flux[kChunk];
-      uint32_t slot[kChunk];
     } c;
     uint32_t front[2][kFront];
   } u;
-  uint32_t rnode[kMaxRanges];
-  uint32_t rpre[kMaxRanges + 1];
-  uint32_t cslot[kSlotCap];
+  // A candidate list is a subtree list or an explicit slot list, never both.
+  union {
+    RangeList rl;
+    uint32_t cslot[kSlotCap];
+  };
+};
+// The adjoint kernel also accumulates dL/drho per material in LDS.
+struct AdjointShared : GatherShared {
   float drho[3 * kDrhoSlots];
 };
+static_assert(kStashCap * sizeof(uint64_t) <= (kBins + 2) * sizeof(uint32_t),
+              "the stash fits the lane bins");
+static_assert(kSlotCap * sizeof(uint32_t) <= sizeof(RangeList), "slot list fits the range list");
 static_assert(kCentreBins <= (kBins + 2) * kWave, "centre bins alias the lane bins");
 
 __device__ __forceinline__ float d2_ref(float qx, float qy, float qz, float px,
@@ -253,29 +273,18 @@ struct SelState {
   uint32_t cnt;   // number of candidate keys inside [klo, khi)
 };
 
-// Per-lane stash of the keys inside [klo, khi), in registers: every access
-// uses a compile-time index, inserts go through selects.  With kSlots the
-// photon's slot is kept beside its key.
-template <bool kSlots>
+// Per-lane stash of the keys inside [klo, khi), in LDS: entry e of lane l
+// is sh.stash[e * kWave + l] (the lane bins are dead while it is in use).
+// Keys past kStashCap are dropped; the selection keeps the count of the
+// stashed bin at or below it.
 struct Stash {
-  uint64_t key[kStashCap];
-  uint32_t slot[kSlots ? kStashCap : 1];
   uint32_t n;
-  __device__ __forceinline__ void reset() {
-#pragma unroll
-    for (int i = 0; i < kStashCap; ++i) key[i] = kKeyMax;
-#pragma unroll
-    for (int i = 0; i < (kSlots ? kStashCap : 1); ++i) slot[i] = 0u;
-    n = 0;
-  }
-  __device__ __forceinline__ void put(uint64_t k, uint32_t s) {
-#pragma unroll
-    for (int i = 0; i < kStashCap; ++i) {
-      const bool here = n == static_cast<uint32_t>(i);
-      key[i] = here ? k : key[i];
-      if (kSlots) slot[i] = here ? s : slot[i];
+  __device__ __forceinline__ void reset() { n = 0; }
+  __device__ __forceinline__ void put(GatherShared& sh, int lane, uint64_t k) {
+    if (n < static_cast<uint32_t>(kStashCap)) {
+      sh.stash[n * kWave + static_cast<uint32_t>(lane)] = k;
+      ++n;
     }
-    n += n < static_cast<uint32_t>(kStashCap) ? 1u : 0u;
   }
 };
 
@@ -311,7 +320,7 @@ __device__ __forceinline__ void node_slots(DescRef D, uint32_t f, int32_t& s0, i
 }
 
 // Writes the exclusive prefix of the slot counts of the nr list entries in
-// sh.rnode to sh.rpre; returns nr, the total slot count in *total.
+// sh.rl.rnode to sh.rl.rpre; returns nr, the total slot count in *total.
 __device__ uint32_t list_prefix(DescRef D, GatherShared& sh, int lane, uint32_t nr,
                                 uint32_t* total) {
   uint32_t run = 0;
@@ -320,14 +329,13 @@ __device__ uint32_t list_prefix(DescRef D, GatherShared& sh, int lane, uint32_t 
     uint32_t cnt = 0;
     if (i < nr) {
       int32_t s0, s1;
-      node_slots(D, sh.rnode[i], s0, s1);
+      node_slots(D, sh.rl.rnode[i], s0, s1);
       cnt = static_cast<uint32_t>(s1 - s0);
     }
     const uint32_t incl = wave_scan_add(cnt);
-    if (i < nr) sh.rpre[i] = run + incl - cnt;
+    if (i < nr) sh.rl.rpre[i] = run + incl - cnt;
     run += static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
   }
-  if (lane == 0) sh.rpre[nr] = run;
   __syncthreads();
   *total = static_cast<uint32_t>(uni(static_cast<int>(run)));
   return static_cast<uint32_t>(uni(static_cast<int>(nr)));
@@ -341,8 +349,8 @@ __device__ uint32_t list_prefix(DescRef D, GatherShared& sh, int lane, uint32_t 
 // keeps (frontier << e) within one wave, so one vector load per lane tests up
 // to 64 descendants at once.  A leaf, or (unless leaf_only) a node entirely
 // inside the sphere, is emitted.  r2 = +inf lists the root.  Writes the
-// subtree roots to sh.rnode and the exclusive prefix of their slot counts to
-// sh.rpre; returns the number of subtrees, the total slot count in *total.
+// subtree roots to sh.rl.rnode and the exclusive prefix of their slot counts to
+// sh.rl.rpre; returns the number of subtrees, the total slot count in *total.
 // When the list would overflow, the walk restarts with whole inner nodes
 // allowed, then with a depth cap above the level that overflowed (at most
 // 2^8 subtrees, so it fits); a frontier that would overflow emits its nodes
@@ -407,7 +415,7 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
           full_depth = depth + e;
           break;
         }
-        if (emit) sh.rnode[nr + lanes_below(em, lane)] = f;
+        if (emit) sh.rl.rnode[nr + lanes_below(em, lane)] = f;
         if (push) sh.u.front[cur ^ 1][nnext + lanes_below(pmk, lane)] = f;
         nr += ne;
         nnext += static_cast<uint32_t>(__popcll(pmk));
@@ -429,7 +437,7 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
   }
   if (whole) {
     nr = 1;
-    if (lane == 0) sh.rnode[0] = 1u;
+    if (lane == 0) sh.rl.rnode[0] = 1u;
     __syncthreads();
   }
   if (fine) *fine = leaf_only && !coarse && !whole && max_depth == depth_leaf;
@@ -447,14 +455,14 @@ __device__ uint32_t filter_list(DescRef D, GatherShared& sh, int lane, uint32_t 
     uint32_t f = 0;
     bool keep = false;
     if (i < nr) {
-      f = sh.rnode[i];
+      f = sh.rl.rnode[i];
       float n2, f2;
       box_dist2(D.boxes[2 * f], D.boxes[2 * f + 1], cx, cy, cz, n2, f2);
       keep = n2 <= hit_lim;
     }
     const uint64_t km = __ballot(keep);
     // every lane's entry was read above; writes land below base + kWave
-    if (keep) sh.rnode[out + lanes_below(km, lane)] = f;
+    if (keep) sh.rl.rnode[out + lanes_below(km, lane)] = f;
     out += static_cast<uint32_t>(__popcll(km));
   }
   __syncthreads();
@@ -467,10 +475,10 @@ __device__ uint32_t filter_list(DescRef D, GatherShared& sh, int lane, uint32_t 
 __device__ __forceinline__ int32_t list_slot(DescRef D, const GatherShared& sh, uint32_t nr,
                                              uint32_t t, uint32_t& r) {
   if (nr == kSlotList) return static_cast<int32_t>(sh.cslot[t]);
-  while (r + 1 < nr && sh.rpre[r + 1] <= t) ++r;
+  while (r + 1 < nr && sh.rl.rpre[r + 1] <= t) ++r;
   int32_t s0, s1;
-  node_slots(D, sh.rnode[r], s0, s1);
-  return s0 + static_cast<int32_t>(t - sh.rpre[r]);
+  node_slots(D, sh.rl.rnode[r], s0, s1);
+  return s0 + static_cast<int32_t>(t - sh.rl.rpre[r]);
 }
 
 // Iterates the candidate photons of the current list: body(cnt) is called
@@ -488,9 +496,8 @@ __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int 
   uint32_t r = 0;
   float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p;
   float2 f = make_float2(0.f, 0.f);
-  int32_t slot = 0;
   if (t < total) {
-    slot = list_slot(D, sh, nr, t, r);
+    const int32_t slot = list_slot(D, sh, nr, t, r);
     p = P[slot];
     if (WF) {
       w = W[slot];
@@ -502,12 +509,11 @@ __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int 
     if (WF) {
       sh.u.c.wif[lane] = w;
       sh.u.c.flux[lane] = f;
-      sh.u.c.slot[lane] = static_cast<uint32_t>(slot);
     }
     __syncthreads();
     t += kChunk;
     if (t < total) {
-      slot = list_slot(D, sh, nr, t, r);
+      const int32_t slot = list_slot(D, sh, nr, t, r);
       p = P[slot];
       if (WF) {
         w = W[slot];
@@ -565,7 +571,7 @@ __device__ __forceinline__ void sweep_list(DescRef D, const GatherShared& sh, in
   for (uint32_t base = 0; base < nr; base += kWave) {
     const uint32_t i = base + static_cast<uint32_t>(lane);
     int32_t s0 = 0, s1 = 0;
-    if (i < nr) node_slots(D, sh.rnode[i], s0, s1);
+    if (i < nr) node_slots(D, sh.rl.rnode[i], s0, s1);
     for (int32_t j0 = 0; j0 < leaf_size; j0 += 8) {
       float4 p[8];
       bool ok[8];
@@ -583,9 +589,10 @@ __device__ __forceinline__ void sweep_list(DescRef D, const GatherShared& sh, in
 }
 
 // Explicit slot list of the photons of the current subtree list whose FP32
-// d^2 from c is <= lim2, written to sh.cslot.  Returns the count, or
-// kSlotList when more than kSlotCap photons qualify (sh.cslot then holds a
-// prefix only and must not be used).
+// d^2 from c is <= lim2.  It is gathered in sh.cscratch and, when it fits,
+// copied to sh.cslot, which replaces the subtree list; returns its length.
+// When more than kSlotCap photons qualify, returns kSlotList and the subtree
+// list is left as it was.
 __device__ uint32_t collect_slots(DescRef D, GatherShared& sh, int lane, uint32_t nr,
                                   uint32_t total, bool leaves, float cx, float cy, float cz,
                                   float lim2) {
@@ -595,12 +602,16 @@ __device__ uint32_t collect_slots(DescRef D, GatherShared& sh, int lane, uint32_
     const bool keep = ok && d2_ref(cx, cy, cz, p.x, p.y, p.z) <= lim2;
     const uint64_t km = __ballot(keep);
     const uint32_t at = n + lanes_below(km, lane);
-    if (keep && at < static_cast<uint32_t>(kSlotCap)) sh.cslot[at] = static_cast<uint32_t>(slot);
+    if (keep && at < static_cast<uint32_t>(kSlotCap)) sh.cscratch[at] = static_cast<uint32_t>(slot);
     n += static_cast<uint32_t>(__popcll(km));
     over = over || n > static_cast<uint32_t>(kSlotCap);
   });
   __syncthreads();
-  return over ? kSlotList : static_cast<uint32_t>(uni(static_cast<int>(n)));
+  if (over) return kSlotList;
+  const uint32_t cnt = static_cast<uint32_t>(uni(static_cast<int>(n)));
+  for (uint32_t i = lane; i < cnt; i += kWave) sh.cslot[i] = sh.cscratch[i];
+  __syncthreads();
+  return cnt;
 }
 
 // A staged candidate as the consume loops see it.
@@ -745,7 +756,7 @@ struct PhaseTimer {
 // on its k-th FP32 d^2.  Afterwards the neighbour set of a part lane is
 // {key < klo} plus the `need` smallest keys of [klo, khi), and [klo, khi)
 // holds at most kStashCap keys.  Leaves the run's candidate list (covering
-// every key below the lane bounds) in sh.rnode/sh.rpre: returns its entry
+// every key below the lane bounds) in sh.rl.rnode/sh.rl.rpre: returns its entry
 // count and its photon count in *total.
 __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool part, float r0sq,
                                float cx, float cy, float cz, float dq, float dmax,
@@ -907,7 +918,7 @@ using Kind = std::integral_constant<int, K>;
 
 template <int MODE>
 __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict__ perm,
-                              GatherShared& sh, int lane, uint32_t* counters,
+                              GatherShared& sh, float* drho, int lane, uint32_t* counters,
                               int count_stats, float split, PhaseTimer& pt) {
   const int64_t qloc = lg * kWave + lane;
   const bool active = qloc < D.m;
@@ -1054,7 +1065,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
       const float dmax = wave_max(dq);
 
       // ---- selection state and candidate list of the run --------------------
-      Stash<kAccum> stash;  // a lane stashes only in its own run
+      Stash stash;  // a lane stashes only in its own run
       stash.reset();
       SelState st;
       uint32_t total = 0;
@@ -1122,7 +1133,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
                 if (n_written < static_cast<uint32_t>(D.k))
                   row[n_written++] = static_cast<int32_t>(idx);
               } else {
-                stash.put(key, kAccum ? sh.u.c.slot[j] : 0u);
+                stash.put(sh, lane, key);
               }
             });
           });
@@ -1147,24 +1158,32 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         uint32_t members = 0;
         const uint32_t n_max = static_cast<uint32_t>(uni(static_cast<int>(
             wave_max(part ? static_cast<float>(stash.n) : 0.0f))));
+        uint64_t sk[kStashCap];  // the lane's stash; unused entries hold kKeyMax
+#pragma unroll
+        for (int e = 0; e < kStashCap; ++e) {
+          sk[e] = static_cast<uint32_t>(e) < stash.n ? sh.stash[e * kWave + lane] : kKeyMax;
+        }
 #pragma unroll 1
         for (uint32_t i = 0; i < n_max; ++i) {
-          uint64_t ki = stash.key[0];
+          uint64_t ki = sk[0];
 #pragma unroll
-          for (int e = 1; e < kStashCap; ++e) ki = (i == static_cast<uint32_t>(e)) ? stash.key[e] : ki;
+          for (int e = 1; e < kStashCap; ++e) ki = (i == static_cast<uint32_t>(e)) ? sk[e] : ki;
           uint32_t rank = 0;
 #pragma unroll
-          for (int j = 0; j < kStashCap; ++j) rank += stash.key[j] < ki ? 1u : 0u;
+          for (int j = 0; j < kStashCap; ++j) rank += sk[j] < ki ? 1u : 0u;
           const bool valid = part && i < stash.n;
           if (valid && rank + 1 == st.need) tau = ki;
           if (valid && rank < st.need) members |= 1u << i;
         }
         if (kAccum) {
-          // members' photons, two per round so their loads overlap
-          auto slot_of = [&](uint32_t i) {
+          // members' photons, two per round so their loads overlap; the
+          // slot of a photon is found from its original index
+          auto index_of = [&](uint32_t i) {
             uint32_t v = 0;
 #pragma unroll
-            for (int e = 0; e < kStashCap; ++e) v = (i == static_cast<uint32_t>(e)) ? stash.slot[e] : v;
+            for (int e = 0; e < kStashCap; ++e) {
+              v = (i == static_cast<uint32_t>(e)) ? static_cast<uint32_t>(sk[e]) : v;
+            }
             return v;
           };
           uint32_t mm = members;
@@ -1174,8 +1193,10 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
             const uint32_t i1 = mm ? static_cast<uint32_t>(__builtin_ctz(mm)) : 32u;
             mm &= mm - 1u;
             const bool v0 = i0 < 32u, v1 = i1 < 32u;
-            const int32_t s0 = static_cast<int32_t>(slot_of(i0));
-            const int32_t s1 = static_cast<int32_t>(slot_of(i1));
+            const uint32_t x0 = index_of(i0), x1 = index_of(i1);
+            int32_t s0 = 0, s1 = 0;
+            if (v0) s0 = D.inv_perm[x0];
+            if (v1) s1 = D.inv_perm[x1];
             float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f), w1 = w0;
             float2 f0 = make_float2(0.f, 0.f), f1 = f0;
             if (v0) { w0 = D.wif4[s0]; f0 = D.flux2[s0]; }
@@ -1195,7 +1216,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
 #pragma unroll
           for (int i = 0; i < kStashCap; ++i) {
             if (((members >> i) & 1u) && n_written < static_cast<uint32_t>(D.k))
-              row[n_written++] = static_cast<int32_t>(stash.key[i] & 0xffffffffull);
+              row[n_written++] = static_cast<int32_t>(sk[i] & 0xffffffffull);
           }
         }
       }
@@ -1251,9 +1272,9 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
           const float inv = 1.0f / (kPiF * denom);
           const float dr = ge_r * s_r * inv, dg = ge_g * s_g * inv, db = ge_b * s_b * inv;
           if (mat < kDrhoSlots) {
-            atomicAdd(&sh.drho[3 * mat + 0], dr);
-            atomicAdd(&sh.drho[3 * mat + 1], dg);
-            atomicAdd(&sh.drho[3 * mat + 2], db);
+            atomicAdd(&drho[3 * mat + 0], dr);
+            atomicAdd(&drho[3 * mat + 1], dg);
+            atomicAdd(&drho[3 * mat + 2], db);
           } else {
             atomicAdd(&D.dl_dalbedo[3 * mat + 0], dr);
             atomicAdd(&D.dl_dalbedo[3 * mat + 1], dg);
@@ -1265,9 +1286,9 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
             const float lgv = ((rho_g / kPiF) * s_g) / denom;
             const float lb = ((rho_b / kPiF) * s_b) / denom;
             if (fgm < kDrhoSlots) {
-              atomicAdd(&sh.drho[3 * fgm + 0], g_r * lr);
-              atomicAdd(&sh.drho[3 * fgm + 1], g_g * lgv);
-              atomicAdd(&sh.drho[3 * fgm + 2], g_b * lb);
+              atomicAdd(&drho[3 * fgm + 0], g_r * lr);
+              atomicAdd(&drho[3 * fgm + 1], g_g * lgv);
+              atomicAdd(&drho[3 * fgm + 2], g_b * lb);
             } else {
               atomicAdd(&D.dl_dalbedo[3 * fgm + 0], g_r * lr);
               atomicAdd(&D.dl_dalbedo[3 * fgm + 1], g_g * lgv);
@@ -1382,10 +1403,12 @@ __device__ __forceinline__ int find_instance(const PM_CONST int64_t* prefix, int
 
 template <int MODE>
 __global__ __launch_bounds__(kWave, PM_GATHER_MIN_WAVES) void gather_kernel(GatherLaunch L) {
-  __shared__ GatherShared sh;
+  __shared__ std::conditional_t<MODE == kModeAdjoint, AdjointShared, GatherShared> sh;
   const int lane = threadIdx.x;
-  if (MODE == kModeAdjoint) {
-    for (int i = lane; i < 3 * kDrhoSlots; i += kWave) sh.drho[i] = 0.0f;
+  float* drho = nullptr;
+  if constexpr (MODE == kModeAdjoint) {
+    drho = sh.drho;
+    for (int i = lane; i < 3 * kDrhoSlots; i += kWave) drho[i] = 0.0f;
     __syncthreads();
   }
   const int nchunks = L.xcd_aware ? kXcds : 1;
@@ -1407,7 +1430,7 @@ __global__ __launch_bounds__(kWave, PM_GATHER_MIN_WAVES) void gather_kernel(Gath
       if (g >= cend) break;
       const int inst = (L.n_inst > 1) ? find_instance(gprefix, L.n_inst, g) : 0;
       DescRef D = descs[inst];
-      process_group<MODE>(D, g - gprefix[inst], L.perm, sh, lane, L.counters,
+      process_group<MODE>(D, g - gprefix[inst], L.perm, sh, drho, lane, L.counters,
                           L.count_stats, L.run_split, pt);
     }
   }
@@ -1415,7 +1438,7 @@ __global__ __launch_bounds__(kWave, PM_GATHER_MIN_WAVES) void gather_kernel(Gath
     __syncthreads();
     DescRef D = descs[0];
     for (int i = lane; i < 3 * min(D.n_mats, kDrhoSlots); i += kWave) {
-      const float v = sh.drho[i];
+      const float v = drho[i];
       if (v != 0.0f) atomicAdd(&D.dl_dalbedo[i], v);
     }
   }

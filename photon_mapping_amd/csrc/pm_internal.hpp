@@ -14,7 +14,10 @@ namespace pm {
 // ---- tuning constants ------------------------------------------------------
 constexpr int kWave = 64;           // CDNA wavefront width
 constexpr int kBins = 32;           // selection histogram bins per query
-constexpr int kStashCap = 12;       // per-query candidates kept past the cut
+#ifndef PM_STASH_CAP
+#define PM_STASH_CAP 12
+#endif
+constexpr int kStashCap = PM_STASH_CAP;  // per-query candidates kept past the cut
 constexpr int kMortonBits = 10;     // per axis
 constexpr int kAnchorWindow = 4;    // anchors tried on each side of a group
 constexpr int kMaxKnnK = 1024;      // row-sort limit of pm_knn
@@ -170,7 +173,7 @@ struct PhotonMapImpl {
 struct Context {
   int device = 0;
   int n_cu = 256;
-  int waves_per_cu = 8;
+  int waves_per_cu = 12;
   int anchor_stride = 8;
   int leaf_size = 8;
   int xcd_aware = 1;

@@ -92,13 +92,21 @@ struct RangeList {
   uint32_t rpre[kMaxRanges];
 };
 
+// Pass B's use of the lane-bin storage: the per-lane stash (key and slot,
+// entry e of lane l at [e * kWave + l]) and the slots of the staged chunk.
+struct StashArea {
+  uint64_t key[kStashCap * kWave];
+  uint32_t slot[kStashCap * kWave];
+  uint32_t chunk_slot[kChunk];
+};
+
 // One wave's LDS (one-wave workgroups).  Buffers whose lifetimes do not
 // overlap share storage, which keeps the block at 13 KB so that 12 waves
 // fit a CU's 160 KB.
 struct GatherShared {
   union {
     uint32_t hist[(kBins + 2) * kWave];  // per-lane bins; centre bins alias the start
-    uint64_t stash[kStashCap * kWave];   // pass B / finish: per-lane keys, entry-major
+    StashArea sb;                        // pass B / finish
     uint32_t cscratch[kSlotCap];         // collect_slots output before it is accepted
   };
   // The chunk staging buffers are dead while a candidate list is built, so
@@ -121,7 +129,7 @@ struct GatherShared {
 struct AdjointShared : GatherShared {
   float drho[3 * kDrhoSlots];
 };
-static_assert(kStashCap * sizeof(uint64_t) <= (kBins + 2) * sizeof(uint32_t),
+static_assert(sizeof(StashArea) <= (kBins + 2) * kWave * sizeof(uint32_t),
               "the stash fits the lane bins");
 static_assert(kSlotCap * sizeof(uint32_t) <= sizeof(RangeList), "slot list fits the range list");
 static_assert(kCentreBins <= (kBins + 2) * kWave, "centre bins alias the lane bins");
@@ -273,16 +281,18 @@ struct SelState {
   uint32_t cnt;   // number of candidate keys inside [klo, khi)
 };
 
-// Per-lane stash of the keys inside [klo, khi), in LDS: entry e of lane l
-// is sh.stash[e * kWave + l] (the lane bins are dead while it is in use).
-// Keys past kStashCap are dropped; the selection keeps the count of the
-// stashed bin at or below it.
+// Per-lane stash of the keys inside [klo, khi) and their photons' slots, in
+// LDS (sh.sb; the lane bins are dead while it is in use).  Keys past
+// kStashCap are dropped; the selection keeps the count of the stashed bin at
+// or below it.
 struct Stash {
   uint32_t n;
   __device__ __forceinline__ void reset() { n = 0; }
-  __device__ __forceinline__ void put(GatherShared& sh, int lane, uint64_t k) {
+  __device__ __forceinline__ void put(GatherShared& sh, int lane, uint64_t k, uint32_t slot) {
     if (n < static_cast<uint32_t>(kStashCap)) {
-      sh.stash[n * kWave + static_cast<uint32_t>(lane)] = k;
+      const uint32_t at = n * kWave + static_cast<uint32_t>(lane);
+      sh.sb.key[at] = k;
+      sh.sb.slot[at] = slot;
       ++n;
     }
   }
@@ -496,8 +506,9 @@ __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int 
   uint32_t r = 0;
   float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p;
   float2 f = make_float2(0.f, 0.f);
+  int32_t slot = 0;
   if (t < total) {
-    const int32_t slot = list_slot(D, sh, nr, t, r);
+    slot = list_slot(D, sh, nr, t, r);
     p = P[slot];
     if (WF) {
       w = W[slot];
@@ -509,11 +520,12 @@ __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int 
     if (WF) {
       sh.u.c.wif[lane] = w;
       sh.u.c.flux[lane] = f;
+      sh.sb.chunk_slot[lane] = static_cast<uint32_t>(slot);
     }
     __syncthreads();
     t += kChunk;
     if (t < total) {
-      const int32_t slot = list_slot(D, sh, nr, t, r);
+      slot = list_slot(D, sh, nr, t, r);
       p = P[slot];
       if (WF) {
         w = W[slot];
@@ -1133,7 +1145,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
                 if (n_written < static_cast<uint32_t>(D.k))
                   row[n_written++] = static_cast<int32_t>(idx);
               } else {
-                stash.put(sh, lane, key);
+                stash.put(sh, lane, key, kAccum ? sh.sb.chunk_slot[j] : 0u);
               }
             });
           });
@@ -1161,7 +1173,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         uint64_t sk[kStashCap];  // the lane's stash; unused entries hold kKeyMax
 #pragma unroll
         for (int e = 0; e < kStashCap; ++e) {
-          sk[e] = static_cast<uint32_t>(e) < stash.n ? sh.stash[e * kWave + lane] : kKeyMax;
+          sk[e] = static_cast<uint32_t>(e) < stash.n ? sh.sb.key[e * kWave + lane] : kKeyMax;
         }
 #pragma unroll 1
         for (uint32_t i = 0; i < n_max; ++i) {
@@ -1176,16 +1188,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
           if (valid && rank < st.need) members |= 1u << i;
         }
         if (kAccum) {
-          // members' photons, two per round so their loads overlap; the
-          // slot of a photon is found from its original index
-          auto index_of = [&](uint32_t i) {
-            uint32_t v = 0;
-#pragma unroll
-            for (int e = 0; e < kStashCap; ++e) {
-              v = (i == static_cast<uint32_t>(e)) ? static_cast<uint32_t>(sk[e]) : v;
-            }
-            return v;
-          };
+          // members' photons, two per round so their loads overlap
           uint32_t mm = members;
           while (wave_any(mm != 0u)) {
             const uint32_t i0 = mm ? static_cast<uint32_t>(__builtin_ctz(mm)) : 32u;
@@ -1193,10 +1196,9 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
             const uint32_t i1 = mm ? static_cast<uint32_t>(__builtin_ctz(mm)) : 32u;
             mm &= mm - 1u;
             const bool v0 = i0 < 32u, v1 = i1 < 32u;
-            const uint32_t x0 = index_of(i0), x1 = index_of(i1);
             int32_t s0 = 0, s1 = 0;
-            if (v0) s0 = D.inv_perm[x0];
-            if (v1) s1 = D.inv_perm[x1];
+            if (v0) s0 = static_cast<int32_t>(sh.sb.slot[i0 * kWave + lane]);
+            if (v1) s1 = static_cast<int32_t>(sh.sb.slot[i1 * kWave + lane]);
             float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f), w1 = w0;
             float2 f0 = make_float2(0.f, 0.f), f1 = f0;
             if (v0) { w0 = D.wif4[s0]; f0 = D.flux2[s0]; }

@@ -15,7 +15,7 @@ namespace pm {
 constexpr int kWave = 64;           // CDNA wavefront width
 constexpr int kBins = 32;           // selection histogram bins per query
 #ifndef PM_STASH_CAP
-#define PM_STASH_CAP 12
+#define PM_STASH_CAP 11
 #endif
 constexpr int kStashCap = PM_STASH_CAP;  // per-query candidates kept past the cut
 constexpr int kMortonBits = 10;     // per axis

@@ -551,6 +551,7 @@ def main() -> int:
         "runs_per_group": st["runs"] / groups,
         "centre_candidates_per_group": st["candidates1"] / groups,
         "lane_candidates_per_run": st["candidates2"] / max(int(st["runs"]), 1),
+        "pass_b_candidates_per_run": st["candidates3"] / max(int(st["runs"]), 1),
         "cut_lists": int(st["cut_lists"]),
         # share of wave time per phase of a group (statistics run)
         "phase_share": dict(zip(

@@ -72,7 +72,7 @@ typedef struct pm_material {
  * times are HIP-event times on the call's stream (filled only when the
  * PM_OPT_TIMING option is on, which synchronises the stream). */
 typedef struct pm_stats {
-  double sort_ms;      /* query Morton keys + radix sort + group anchors  */
+  double sort_ms;      /* query Hilbert keys + anchors + radix sort       */
   double gather_ms;    /* the persistent gather kernel                    */
   double post_ms;      /* knn row sort / adjoint flush                    */
   double build_ms;     /* last photon-map build (incl. anchors)           */
@@ -87,6 +87,8 @@ typedef struct pm_stats {
    * radius bound, run-centre bound, candidate lists, histogram pass,
    * thresholds/refinement, selection pass, stash/scatter/outputs, queue */
   int64_t phase_cycles[8];
+  int64_t candidates3; /* photons of the selection-pass lists, after they
+                        * shrink to the lane windows' sphere              */
 } pm_stats;
 
 /* Options for pm_context_set_option. */
@@ -126,7 +128,7 @@ int pm_photonmap_create_aos(pm_context* ctx, int64_t n, const float* photons,
                             int mem, pm_photonmap** out_map);
 int pm_photonmap_destroy(pm_photonmap* map);
 /* Replaces PhotonMap::build / KdTree::buildTree (include/photon_map.h:157-164,
- * 202-206): GPU Morton sort, balanced tree over leaf buckets of leaf_size
+ * 202-206): GPU Hilbert-curve sort, balanced tree over leaf buckets of leaf_size
  * photons (0 = context default; 4..256). */
 int pm_photonmap_build(pm_photonmap* map, int leaf_size, void* stream);
 /* Replaces the flux of every photon (3*n SoA, original photon order)

@@ -2,14 +2,15 @@
 //
 // Build (replaces KdTree::buildTree, include/photon_map.h:61-105,157-164):
 //   1. bounding box of the photon positions (and a finiteness check);
-//   2. 30-bit Morton code per photon, 10 bits per axis;
+//   2. 30-bit Hilbert code per photon over a 10-bit grid per axis;
 //   3. stable radix sort of (code, original index);
 //   4. reorder into slot order: pos4 = (x, y, z, index bits),
 //      wif4 = (wi, flux.r), flux2 = (flux.g, flux.b), plus index -> slot;
 //   5. a balanced binary tree over leaf buckets of leaf_size consecutive
 //      slots: leaf boxes, then each level's boxes as unions of its children
-//      (median split by count over the Morton order);
-//   6. radius anchors every anchor_stride slots (centroid + Morton code).
+//      (median split by count over the Hilbert order);
+//   6. radius anchors every anchor_stride slots (centroid + Hilbert code),
+//      and a table from 16-bit code prefixes to anchors.
 // The anchors' k-NN radius bounds are computed lazily per k by running the
 // gather kernel itself in anchor mode.
 
@@ -44,23 +45,16 @@ __device__ __forceinline__ uint32_t quantize(float v, float lo, float scale) {
   return static_cast<uint32_t>(t);
 }
 
-__device__ __forceinline__ uint32_t morton3(float x, float y, float z, const float* lo,
-                                            const float* scale) {
-  return (expand10(quantize(x, lo[0], scale[0])) << 2) |
-         (expand10(quantize(y, lo[1], scale[1])) << 1) |
-         expand10(quantize(z, lo[2], scale[2]));
-}
-
 // Hilbert index of the 10-bit quantized position (Skilling 2004, "Programming
 // the Hilbert curve", AIP Conf. Proc. 707: axes to transposed index, then the
-// transpose's bits interleaved like a Morton code).  Consecutive indices are
-// always face-adjacent cells, so runs of sorted queries stay compact where a
-// Morton order jumps.
+// transpose's bits interleaved).  Consecutive indices are always
+// face-adjacent cells, so runs of sorted photons or queries stay compact
+// where a Morton (Z) order jumps.
 __device__ __forceinline__ uint32_t hilbert3(float x, float y, float z, const float* lo,
                                              const float* scale) {
   uint32_t X[3] = {quantize(x, lo[0], scale[0]), quantize(y, lo[1], scale[1]),
                    quantize(z, lo[2], scale[2])};
-  constexpr uint32_t M = 1u << (kMortonBits - 1);
+  constexpr uint32_t M = 1u << (kGridBits - 1);
   for (uint32_t Q = M; Q > 1; Q >>= 1) {
     const uint32_t P = Q - 1;
 #pragma unroll
@@ -147,7 +141,7 @@ __global__ __launch_bounds__(kThreads) void photon_codes_kernel(
     uint32_t* __restrict__ vals) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= n) return;
-  codes[i] = morton3(pos[i], pos[n + i], pos[2 * n + i], qz.lo, qz.scale);
+  codes[i] = hilbert3(pos[i], pos[n + i], pos[2 * n + i], qz.lo, qz.scale);
   vals[i] = static_cast<uint32_t>(i);
 }
 
@@ -225,6 +219,23 @@ __global__ __launch_bounds__(kThreads) void anchor_init_kernel(
   anchor_code[a] = codes[s0];
 }
 
+// Entry p of the anchor table is the first anchor whose Hilbert code is at or
+// above p << (30 - kAnchorTabBits) (n_anchors when there is none), for
+// p = 0 .. 2^kAnchorTabBits; the codes are ascending.
+__global__ __launch_bounds__(kThreads) void anchor_tab_kernel(
+    const uint32_t* __restrict__ anchor_code, int n_anchors, uint32_t* __restrict__ tab) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p > (1 << kAnchorTabBits)) return;
+  const uint64_t start = static_cast<uint64_t>(p) << (3 * kGridBits - kAnchorTabBits);
+  int lo = 0, hi = n_anchors;
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (anchor_code[m] < start) lo = m + 1;
+    else hi = m;
+  }
+  tab[p] = static_cast<uint32_t>(lo);
+}
+
 // ---- query preparation ---------------------------------------------------------
 
 __device__ __forceinline__ int find_inst(const int64_t* __restrict__ prefix, int n_inst,
@@ -239,8 +250,7 @@ __device__ __forceinline__ int find_inst(const int64_t* __restrict__ prefix, int
 }
 
 // Sort key of every query, and its nearest radius anchor: the last anchor
-// whose first Morton code is <= the query's Morton code (upper bound, one
-// back).  Both are computed in original query order, so the position reads
+// whose first code is <= the query's code (upper bound, one back).  Both are computed in original query order, so the position reads
 // are coalesced; q_anchor is indexed by global query index.
 template <typename K>
 __global__ __launch_bounds__(kThreads) void query_keys_kernel(
@@ -259,15 +269,21 @@ __global__ __launch_bounds__(kThreads) void query_keys_kernel(
   } else {
     x = D.q_soa[li]; y = D.q_soa[D.m + li]; z = D.q_soa[2 * D.m + li];
   }
-  // queries run in Hilbert order (the photons and anchors stay in Morton order)
+  // queries run in the photons' Hilbert order
   const uint32_t code = hilbert3(x, y, z, D.qlo, D.qscale);
   keys[i] = static_cast<K>((static_cast<uint64_t>(inst) << 32) | code);
   vals[i] = static_cast<uint32_t>(i);
-  const uint32_t mc = morton3(x, y, z, D.qlo, D.qscale);
-  int lo = 0, hi = D.n_anchors;
+  // The upper bound of the code lies inside its prefix bucket of the table:
+  // anchors before the bucket have smaller codes, those after it larger.
+  int lo = 0, hi = 0;
+  if (D.n_anchors > 0) {
+    const uint32_t pre = code >> (3 * kGridBits - kAnchorTabBits);
+    lo = static_cast<int>(D.anchor_tab[pre]);
+    hi = static_cast<int>(D.anchor_tab[pre + 1]);
+  }
   while (lo < hi) {
     const int m = (lo + hi) >> 1;
-    if (D.anchor_code[m] <= mc) lo = m + 1;
+    if (D.anchor_code[m] <= code) lo = m + 1;
     else hi = m;
   }
   q_anchor[i] = static_cast<uint32_t>(lo > 0 ? lo - 1 : 0);
@@ -366,6 +382,7 @@ void fill_desc_map_part(const PhotonMapImpl* map, GatherDesc* d) {
   d->boxes = map->boxes.as<float4>();
   d->anchors = map->anchors.as<float4>();
   d->anchor_code = map->anchor_code.as<uint32_t>();
+  d->anchor_tab = map->anchor_tab.as<uint32_t>();
   d->inv_perm = map->inv_perm.as<int32_t>();
   d->pos_orig = map->pos_soa.as<float>();
   for (int c = 0; c < 3; ++c) {
@@ -417,7 +434,7 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
     map->qscale[c] = ext > 0.0f ? 1024.0f / ext : 1.0f;
   }
 
-  // 2-3. Morton codes, stable sort
+  // 2-3. Hilbert codes, stable sort
   if (int rc = ctx->ws.keys.reserve(n * sizeof(uint32_t))) return rc;
   if (int rc = ctx->ws.vals.reserve(n * sizeof(uint32_t))) return rc;
   if (int rc = ctx->ws.keys2.reserve(n * sizeof(uint32_t))) return rc;
@@ -432,7 +449,7 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
                      pos, n, qz, k0, v0);
   PM_HIP(hipGetLastError());
   int which = 0;
-  if (int rc = radix_sort_u32(&ctx->ws, k0, v0, k1, v1, n, 3 * kMortonBits, s, &which)) return rc;
+  if (int rc = radix_sort_u32(&ctx->ws, k0, v0, k1, v1, n, 3 * kGridBits, s, &which)) return rc;
   const uint32_t* sorted_codes = which ? k1 : k0;
   const uint32_t* order = which ? v1 : v0;
 
@@ -475,6 +492,12 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
                      dim3(kThreads), 0, s, map->pos4.as<float4>(),
                      map->codes.as<uint32_t>(), n, map->anchor_stride, map->n_anchors,
                      map->anchors.as<float4>(), map->anchor_code.as<uint32_t>());
+  PM_HIP(hipGetLastError());
+  constexpr int kTab = (1 << kAnchorTabBits) + 1;
+  if (int rc = map->anchor_tab.reserve(static_cast<size_t>(kTab) * sizeof(uint32_t))) return rc;
+  hipLaunchKernelGGL(anchor_tab_kernel, dim3(ceil_div(kTab, kThreads)), dim3(kThreads), 0, s,
+                     map->anchor_code.as<uint32_t>(), map->n_anchors,
+                     map->anchor_tab.as<uint32_t>());
   PM_HIP(hipGetLastError());
   map->built = true;
   return PM_OK;
@@ -572,7 +595,7 @@ int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
     hipLaunchKernelGGL(query_keys_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
                        dev_q_prefix, n_inst, total_q, k0, v0, ws->q_anchor.as<uint32_t>());
     PM_HIP(hipGetLastError());
-    if (int rc = radix_sort_u32(ws, k0, v0, k1, v1, total_q, 3 * kMortonBits, s, &which))
+    if (int rc = radix_sort_u32(ws, k0, v0, k1, v1, total_q, 3 * kGridBits, s, &which))
       return rc;
   } else {
     uint64_t* k0 = ws->keys.as<uint64_t>();

@@ -128,7 +128,7 @@ struct CallTimer {
 
 // Shared driver of every gather: the instances' descriptors (map part and
 // query/output part filled, sizes in d.m) are laid out in query and group
-// order, staged, the queries of all instances Morton-sorted together, and
+// order, staged, the queries of all instances sorted together along the curve, and
 // one persistent launch covers every group.  knn_out_idx (single kNN
 // instance only) receives the sorted rows.
 int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream_t s,
@@ -193,7 +193,7 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
       ctx->stats.post_ms = elapsed_ms(ctx->ev[2], ctx->ev[3]);
     }
     if (ctx->count_stats) {
-      uint32_t c[6] = {0, 0, 0, 0, 0, 0};
+      uint32_t c[7] = {0, 0, 0, 0, 0, 0, 0};
       PM_HIP(hipMemcpy(c, ctx->ws.counters.as<uint32_t>() + kStatBase, sizeof(c),
                        hipMemcpyDeviceToHost));
       ctx->stats.fallbacks = c[kStatFallback - kStatBase];
@@ -202,6 +202,7 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
       ctx->stats.candidates2 = c[kStatCand2 - kStatBase];
       ctx->stats.cut_lists = c[kStatOverflow - kStatBase];
       ctx->stats.runs = c[kStatRuns - kStatBase];
+      ctx->stats.candidates3 = c[kStatCand3 - kStatBase];
       uint64_t ph[kPhases];
       PM_HIP(hipMemcpy(ph, ctx->ws.counters.as<uint32_t>() + kPhaseBase, sizeof(ph),
                        hipMemcpyDeviceToHost));

@@ -626,6 +626,41 @@ __device__ uint32_t collect_slots(DescRef D, GatherShared& sh, int lane, uint32_
   return cnt;
 }
 
+// Keeps the entries of the explicit slot list whose photon has FP32 d^2 from
+// c <= lim2, in order, and returns the new length.  Four batches of 64
+// entries are loaded before any is tested, so their loads overlap; a batch's
+// kept entries land below the next batch, which is read before it can be
+// overwritten.
+__device__ uint32_t filter_slots(DescRef D, GatherShared& sh, int lane, uint32_t total,
+                                 float cx, float cy, float cz, float lim2) {
+  const float4* __restrict__ P = D.pos4;
+  uint32_t out = 0;
+  for (uint32_t base = 0; base < total; base += 4 * kWave) {
+    uint32_t sl[4];
+    float4 p[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t t = base + static_cast<uint32_t>(u * kWave + lane);
+      sl[u] = 0;
+      p[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (t < total) {
+        sl[u] = sh.cslot[t];
+        p[u] = P[sl[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t t = base + static_cast<uint32_t>(u * kWave + lane);
+      const bool keep = t < total && d2_ref(cx, cy, cz, p[u].x, p[u].y, p[u].z) <= lim2;
+      const uint64_t km = __ballot(keep);
+      if (keep) sh.cslot[out + lanes_below(km, lane)] = sl[u];
+      out += static_cast<uint32_t>(__popcll(km));
+    }
+  }
+  __syncthreads();
+  return static_cast<uint32_t>(uni(static_cast<int>(out)));
+}
+
 // A staged candidate as the consume loops see it.
 struct Cand {
   float4 p;   // position, original index bits
@@ -1097,6 +1132,19 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
       } else {
         nr = run_select(D, qx, qy, qz, part, r0sq, cx, cy, cz, dq, dmax, k_eff, sh, lane,
                         counters, count_stats, st, pt, &total);
+        // With the lane windows known, every key below a lane's khi lies
+        // within sqrt(d^2 of khi) + |q - c| of c: the slot list shrinks to
+        // the sphere that holds them all before the later passes read it.
+        if (nr == kSlotList) {
+          const float kd2 = __uint_as_float(static_cast<uint32_t>(st.khi >> 32));
+          float xr = 0.0f;
+          if (part) xr = kd2 <= FLT_MAX ? sqrt_up(cull_bound(kd2)) * 1.00002f + dq : INFINITY;
+          const float rb = wave_max(xr) * 1.0002f;
+          if (rb < INFINITY) {
+            total = filter_slots(D, sh, lane, total, cx, cy, cz, rb * rb + 1e-30f);
+          }
+          if (count_stats && lane == 0) atomicAdd(&counters[kStatCand3], total);
+        }
       }
 
       // ---- pass B: consume neighbours below klo, stash [klo, khi) ---------------
@@ -1177,9 +1225,8 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         }
 #pragma unroll 1
         for (uint32_t i = 0; i < n_max; ++i) {
-          uint64_t ki = sk[0];
-#pragma unroll
-          for (int e = 1; e < kStashCap; ++e) ki = (i == static_cast<uint32_t>(e)) ? sk[e] : ki;
+          // (past the lane's own entries this is a stale key; it is not used)
+          const uint64_t ki = sh.sb.key[i * kWave + lane];
           uint32_t rank = 0;
 #pragma unroll
           for (int j = 0; j < kStashCap; ++j) rank += sk[j] < ki ? 1u : 0u;

@@ -18,8 +18,9 @@ constexpr int kBins = 32;           // selection histogram bins per query
 #define PM_STASH_CAP 11
 #endif
 constexpr int kStashCap = PM_STASH_CAP;  // per-query candidates kept past the cut
-constexpr int kMortonBits = 10;     // per axis
+constexpr int kGridBits = 10;       // space-filling-curve grid bits per axis
 constexpr int kAnchorWindow = 4;    // anchors tried on each side of a group
+constexpr int kAnchorTabBits = 16;  // code prefix bits of the anchor lookup table
 constexpr int kMaxKnnK = 1024;      // row-sort limit of pm_knn
 constexpr int kXcds = 8;            // MI355X accelerator complex dies
 
@@ -32,6 +33,7 @@ constexpr int kStatCand1 = kStatBase + 2;      // centre-bound candidate photons
 constexpr int kStatCand2 = kStatBase + 3;      // per-lane pass candidate photons
 constexpr int kStatOverflow = kStatBase + 4;   // candidate lists cut short
 constexpr int kStatRuns = kStatBase + 5;       // runs of nearby lanes
+constexpr int kStatCand3 = kStatBase + 6;      // selection-pass candidate photons
 // Wave cycles per phase of a group (uint64 each, from word kPhaseBase).
 constexpr int kPhaseBase = 32;
 constexpr int kPhases = 8;
@@ -70,20 +72,21 @@ struct Workspace {
 // ---- gather instance descriptor (device-resident, read with scalar loads) ----
 // One per photon-map x query-batch pairing.  All pointers are device memory.
 struct alignas(16) GatherDesc {
-  // photon map, sorted by Morton code ("slot" order)
+  // photon map, sorted by Hilbert code ("slot" order)
   const float4* pos4;      // slot -> (x, y, z, original index as bits)
   const float4* wif4;      // slot -> (wi.x, wi.y, wi.z, flux.r)
   const float2* flux2;     // slot -> (flux.g, flux.b)
   const float4* boxes;     // heap node -> [2n] = min, [2n+1] = max
   const float4* anchors;   // anchor -> (centroid xyz, k-radius bound)
-  const uint32_t* anchor_code;  // anchor -> Morton code of its first slot
+  const uint32_t* anchor_code;  // anchor -> Hilbert code of its first slot
+  const uint32_t* anchor_tab;   // code prefix -> first anchor whose code is >= it
   const int32_t* inv_perm; // original index -> slot
   const float* pos_orig;   // 3*n positions in original photon order (SoA)
   // queries
   const pm_query* q;       // RADIANCE / ADJOINT: local query records
   const float* q_soa;      // KNN: 3*m positions
   const pm_material* mats;
-  const uint32_t* q_anchor;  // local query index -> nearest anchor (Morton order)
+  const uint32_t* q_anchor;  // local query index -> nearest anchor (slot order)
   // outputs
   float* out_rgb;
   float* out_r2;
@@ -101,8 +104,8 @@ struct alignas(16) GatherDesc {
   int64_t q_base;          // first sorted position of this instance
   int64_t group_base;      // first global group of this instance
   int64_t n_groups;
-  float qlo[3];            // Morton quantisation origin
-  float qscale[3];         // Morton quantisation scale (cells per unit)
+  float qlo[3];            // curve quantisation origin
+  float qscale[3];         // curve quantisation scale (cells per unit)
   int32_t n;               // photons
   int32_t leaf_size;
   int32_t n_leaves;
@@ -166,7 +169,7 @@ struct PhotonMapImpl {
   // original-order inputs (SoA, 3n each)
   DevBuf pos_soa, wi_soa, flux_soa;
   // built structure
-  DevBuf pos4, wif4, flux2, boxes, anchors, anchor_code, inv_perm, codes;
+  DevBuf pos4, wif4, flux2, boxes, anchors, anchor_code, anchor_tab, inv_perm, codes;
   DevBuf desc;             // one GatherDesc for single-instance calls
 };
 

@@ -249,14 +249,12 @@ __device__ __forceinline__ int find_inst(const int64_t* __restrict__ prefix, int
   return lo;
 }
 
-// Sort key of every query, and its nearest radius anchor: the last anchor
-// whose first code is <= the query's code (upper bound, one back).  Both are computed in original query order, so the position reads
-// are coalesced; q_anchor is indexed by global query index.
+// Sort key of every query: its instance and its Hilbert code, in the
+// photons' order along the curve; the value is the global query index.
 template <typename K>
 __global__ __launch_bounds__(kThreads) void query_keys_kernel(
     const GatherDesc* __restrict__ descs, const int64_t* __restrict__ q_prefix,
-    int n_inst, int64_t total, K* __restrict__ keys, uint32_t* __restrict__ vals,
-    uint32_t* __restrict__ q_anchor) {
+    int n_inst, int64_t total, K* __restrict__ keys, uint32_t* __restrict__ vals) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= total) return;
   const int inst = n_inst > 1 ? find_inst(q_prefix, n_inst, i) : 0;
@@ -269,10 +267,26 @@ __global__ __launch_bounds__(kThreads) void query_keys_kernel(
   } else {
     x = D.q_soa[li]; y = D.q_soa[D.m + li]; z = D.q_soa[2 * D.m + li];
   }
-  // queries run in the photons' Hilbert order
   const uint32_t code = hilbert3(x, y, z, D.qlo, D.qscale);
   keys[i] = static_cast<K>((static_cast<uint64_t>(inst) << 32) | code);
   vals[i] = static_cast<uint32_t>(i);
+}
+
+// Nearest radius anchor of every sorted query: the last anchor whose first
+// code is <= the query's code (upper bound, one back).  It runs over the
+// sorted keys, so reads and writes are coalesced and neighbouring queries
+// search the same table bucket; q_anchor is indexed by sorted position.  An
+// instance's sorted positions are its global query indices' range, since
+// the instance leads the key.
+template <typename K>
+__global__ __launch_bounds__(kThreads) void query_anchor_kernel(
+    const GatherDesc* __restrict__ descs, const int64_t* __restrict__ q_prefix, int n_inst,
+    int64_t total, const K* __restrict__ sorted_keys, uint32_t* __restrict__ q_anchor) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= total) return;
+  const int inst = n_inst > 1 ? find_inst(q_prefix, n_inst, i) : 0;
+  const GatherDesc& D = descs[inst];
+  const uint32_t code = static_cast<uint32_t>(sorted_keys[i] & 0xffffffffu);
   // The upper bound of the code lies inside its prefix bucket of the table:
   // anchors before the bucket have smaller codes, those after it larger.
   int lo = 0, hi = 0;
@@ -593,19 +607,25 @@ int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
     uint32_t* k0 = ws->keys.as<uint32_t>();
     uint32_t* k1 = ws->keys2.as<uint32_t>();
     hipLaunchKernelGGL(query_keys_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, n_inst, total_q, k0, v0, ws->q_anchor.as<uint32_t>());
+                       dev_q_prefix, n_inst, total_q, k0, v0);
     PM_HIP(hipGetLastError());
     if (int rc = radix_sort_u32(ws, k0, v0, k1, v1, total_q, 3 * kGridBits, s, &which))
       return rc;
+    hipLaunchKernelGGL(query_anchor_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
+                       dev_q_prefix, n_inst, total_q, which ? k1 : k0,
+                       ws->q_anchor.as<uint32_t>());
   } else {
     uint64_t* k0 = ws->keys.as<uint64_t>();
     uint64_t* k1 = ws->keys2.as<uint64_t>();
     int ib = 1;
     while ((1 << ib) < n_inst) ++ib;
     hipLaunchKernelGGL(query_keys_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, n_inst, total_q, k0, v0, ws->q_anchor.as<uint32_t>());
+                       dev_q_prefix, n_inst, total_q, k0, v0);
     PM_HIP(hipGetLastError());
     if (int rc = radix_sort_u64(ws, k0, v0, k1, v1, total_q, 32 + ib, s, &which)) return rc;
+    hipLaunchKernelGGL(query_anchor_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
+                       dev_q_prefix, n_inst, total_q, which ? k1 : k0,
+                       ws->q_anchor.as<uint32_t>());
   }
   PM_HIP(hipGetLastError());
   *perm_out = which ? v1 : v0;

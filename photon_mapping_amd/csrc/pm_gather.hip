@@ -1048,7 +1048,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
     // anchor a lie within r_a of its centroid c_a, hence within
     // r_a + |q - c_a| of q.  The anchor radii are already rounded upward.
     const float4* __restrict__ A = D.anchors;
-    const int32_t center = static_cast<int32_t>(D.q_anchor[qi]);
+    const int32_t center = static_cast<int32_t>(D.q_anchor[active ? qloc : lg * kWave]);
     const int32_t a0 = max(center - kAnchorWindow, 0);
     const int32_t a1 = min(center + kAnchorWindow, D.n_anchors - 1);
     float best = INFINITY;

@@ -86,7 +86,7 @@ struct alignas(16) GatherDesc {
   const pm_query* q;       // RADIANCE / ADJOINT: local query records
   const float* q_soa;      // KNN: 3*m positions
   const pm_material* mats;
-  const uint32_t* q_anchor;  // local query index -> nearest anchor (slot order)
+  const uint32_t* q_anchor;  // local sorted position -> nearest anchor (slot order)
   // outputs
   float* out_rgb;
   float* out_r2;
@@ -225,8 +225,8 @@ int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** 
 void fill_desc_map_part(const PhotonMapImpl* map, GatherDesc* d);
 
 // Hilbert keys for all queries of all instances, stable sort, and the
-// nearest radius anchor of every query (ws->q_anchor, global query order;
-// instance i starts at q_prefix[i], which is also its first sorted position).
+// nearest radius anchor of every sorted query (ws->q_anchor, global sorted
+// order; instance i starts at q_prefix[i] there, as in query order).
 // Reserves every buffer prepare_queries and the gather launch use.
 int prepare_reserve(Workspace* ws, int n_inst, int64_t total_q, int64_t total_groups);
 int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,

@@ -32,13 +32,16 @@ __global__ __launch_bounds__(kSortThreads) void digit_hist_kernel(
   h[t] = 0;
   __syncthreads();
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
-#pragma unroll 4
+  K k[kSortItems];
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int64_t i = base + static_cast<int64_t>(r) * kSortThreads + t;
-    if (i < n) {
-      const uint32_t d = static_cast<uint32_t>(keys[i] >> shift) & 255u;
-      atomicAdd(&h[d], 1u);
-    }
+    k[r] = i < n ? keys[i] : K(0);
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = base + static_cast<int64_t>(r) * kSortThreads + t;
+    if (i < n) atomicAdd(&h[static_cast<uint32_t>(k[r] >> shift) & 255u], 1u);
   }
   __syncthreads();
   counts[static_cast<int64_t>(t) * n_tiles + blockIdx.x] = h[t];
@@ -156,20 +159,30 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   loc[t] = l0;
   seen[t] = 0;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // every element of the thread is loaded up front, so the tile's loads are
+  // in flight together rather than one round at a time
+  K keys_r[kSortItems];
+  uint32_t vals_r[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int li = r * kSortThreads + t;
+    keys_r[r] = 0;
+    vals_r[r] = 0;
+    if (li < tile_n) {
+      keys_r[r] = kin[base + li];
+      vals_r[r] = vin[base + li];
+    }
+  }
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
 #pragma unroll
     for (int i = 0; i < kSortThreads / 64; ++i) wcnt[i][t] = 0;
     __syncthreads();
     const int li = r * kSortThreads + t;
     const bool valid = li < tile_n;
-    K key = 0;
-    uint32_t val = 0;
-    uint32_t d = 0;
-    if (valid) {
-      key = kin[base + li];
-      val = vin[base + li];
-      d = static_cast<uint32_t>(key >> shift) & 255u;
-    }
+    const K key = keys_r[r];
+    const uint32_t val = vals_r[r];
+    const uint32_t d = valid ? static_cast<uint32_t>(key >> shift) & 255u : 0u;
     const uint64_t vmask = __ballot(valid);
     uint64_t peers = valid ? vmask : ~vmask;
 #pragma unroll

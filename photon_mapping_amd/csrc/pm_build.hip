@@ -11,8 +11,8 @@
 //      (median split by count over the Hilbert order);
 //   6. radius anchors every anchor_stride slots (centroid + Hilbert code),
 //      and a table from 16-bit code prefixes to anchors.
-// The anchors' k-NN radius bounds are computed lazily per k by running the
-// gather kernel itself in anchor mode.
+// The anchors' k-NN radius bounds are computed lazily per k by the anchor
+// bound kernel (launched through the gather driver in anchor mode).
 
 #include <hip/hip_runtime.h>
 
@@ -564,7 +564,7 @@ int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** 
   L.perm = nullptr;
   L.q_prefix = nullptr;
   L.counters = ctx->ws.counters.as<uint32_t>();
-  L.total_groups = d.n_groups;
+  L.total_groups = d.m;  // the anchor kernel takes one anchor per wave at a time
   L.n_inst = 1;
   L.xcd_aware = 0;
   L.count_stats = 0;

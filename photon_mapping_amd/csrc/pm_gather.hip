@@ -980,11 +980,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
   bool lambert = false;
   int32_t mat = -1;
   float rho_r = 0.f, rho_g = 0.f, rho_b = 0.f;
-  if (MODE == kModeAnchor) {
-    qi = active ? qloc : lg * kWave;
-    const float4 a = D.anchors[qi];
-    qx = a.x; qy = a.y; qz = a.z;
-  } else {
+  {
     const int64_t sp = D.q_base + (active ? qloc : lg * kWave);
     qi = static_cast<int64_t>(perm[sp]) - D.q_base;
     if (MODE == kModeKnn) {
@@ -1016,33 +1012,6 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
     r0sq = __uint_as_float(static_cast<uint32_t>(tau_in >> 32));
   } else if (all) {
     r0sq = INFINITY;
-  } else if (MODE == kModeAnchor) {
-    // Union box of the leaves around this anchor that hold >= k photons;
-    // every one of those photons is inside it, so the farthest box corner
-    // bounds the k-NN distance of the anchor centroid.
-    const int32_t leaf_size = D.leaf_size;
-    const int32_t leaf = static_cast<int32_t>((qi * D.anchor_stride) / leaf_size);
-    int32_t j0 = leaf, j1 = leaf;
-    auto count_of = [&](int32_t a, int32_t b) -> int64_t {
-      const int64_t hi = min(static_cast<int64_t>(b + 1) * leaf_size, static_cast<int64_t>(D.n));
-      return hi - static_cast<int64_t>(a) * leaf_size;
-    };
-    while (count_of(j0, j1) < static_cast<int64_t>(k_eff)) {
-      if (j0 > 0) --j0;
-      if (j1 < D.n_leaves - 1) ++j1;
-    }
-    float4 mn = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
-    float4 mx = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
-    for (int32_t j = j0; j <= j1; ++j) {
-      const float4 a = D.boxes[2 * (D.first_leaf + j)];
-      const float4 b = D.boxes[2 * (D.first_leaf + j) + 1];
-      mn.x = fminf(mn.x, a.x); mn.y = fminf(mn.y, a.y); mn.z = fminf(mn.z, a.z);
-      mx.x = fmaxf(mx.x, b.x); mx.y = fmaxf(mx.y, b.y); mx.z = fmaxf(mx.z, b.z);
-    }
-    const float ex = fmaxf(fabsf(qx - mn.x), fabsf(mx.x - qx));
-    const float ey = fmaxf(fabsf(qy - mn.y), fabsf(mx.y - qy));
-    const float ez = fmaxf(fabsf(qz - mn.z), fabsf(mx.z - qz));
-    r0sq = (ex * ex + ey * ey + ez * ez) * 1.0001f + 1e-30f;
   } else {
     // Triangle inequality on precomputed anchors: the k nearest photons of
     // anchor a lie within r_a of its centroid c_a, hence within
@@ -1409,10 +1378,6 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
 
   if (MODE == kModeAdjoint || !active) return;
   const float r2 = (k_eff > 0) ? __uint_as_float(static_cast<uint32_t>(tau >> 32)) : 0.0f;
-  if (MODE == kModeAnchor) {
-    D.anchor_out[4 * qi + 3] = sqrt_up(r2) * 1.00002f + 1e-30f;
-    return;
-  }
   if (D.out_r2) D.out_r2[qi] = r2;
   if (D.out_key) D.out_key[qi] = (k_eff > 0) ? tau : 0ull;
   if (kRows) {
@@ -1493,6 +1458,88 @@ __global__ __launch_bounds__(kWave, PM_GATHER_MIN_WAVES) void gather_kernel(Gath
   }
 }
 
+// Radius bound r_a of every anchor for k_eff: the k_eff nearest photons of
+// the anchor's centroid lie within r_a.  One anchor per wave at a time.
+//   * Any list of photons of which at least k_eff have FP32 d^2(c, p) below
+//     some t gives the bound t: missing photons could only be nearer.  A
+//     wave-parallel histogram of d^2 over a candidate list, 256 bins over
+//     [0, rb), finds the smallest such bin end below rb when there is one.
+//   * rb starts from the local photon density: the spread s of the anchor's
+//     own photons, scaled to k_eff photons on a surface (s^2 k_eff / stride,
+//     times 2.25).  Consecutive slots can straddle a jump of the curve (it
+//     leaves a surface and re-enters it elsewhere), which inflates s, so the
+//     search keeps a bracket: a sphere whose leaf list overflows (a coarse
+//     list, slow to sweep) shrinks it, one holding fewer than k_eff photons
+//     grows it, and no coarse list is ever swept: in a dark region next to
+//     a bright one a single wave would otherwise sweep tens of thousands of
+//     photons.  rb never exceeds the distance to the farthest of the k_eff
+//     consecutive slots centred on the anchor (they are k_eff photons), and
+//     that distance is the bound when no sphere succeeds within 7 tries.
+//   * A second histogram over [0, first bound) narrows the bound to one of
+//     its bins (about 0.2% of the radius).
+// A bound is all the gather needs from an anchor (it is used through the
+// triangle inequality), so no exact selection is run.
+__global__ __launch_bounds__(kWave) void anchor_bound_kernel(GatherLaunch L) {
+  __shared__ GatherShared sh;
+  const int lane = threadIdx.x;
+  DescRef D = cptr(L.descs)[0];
+  const uint32_t k_eff = static_cast<uint32_t>(D.k_eff);
+  const int64_t n = D.n;
+  const int64_t stride = D.anchor_stride;
+  const float4* __restrict__ P = D.pos4;
+  for (int64_t a = blockIdx.x; a < D.m; a += gridDim.x) {
+    const float4 an = D.anchors[a];
+    const float cx = an.x, cy = an.y, cz = an.z;
+    // the guaranteed bound: the farthest of k_eff consecutive slots
+    const int64_t mid = a * stride + stride / 2;
+    int64_t s0 = mid - static_cast<int64_t>(k_eff / 2);
+    s0 = max(min(s0, n - static_cast<int64_t>(k_eff)), static_cast<int64_t>(0));
+    float far2 = 0.0f;
+    for (int64_t sl = s0 + lane; sl < s0 + static_cast<int64_t>(k_eff); sl += kWave) {
+      const float4 p = P[sl];
+      far2 = fmaxf(far2, d2_ref(cx, cy, cz, p.x, p.y, p.z));
+    }
+    const float r0sq = wave_max(far2) * 1.0001f + 1e-30f;
+    // the density guess
+    const int64_t a0 = a * stride;
+    const int64_t na = min(a0 + stride, n) - a0;
+    float sp2 = 0.0f;
+    for (int64_t sl = a0 + lane; sl < a0 + na; sl += kWave) {
+      const float4 p = P[sl];
+      sp2 = fmaxf(sp2, d2_ref(cx, cy, cz, p.x, p.y, p.z));
+    }
+    sp2 = wave_max(sp2);
+    float rb = fminf(sp2 * (static_cast<float>(k_eff) / static_cast<float>(na)) * 2.25f, r0sq);
+    float few = 0.0f;       // largest rb seen holding fewer than k_eff photons
+    float over = INFINITY;  // smallest rb seen whose leaf list overflowed
+    float r = sqrt_up(r0sq) * 1.00002f + 1e-30f;
+    for (int attempt = 0; attempt < 7 && rb > 0.0f; ++attempt) {
+      uint32_t total = 0;
+      bool fine = false;
+      const uint32_t nr = build_list(D, cx, cy, cz, rb, sh, lane, &total, nullptr, true, &fine);
+      if (!fine) {
+        over = rb;
+        rb = few > 0.0f ? sqrtf(few * over) : rb * 0.25f;
+        continue;
+      }
+      float t_lo = 0.0f, t_hi = INFINITY;
+      if (centre_bound(D, sh, lane, nr, total, fine, cx, cy, cz, rb, k_eff, &t_lo, &t_hi)) {
+        const float rb2 = t_hi;
+        r = fminf(r, sqrt_up(rb2) * 1.00002f + 1e-30f);
+        if (centre_bound(D, sh, lane, nr, total, fine, cx, cy, cz, rb2, k_eff, &t_lo, &t_hi)) {
+          r = fminf(r, sqrt_up(t_hi) * 1.00002f + 1e-30f);
+        }
+        break;
+      }
+      if (!(rb < r0sq)) break;  // the slot sphere held its k_eff photons: cannot happen
+      few = rb;
+      rb = over < INFINITY ? sqrtf(few * over) : fminf(rb * 4.0f, r0sq);
+    }
+    if (lane == 0) D.anchor_out[4 * a + 3] = r;
+    __syncthreads();
+  }
+}
+
 __global__ void zero_counters_kernel(uint32_t* c, int n) {
   const int i = threadIdx.x;
   if (i < n) c[i] = 0;
@@ -1515,8 +1562,8 @@ int launch_gather(Context* ctx, const GatherLaunch& L, hipStream_t s) {
     case kModeKnn:
       hipLaunchKernelGGL(gather_kernel<kModeKnn>, grid, dim3(kWave), 0, s, L);
       break;
-    case kModeAnchor:
-      hipLaunchKernelGGL(gather_kernel<kModeAnchor>, grid, dim3(kWave), 0, s, L);
+    case kModeAnchor:  // one anchor per wave at a time (total_groups = anchors)
+      hipLaunchKernelGGL(anchor_bound_kernel, grid, dim3(kWave), 0, s, L);
       break;
     case kModeAdjoint:
       hipLaunchKernelGGL(gather_kernel<kModeAdjoint>, grid, dim3(kWave), 0, s, L);

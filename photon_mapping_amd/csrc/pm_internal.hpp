@@ -123,7 +123,7 @@ struct alignas(16) GatherDesc {
 enum GatherMode : int {
   kModeRadiance = 0,
   kModeKnn = 1,
-  kModeAnchor = 2,
+  kModeAnchor = 2,    // anchor radius bounds (anchor_bound_kernel)
   kModeAdjoint = 3,
 };
 

@@ -114,7 +114,7 @@ struct GatherShared {
   union {
     struct {
       float4 pos[kChunk];
-      float4 wif[kChunk];
+      float4 wif[kChunk];  // with a shared normal: effective flux (rgb, ambiguous flag)
       float2 flux[kChunk];
     } c;
     uint32_t front[2][kFront];
@@ -491,14 +491,39 @@ __device__ __forceinline__ int32_t list_slot(DescRef D, const GatherShared& sh, 
   return s0 + static_cast<int32_t>(t - sh.rl.rpre[r]);
 }
 
-// Iterates the candidate photons of the current list: body(cnt) is called
-// once per staged chunk, after sh.u.c.pos (and with WF also sh.u.c.wif/flux)
-// hold cnt candidates.  The next chunk's loads are in flight while body
+// What the candidate loop stages per photon besides its position.
+enum Stage : int {
+  kStagePos = 0,   // position only
+  kStageFull = 1,  // + wi and flux, and the slot
+  kStageEff = 2,   // + the flux the run's shared Lambert sign lets through, and the slot
+};
+
+// A normal shared (within eps) by the accumulating lanes of a run, and the
+// margin term m = eps * 1.0001 + 1e-6 * max |n| of the sign decision.
+struct SharedNormal {
+  float x, y, z, m;
+};
+
+// Iterates the candidate photons of the current list: body(cnt, amb) is
+// called once per staged chunk, after sh.u.c.pos (and with kStageFull also
+// sh.u.c.wif/flux, with kStageEff the effective flux in sh.u.c.wif) hold cnt
+// candidates; amb has bit j set for each staged photon j whose sign
+// kStageEff left open.  The next chunk's loads are in flight while body
 // runs.
-template <bool WF, typename Body>
-__device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int lane,
-                                               uint32_t nr, uint32_t total, Body&& body) {
+//
+// kStageEff decides each photon's Lambert sign once for all lanes: with d0 =
+// dot(wi, n0) in the reference's FP32 order, |d0| > |wi|_1 * m implies that
+// every lane's own FP32 dot(wi, n) has the sign of d0 (|n - n0| <= eps moves
+// the exact dot by at most |wi| eps, and each FP32 dot is within 1.8e-7
+// |wi|_1 |n| of exact).  The staged flux is then the photon's (positive sign)
+// or zero; otherwise it is zero and the photon is left to the lane's own
+// test (its bit in amb).
+template <int ST, typename Body>
+__device__ __forceinline__ void for_candidates_st(DescRef D, GatherShared& sh, int lane,
+                                                  uint32_t nr, uint32_t total,
+                                                  const SharedNormal& sn, Body&& body) {
   if (total == 0) return;
+  constexpr bool kWF = ST != kStagePos;
   const float4* __restrict__ P = D.pos4;
   const float4* __restrict__ W = D.wif4;
   const float2* __restrict__ F = D.flux2;
@@ -510,33 +535,50 @@ __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int 
   if (t < total) {
     slot = list_slot(D, sh, nr, t, r);
     p = P[slot];
-    if (WF) {
+    if (kWF) {
       w = W[slot];
       f = F[slot];
     }
   }
   for (uint32_t o = 0; o < total; o += kChunk) {
     sh.u.c.pos[lane] = p;
-    if (WF) {
+    if (ST == kStageFull) {
       sh.u.c.wif[lane] = w;
       sh.u.c.flux[lane] = f;
-      sh.sb.chunk_slot[lane] = static_cast<uint32_t>(slot);
     }
+    uint64_t amb_mask = 0ull;
+    if (ST == kStageEff) {
+      const float d0 = dot_ref(w.x, w.y, w.z, sn.x, sn.y, sn.z);
+      const float mw = (fabsf(w.x) + fabsf(w.y) + fabsf(w.z)) * sn.m + 1e-30f;
+      const bool amb = !(fabsf(d0) > mw);
+      const bool inc = !amb && d0 > 0.0f;
+      sh.u.c.wif[lane] = make_float4(inc ? w.w : 0.0f, inc ? f.x : 0.0f, inc ? f.y : 0.0f, 0.0f);
+      amb_mask = __ballot(amb && o + static_cast<uint32_t>(lane) < total);
+    }
+    if (kWF) sh.sb.chunk_slot[lane] = static_cast<uint32_t>(slot);
     __syncthreads();
     t += kChunk;
     if (t < total) {
       slot = list_slot(D, sh, nr, t, r);
       p = P[slot];
-      if (WF) {
+      if (kWF) {
         w = W[slot];
         f = F[slot];
       }
     }
     const uint32_t cnt = total - o < static_cast<uint32_t>(kChunk) ? total - o
                                                                     : static_cast<uint32_t>(kChunk);
-    body(cnt);
+    body(cnt, amb_mask);
     __syncthreads();
   }
+}
+
+template <bool WF, typename Body>
+__device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int lane,
+                                               uint32_t nr, uint32_t total, Body&& body) {
+  const SharedNormal none{0.0f, 0.0f, 0.0f, 0.0f};
+  for_candidates_st<WF ? kStageFull : kStagePos>(D, sh, lane, nr, total, none,
+                                                 [&](uint32_t cnt, uint64_t) { body(cnt); });
 }
 
 // Lane-parallel sweep of the current list: every lane takes every 64th
@@ -668,34 +710,31 @@ struct Cand {
   float2 f;   // flux.g, flux.b
 };
 
-template <bool WF>
+template <int ST>
 __device__ __forceinline__ Cand load_cand(const GatherShared& sh, uint32_t j) {
   Cand c;
   c.p = sh.u.c.pos[j];
-  if (WF) {
-    c.w = sh.u.c.wif[j];
-    c.f = sh.u.c.flux[j];
-  } else {
-    c.w = make_float4(0.f, 0.f, 0.f, 0.f);
-    c.f = make_float2(0.f, 0.f);
-  }
+  c.w = make_float4(0.f, 0.f, 0.f, 0.f);
+  c.f = make_float2(0.f, 0.f);
+  if (ST != kStagePos) c.w = sh.u.c.wif[j];
+  if (ST == kStageFull) c.f = sh.u.c.flux[j];
   return c;
 }
 
 // Calls use(cand) for the cnt staged candidates, reading them in blocks
-// (8 positions, or 4 full records with WF) into registers first, so that the
+// (8 positions, or 4 records when more is staged) into registers first, so that the
 // LDS reads of a block are in flight together.  use returns a per-lane flag;
 // the flags form a 64-bit mask over the chunk, which is returned for
 // draining.
-template <bool WF, typename Use>
-__device__ __forceinline__ uint64_t consume(const GatherShared& sh, uint32_t cnt, Use&& use) {
-  constexpr uint32_t kBlk = WF ? 4 : 8;
+template <int ST, typename Use>
+__device__ __forceinline__ uint64_t consume_st(const GatherShared& sh, uint32_t cnt, Use&& use) {
+  constexpr uint32_t kBlk = ST == kStagePos ? 8 : 4;
   uint32_t mlo = 0, mhi = 0;
   uint32_t j = 0;
   for (; j + kBlk <= cnt; j += kBlk) {
     Cand c[kBlk];
 #pragma unroll
-    for (uint32_t u = 0; u < kBlk; ++u) c[u] = load_cand<WF>(sh, j + u);
+    for (uint32_t u = 0; u < kBlk; ++u) c[u] = load_cand<ST>(sh, j + u);
     uint32_t bits = 0;
 #pragma unroll
     for (uint32_t u = 0; u < kBlk; ++u) bits |= use(c[u]) ? (1u << u) : 0u;
@@ -703,11 +742,16 @@ __device__ __forceinline__ uint64_t consume(const GatherShared& sh, uint32_t cnt
     else mhi |= bits << (j - 32);
   }
   for (; j < cnt; ++j) {
-    const uint32_t bit = use(load_cand<WF>(sh, j)) ? 1u : 0u;
+    const uint32_t bit = use(load_cand<ST>(sh, j)) ? 1u : 0u;
     if (j < 32) mlo |= bit << j;
     else mhi |= bit << (j - 32);
   }
   return (static_cast<uint64_t>(mhi) << 32) | mlo;
+}
+
+template <bool WF, typename Use>
+__device__ __forceinline__ uint64_t consume(const GatherShared& sh, uint32_t cnt, Use&& use) {
+  return consume_st<WF ? kStageFull : kStagePos>(sh, cnt, use);
 }
 
 // Calls fn(j) for every set bit j of each lane's mask (per-lane loop; the
@@ -1121,13 +1165,38 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
       // follows once tau is known)
       if (!reuse) {
         const bool acc_lane = part && lane_lambert;
-        auto pass_b = [&](auto kind_tag) {
+        // When the accumulating lanes' normals agree within 5%, each photon's
+        // Lambert sign is decided once per chunk (kStageEff) instead of per
+        // lane; photons it leaves open take the lane's own test in the drain.
+        SharedNormal sn{0.0f, 0.0f, 0.0f, 0.0f};
+        bool shared = false;
+        if (kAccum && !all) {
+          const uint64_t am = __ballot(acc_lane);
+          if (am != 0ull) {
+            const int l0 = __builtin_ctzll(am);
+            sn.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nx), l0));
+            sn.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ny), l0));
+            sn.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nz), l0));
+            const float ex = nx - sn.x, ey = ny - sn.y, ez = nz - sn.z;
+            const float dev = acc_lane ? sqrtf(ex * ex + ey * ey + ez * ez) : 0.0f;
+            const float nl = acc_lane ? sqrtf(nx * nx + ny * ny + nz * nz) : 0.0f;
+            const float eps = wave_max(dev);
+            const float nmax = wave_max(nl);
+            if (eps <= 0.05f * nmax && nmax <= FLT_MAX) {
+              shared = true;
+              sn.m = eps * 1.0001f + 1e-6f * nmax;
+            }
+          }
+        }
+        auto pass_b = [&](auto kind_tag, auto shared_tag) {
           constexpr int kind = decltype(kind_tag)::value;  // 0 all, 1 keys, 2 d^2 bits
+          constexpr bool kShared = decltype(shared_tag)::value;
+          constexpr int kSt = kShared ? kStageEff : (kAccum ? kStageFull : kStagePos);
           const uint64_t span = st.khi - st.klo;
           const uint32_t lo32 = static_cast<uint32_t>(st.klo >> 32);
           const uint32_t span32 = static_cast<uint32_t>(st.khi >> 32) - lo32;
-          for_candidates<kAccum>(D, sh, lane, nr, total, [&](uint32_t cnt) {
-            const uint64_t m = consume<kAccum>(sh, cnt, [&](const Cand& c) {
+          for_candidates_st<kSt>(D, sh, lane, nr, total, sn, [&](uint32_t cnt, uint64_t amb) {
+            const uint64_t m = consume_st<kSt>(sh, cnt, [&](const Cand& c) {
               const float d2 = d2_ref(qx, qy, qz, c.p.x, c.p.y, c.p.z);
               bool nb, flag;
               if constexpr (kind == 0) {
@@ -1144,7 +1213,12 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
                 nb = db < lo32;
                 flag = part && ((db - lo32) < span32 || (kRows && nb));
               }
-              if (kAccum) {
+              if constexpr (kShared) {
+                const float sel = (acc_lane && nb) ? 1.0f : 0.0f;
+                acc_r = fmaf(sel, c.w.x, acc_r);
+                acc_g = fmaf(sel, c.w.y, acc_g);
+                acc_b = fmaf(sel, c.w.z, acc_b);
+              } else if (kAccum) {
                 const float cos_i = dot_ref(c.w.x, c.w.y, c.w.z, nx, ny, nz);
                 const float sel = (acc_lane && nb && !(cos_i < 0.0f)) ? 1.0f : 0.0f;
                 acc_r = fmaf(sel, c.w.w, acc_r);
@@ -1165,14 +1239,36 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
                 stash.put(sh, lane, key, kAccum ? sh.sb.chunk_slot[j] : 0u);
               }
             });
+            // photons whose sign the shared normal left open (rare): each
+            // neighbouring lane's own test
+            while (kShared && amb != 0ull) {
+              const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(amb));
+              amb &= amb - 1ull;
+              const float4 p = sh.u.c.pos[j];
+              const float d2 = d2_ref(qx, qy, qz, p.x, p.y, p.z);
+              if (acc_lane && make_key(d2, __float_as_uint(p.w)) < st.klo) {
+                const uint32_t s = sh.sb.chunk_slot[j];
+                const float4 w = D.wif4[s];
+                const float2 f = D.flux2[s];
+                if (!(dot_ref(w.x, w.y, w.z, nx, ny, nz) < 0.0f)) {
+                  acc_r = acc_r + w.w;
+                  acc_g = acc_g + f.x;
+                  acc_b = acc_b + f.y;
+                }
+              }
+            }
           });
         };
+        using No = std::false_type;
+        using Yes = std::true_type;
         if (all) {
-          pass_b(Kind<0>{});
+          pass_b(Kind<0>{}, No{});
         } else if (wave_any(part && ((st.klo | st.khi) & 0xffffffffull) != 0ull)) {
-          pass_b(Kind<1>{});
+          if (shared) pass_b(Kind<1>{}, Yes{});
+          else pass_b(Kind<1>{}, No{});
         } else {
-          pass_b(Kind<2>{});
+          if (shared) pass_b(Kind<2>{}, Yes{});
+          else pass_b(Kind<2>{}, No{});
         }
         pt.mark(5, lane);
       }

@@ -49,6 +49,49 @@ def test_gather_surfaces(gpu, oracle, rng, k):
     _check(rgb, oracle.radiance_fp64(wi, flux, q, mats, bidx, br2, n_emit))
 
 
+@pytest.mark.parametrize("tilt", [0.0, 1e-3])
+def test_gather_shared_normal_grazing_photons(gpu, oracle, rng, tilt):
+    """Queries on one plane share their normal (exactly, or within ~1e-3), so
+    the kernel decides each photon's Lambert sign once per chunk and leaves
+    photons near grazing incidence (wi . n exactly 0, or a few ulps either
+    side) to each query's own FP32 test.  The estimate must follow the
+    oracle's exact sign tests."""
+    from util import unit
+
+    n, m, k = 20000, 3000, 64
+    pos = np.zeros((n, 3), np.float32)
+    pos[:, 0] = rng.random(n)
+    pos[:, 2] = rng.random(n)
+    wi = unit(rng.normal(size=(n, 3)))
+    wi[:, 1] = np.abs(wi[:, 1])
+    graze = rng.random(n) < 0.4
+    ang = rng.uniform(0.0, 2.0 * np.pi, graze.sum())
+    dy = rng.choice(np.array([0.0, 1e-8, -1e-8, 1e-7, -1e-7, 1e-6, -1e-6, 3e-6, -3e-6]),
+                    graze.sum())
+    wi[graze] = np.stack([np.cos(ang), dy, np.sin(ang)], 1).astype(np.float32)
+    flux = rng.uniform(0.1, 1.0, (n, 3)).astype(np.float32)
+    qpos = pos[rng.integers(0, n, m)].copy()
+    qpos[:, 0] += rng.normal(scale=0.002, size=m).astype(np.float32)
+    qpos[:, 2] += rng.normal(scale=0.002, size=m).astype(np.float32)
+    nrm = np.tile(np.array([0.0, 1.0, 0.0], np.float32), (m, 1))
+    if tilt:
+        nrm = unit(nrm + rng.normal(scale=tilt, size=nrm.shape)).astype(np.float32)
+    q = query_records(rng, qpos, nrm)
+    mats = materials(3)
+    print(f"gather shared normal tilt={tilt}", flush=True)
+    rgb, r2, _ = _map(gpu, pos, wi, flux).gather_radiance(q, mats, k, 1e5)
+    bidx, br2 = oracle.brute_knn(pos, q[:, :3], k)
+    np.testing.assert_array_equal(r2.view(np.uint32), br2.view(np.uint32))
+    expect = oracle.radiance_fp64(wi, flux, q, mats, bidx, br2, 1e5)
+    _check(rgb, expect)
+    # the grazing photons matter: dropping them changes the estimate
+    flat = wi.copy()
+    flat[graze] = 0.0
+    flat[graze, 1] = -1.0
+    assert not np.allclose(oracle.radiance_fp64(flat, flux, q, mats, bidx, br2, 1e5), expect,
+                           rtol=1e-3)
+
+
 def test_gather_cube_random_normals(gpu, oracle, rng):
     pos, wi, flux = photons_cube(rng, 20000)
     q = query_records(rng, queries_for(rng, pos, 2000))

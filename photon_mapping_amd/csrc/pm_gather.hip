@@ -581,6 +581,28 @@ __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int 
                                                  [&](uint32_t cnt, uint64_t) { body(cnt); });
 }
 
+// The normal of the lanes with on set, when they agree: true (and *sn set,
+// with its margin term) when every such lane's normal lies within 5% of its
+// length of the first one's.
+__device__ __forceinline__ bool shared_normal(bool on, float nx, float ny, float nz,
+                                              SharedNormal* sn) {
+  *sn = SharedNormal{0.0f, 0.0f, 0.0f, 0.0f};
+  const uint64_t am = __ballot(on);
+  if (am == 0ull) return false;
+  const int l0 = __builtin_ctzll(am);
+  sn->x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nx), l0));
+  sn->y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ny), l0));
+  sn->z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nz), l0));
+  const float ex = nx - sn->x, ey = ny - sn->y, ez = nz - sn->z;
+  const float dev = on ? sqrtf(ex * ex + ey * ey + ez * ez) : 0.0f;
+  const float nl = on ? sqrtf(nx * nx + ny * ny + nz * nz) : 0.0f;
+  const float eps = wave_max(dev);
+  const float nmax = wave_max(nl);
+  if (!(eps <= 0.05f * nmax && nmax <= FLT_MAX)) return false;
+  sn->m = eps * 1.0001f + 1e-6f * nmax;
+  return true;
+}
+
 // Lane-parallel sweep of the current list: every lane takes every 64th
 // candidate, four loads in flight at a time, and calls fn(u, ok, position,
 // slot) for its four candidates of a step with every lane active (ok tells
@@ -1169,25 +1191,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         // Lambert sign is decided once per chunk (kStageEff) instead of per
         // lane; photons it leaves open take the lane's own test in the drain.
         SharedNormal sn{0.0f, 0.0f, 0.0f, 0.0f};
-        bool shared = false;
-        if (kAccum && !all) {
-          const uint64_t am = __ballot(acc_lane);
-          if (am != 0ull) {
-            const int l0 = __builtin_ctzll(am);
-            sn.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nx), l0));
-            sn.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ny), l0));
-            sn.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nz), l0));
-            const float ex = nx - sn.x, ey = ny - sn.y, ez = nz - sn.z;
-            const float dev = acc_lane ? sqrtf(ex * ex + ey * ey + ez * ez) : 0.0f;
-            const float nl = acc_lane ? sqrtf(nx * nx + ny * ny + nz * nz) : 0.0f;
-            const float eps = wave_max(dev);
-            const float nmax = wave_max(nl);
-            if (eps <= 0.05f * nmax && nmax <= FLT_MAX) {
-              shared = true;
-              sn.m = eps * 1.0001f + 1e-6f * nmax;
-            }
-          }
-        }
+        const bool shared = kAccum && !all && shared_normal(acc_lane, nx, ny, nz, &sn);
         auto pass_b = [&](auto kind_tag, auto shared_tag) {
           constexpr int kind = decltype(kind_tag)::value;  // 0 all, 1 keys, 2 d^2 bits
           constexpr bool kShared = decltype(shared_tag)::value;
@@ -1369,19 +1373,53 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         float s_r = 0.f, s_g = 0.f, s_b = 0.f;
         const uint32_t t_hi = static_cast<uint32_t>(tau >> 32);
         const uint32_t t_lo = static_cast<uint32_t>(tau & 0xffffffffull);
-        for_candidates<true>(D, sh, lane, nr, total, [&](uint32_t cnt) {
-          consume<true>(sh, cnt, [&](const Cand& c) {
-            const float d2 = d2_ref(qx, qy, qz, c.p.x, c.p.y, c.p.z);
-            const uint32_t db = __float_as_uint(d2);
-            const bool nb = db < t_hi || (db == t_hi && __float_as_uint(c.p.w) <= t_lo);
-            const float cos_i = dot_ref(c.w.x, c.w.y, c.w.z, nx, ny, nz);
-            const float sel = (contributes && nb && !(cos_i < 0.0f)) ? 1.0f : 0.0f;
-            s_r = fmaf(sel, c.w.w, s_r);
-            s_g = fmaf(sel, c.f.x, s_g);
-            s_b = fmaf(sel, c.f.y, s_b);
-            return false;
+        // (the Lambert sign is decided per chunk when the contributing
+        // lanes share their normal, as in the forward selection pass)
+        SharedNormal sn;
+        const bool shared = shared_normal(contributes, nx, ny, nz, &sn);
+        auto pass1 = [&](auto shared_tag) {
+          constexpr bool kSh = decltype(shared_tag)::value;
+          constexpr int kSt = kSh ? kStageEff : kStageFull;
+          for_candidates_st<kSt>(D, sh, lane, nr, total, sn, [&](uint32_t cnt, uint64_t amb) {
+            consume_st<kSt>(sh, cnt, [&](const Cand& c) {
+              const float d2 = d2_ref(qx, qy, qz, c.p.x, c.p.y, c.p.z);
+              const uint32_t db = __float_as_uint(d2);
+              const bool nb = db < t_hi || (db == t_hi && __float_as_uint(c.p.w) <= t_lo);
+              if constexpr (kSh) {
+                const float sel = (contributes && nb) ? 1.0f : 0.0f;
+                s_r = fmaf(sel, c.w.x, s_r);
+                s_g = fmaf(sel, c.w.y, s_g);
+                s_b = fmaf(sel, c.w.z, s_b);
+              } else {
+                const float cos_i = dot_ref(c.w.x, c.w.y, c.w.z, nx, ny, nz);
+                const float sel = (contributes && nb && !(cos_i < 0.0f)) ? 1.0f : 0.0f;
+                s_r = fmaf(sel, c.w.w, s_r);
+                s_g = fmaf(sel, c.f.x, s_g);
+                s_b = fmaf(sel, c.f.y, s_b);
+              }
+              return false;
+            });
+            while (kSh && amb != 0ull) {
+              const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(amb));
+              amb &= amb - 1ull;
+              const float4 p = sh.u.c.pos[j];
+              const uint32_t db = __float_as_uint(d2_ref(qx, qy, qz, p.x, p.y, p.z));
+              const bool nb = db < t_hi || (db == t_hi && __float_as_uint(p.w) <= t_lo);
+              if (contributes && nb) {
+                const uint32_t sl = sh.sb.chunk_slot[j];
+                const float4 w = D.wif4[sl];
+                const float2 f = D.flux2[sl];
+                if (!(dot_ref(w.x, w.y, w.z, nx, ny, nz) < 0.0f)) {
+                  s_r = s_r + w.w;
+                  s_g = s_g + f.x;
+                  s_b = s_b + f.y;
+                }
+              }
+            }
           });
-        });
+        };
+        if (shared) pass1(std::true_type{});
+        else pass1(std::false_type{});
         if (contributes) {
           const float inv = 1.0f / (kPiF * denom);
           const float dr = ge_r * s_r * inv, dg = ge_g * s_g * inv, db = ge_b * s_b * inv;
@@ -1440,24 +1478,38 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
               w = W[slot];
             }
             const uint32_t idx = __float_as_uint(p.w);
-            float a_r = 0.f, a_g = 0.f, a_b = 0.f;
-            uint64_t mm = cm;
-            while (mm != 0ull) {
-              const int i = __builtin_ctzll(mm);
-              mm &= mm - 1ull;
-              const float4 A = qa[i];
-              const float4 B = qb[i];
-              const float4 C = qc[i];
-              const float d2 = d2_ref(A.x, A.y, A.z, p.x, p.y, p.z);
-              const uint32_t db = __float_as_uint(d2);
-              const uint32_t th = __float_as_uint(A.w);
-              const bool nb = db < th || (db == th && idx <= __float_as_uint(B.w));
-              const float cos_i = dot_ref(w.x, w.y, w.z, B.x, B.y, B.z);
-              const float sel = (nb && !(cos_i < 0.0f)) ? 1.0f : 0.0f;
-              a_r = fmaf(sel, C.x, a_r);
-              a_g = fmaf(sel, C.y, a_g);
-              a_b = fmaf(sel, C.z, a_b);
+            // the photon's sign under the shared normal, when it settles it
+            bool per_query = true, inc = false;
+            if (shared) {
+              const float d0 = dot_ref(w.x, w.y, w.z, sn.x, sn.y, sn.z);
+              const float mw = (fabsf(w.x) + fabsf(w.y) + fabsf(w.z)) * sn.m + 1e-30f;
+              per_query = !(fabsf(d0) > mw);
+              inc = !per_query && d0 > 0.0f;
             }
+            float a_r = 0.f, a_g = 0.f, a_b = 0.f;
+            auto queries = [&](auto test_tag) {
+              constexpr bool kTest = decltype(test_tag)::value;
+              uint64_t mm = cm;
+              while (mm != 0ull) {
+                const int i = __builtin_ctzll(mm);
+                mm &= mm - 1ull;
+                const float4 A = qa[i];
+                const float4 B = qb[i];
+                const float4 C = qc[i];
+                const float d2 = d2_ref(A.x, A.y, A.z, p.x, p.y, p.z);
+                const uint32_t db = __float_as_uint(d2);
+                const uint32_t th = __float_as_uint(A.w);
+                const bool nb = db < th || (db == th && idx <= __float_as_uint(B.w));
+                bool on = nb && inc;
+                if (kTest && per_query) on = nb && !(dot_ref(w.x, w.y, w.z, B.x, B.y, B.z) < 0.0f);
+                const float sel = on ? 1.0f : 0.0f;
+                a_r = fmaf(sel, C.x, a_r);
+                a_g = fmaf(sel, C.y, a_g);
+                a_b = fmaf(sel, C.z, a_b);
+              }
+            };
+            if (__ballot(ok && per_query) != 0ull) queries(std::true_type{});
+            else if (__ballot(ok && inc) != 0ull) queries(std::false_type{});
             if (ok && (a_r != 0.0f || a_g != 0.0f || a_b != 0.0f)) {
               atomicAdd(&dflux[idx], a_r);
               atomicAdd(&dflux[n + idx], a_g);

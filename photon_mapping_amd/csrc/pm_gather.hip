@@ -886,9 +886,12 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
   uint32_t nr = 0;
   bool fine = false;
   if (r0c < INFINITY) {
-    // The list of the centre sweep also covers the run's final sphere
-    // (r_c <= r0c up to the margins), which is then a filter of it.
-    const float rl = (r0c * 1.0001f + 2.0f * dmax) * 1.0004f + 2e-30f;
+    // The list of the centre sweep also covers the run's final sphere, which
+    // is then a filter of it: that sphere reaches no farther than r_c + 2
+    // dmax (r_c <= r0c up to the margins), nor than the largest lane bound
+    // plus its |q - c| (each lane's upper end is at most its own bound).
+    const float xmax = wave_max(part ? xq : 0.0f);
+    const float rl = fminf((r0c * 1.0001f + 2.0f * dmax) * 1.0004f, xmax * 1.0005f) + 2e-30f;
     nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, true, &fine);
     if (count_stats && lane == 0) atomicAdd(&counters[kStatCand1], *total);
     float t_lo = 0.0f, t_hi = INFINITY;
@@ -905,11 +908,15 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
   // fewer than k lie within rc_lo of c, so fewer than k within
   // rc_lo - |q - c| of q.  Every neighbour of every lane lies within
   // rc_hi + 2 dmax of c.
+  // The lane's own radius bound r0sq (from the anchors) may be the smaller
+  // upper end; the run's sphere then only has to reach every lane's.
   const float rq = (rc_hi + dq) * 1.00002f;
   float hi2 = rq * rq * 1.0001f + 1e-30f;
+  if (part) hi2 = fminf(hi2, r0sq * 1.0001f + 1e-30f);
   const float lo = fmaxf(rc_lo - dq, 0.0f) * 0.99999f;
   float lo2 = part ? lo * lo * 0.99999f : 0.0f;
-  const float rs = (rc_hi + 2.0f * dmax) * 1.0002f;
+  const float xs = part ? sqrt_up(hi2) * 1.00002f + dq : 0.0f;
+  const float rs = fminf((rc_hi + 2.0f * dmax) * 1.0002f, wave_max(xs) * 1.0002f);
   float rs2 = rs * rs + 1e-30f;
 
   st.klo = 0; st.khi = 0; st.need = 0; st.cnt = 0;

@@ -19,7 +19,14 @@ constexpr int kBins = 32;           // selection histogram bins per query
 #endif
 constexpr int kStashCap = PM_STASH_CAP;  // per-query candidates kept past the cut
 constexpr int kGridBits = 10;       // space-filling-curve grid bits per axis
-constexpr int kAnchorWindow = 4;    // anchors tried on each side of a group
+#ifndef PM_ANCHOR_WINDOW
+#define PM_ANCHOR_WINDOW 4
+#endif
+#ifndef PM_ANCHOR_STRIDE
+#define PM_ANCHOR_STRIDE 4
+#endif
+constexpr int kAnchorWindow = PM_ANCHOR_WINDOW;  // anchors tried on each side of a query's
+constexpr int kAnchorStride = PM_ANCHOR_STRIDE;  // default photons per radius anchor
 constexpr int kAnchorTabBits = 16;  // code prefix bits of the anchor lookup table
 constexpr int kMaxKnnK = 1024;      // row-sort limit of pm_knn
 constexpr int kXcds = 8;            // MI355X accelerator complex dies
@@ -152,7 +159,7 @@ struct PhotonMapImpl {
   int leaf_size = 8;
   int n_leaves = 0;
   int first_leaf = 1;
-  int anchor_stride = 8;
+  int anchor_stride = kAnchorStride;
   int n_anchors = 0;
   // radius bounds of the anchors, one set per k (least recently used
   // replaced); each holds the centroids with w = the k-NN radius bound
@@ -177,7 +184,7 @@ struct Context {
   int device = 0;
   int n_cu = 256;
   int waves_per_cu = 12;
-  int anchor_stride = 8;
+  int anchor_stride = kAnchorStride;
   int leaf_size = 8;
   int xcd_aware = 1;
   float run_split = 0.75f;

@@ -909,14 +909,19 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
   // rc_lo - |q - c| of q.  Every neighbour of every lane lies within
   // rc_hi + 2 dmax of c.
   // The lane's own radius bound r0sq (from the anchors) may be the smaller
-  // upper end; the run's sphere then only has to reach every lane's.
+  // upper end; the run's sphere then only has to reach every lane's.  Only
+  // with a centre bracket: valid lane bounds always give one, and without it
+  // the run falls back to unbounded brackets, so that the neighbour sets
+  // never depend on the lane bounds being right.
+  const bool capped = rc_hi < INFINITY;
   const float rq = (rc_hi + dq) * 1.00002f;
   float hi2 = rq * rq * 1.0001f + 1e-30f;
-  if (part) hi2 = fminf(hi2, r0sq * 1.0001f + 1e-30f);
+  if (part && capped) hi2 = fminf(hi2, r0sq * 1.0001f + 1e-30f);
   const float lo = fmaxf(rc_lo - dq, 0.0f) * 0.99999f;
   float lo2 = part ? lo * lo * 0.99999f : 0.0f;
   const float xs = part ? sqrt_up(hi2) * 1.00002f + dq : 0.0f;
-  const float rs = fminf((rc_hi + 2.0f * dmax) * 1.0002f, wave_max(xs) * 1.0002f);
+  const float xsm = wave_max(xs);
+  const float rs = capped ? fminf((rc_hi + 2.0f * dmax) * 1.0002f, xsm * 1.0002f) : INFINITY;
   float rs2 = rs * rs + 1e-30f;
 
   st.klo = 0; st.khi = 0; st.need = 0; st.cnt = 0;

@@ -43,6 +43,28 @@ def test_knn_bruteforce_parity(gpu, oracle, rng, gen, k):
     np.testing.assert_array_equal(idx, kidx)
 
 
+@pytest.mark.parametrize("k", [1, 100, 500])
+def test_knn_dense_cluster_beside_sparse_region(gpu, oracle, rng, k):
+    """A dense cluster inside a sparse cloud: the k-NN spheres of sparse
+    photons near the cluster reach into it (their radius bounds must come
+    from small spheres, not from sweeping the cluster), and queries sit in
+    the cluster, around it and far from it."""
+    from util import unit
+
+    dense = (rng.random((27000, 3)) * 0.02 + 0.49).astype(np.float32)
+    sparse = rng.random((3000, 3)).astype(np.float32)
+    pos = np.concatenate([dense, sparse])
+    wi = unit(rng.normal(size=pos.shape))
+    flux = rng.uniform(0.1, 1.0, pos.shape).astype(np.float32)
+    q = np.concatenate([queries_for(rng, pos, 1500),
+                        rng.random((1000, 3)).astype(np.float32)])
+    print(f"knn dense cluster k={k}", flush=True)
+    idx, r2 = _map(gpu, pos, wi, flux).knn(q, k)
+    bidx, br2 = oracle.brute_knn(pos, q, k)
+    np.testing.assert_array_equal(idx, bidx)
+    np.testing.assert_array_equal(r2.view(np.uint32), br2.view(np.uint32))
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 31, 63, 64, 65, 100, 101, 257])
 @pytest.mark.parametrize("k", [1, 64, 100, 130])
 def test_knn_small_maps_and_k_over_n(gpu, oracle, rng, n, k):

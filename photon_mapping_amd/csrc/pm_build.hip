@@ -29,25 +29,29 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ __forceinline__ uint32_t expand10(uint32_t v) {
-  v &= 1023u;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
+// Spreads the low 21 bits of v to every third bit (bit i to bit 3i).
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x1fffffu;
+  x = (x | (x << 32)) & 0x001f00000000ffffull;
+  x = (x | (x << 16)) & 0x001f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
 }
 
 __device__ __forceinline__ uint32_t quantize(float v, float lo, float scale) {
   const float t = (v - lo) * scale;
   if (!(t > 0.0f)) return 0u;
-  if (t >= 1023.0f) return 1023u;
+  constexpr uint32_t kTop = (1u << kGridBits) - 1u;
+  if (t >= static_cast<float>(kTop)) return kTop;
   return static_cast<uint32_t>(t);
 }
 
-// Hilbert index of the 10-bit quantized position (Skilling 2004, "Programming
-// the Hilbert curve", AIP Conf. Proc. 707: axes to transposed index, then the
-// transpose's bits interleaved).  Consecutive indices are always
+// Hilbert index of the kGridBits-bit quantized position (Skilling 2004,
+// "Programming the Hilbert curve", AIP Conf. Proc. 707: axes to transposed
+// index, then the transpose's bits interleaved), cut to its top kCodeBits
+// bits.  Consecutive indices are always
 // face-adjacent cells, so runs of sorted photons or queries stay compact
 // where a Morton (Z) order jumps.
 __device__ __forceinline__ uint32_t hilbert3(float x, float y, float z, const float* lo,
@@ -76,7 +80,8 @@ __device__ __forceinline__ uint32_t hilbert3(float x, float y, float z, const fl
   X[0] ^= t;
   X[1] ^= t;
   X[2] ^= t;
-  return (expand10(X[0]) << 2) | (expand10(X[1]) << 1) | expand10(X[2]);
+  const uint64_t h = (spread3(X[0]) << 2) | (spread3(X[1]) << 1) | spread3(X[2]);
+  return static_cast<uint32_t>(h >> (3 * kGridBits - kCodeBits));
 }
 
 // Per-block min/max of positions plus a count of non-finite coordinates.
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(kThreads) void anchor_tab_kernel(
     const uint32_t* __restrict__ anchor_code, int n_anchors, uint32_t* __restrict__ tab) {
   const int p = blockIdx.x * kThreads + threadIdx.x;
   if (p > (1 << kAnchorTabBits)) return;
-  const uint64_t start = static_cast<uint64_t>(p) << (3 * kGridBits - kAnchorTabBits);
+  const uint64_t start = static_cast<uint64_t>(p) << (kCodeBits - kAnchorTabBits);
   int lo = 0, hi = n_anchors;
   while (lo < hi) {
     const int m = (lo + hi) >> 1;
@@ -291,7 +296,7 @@ __global__ __launch_bounds__(kThreads) void query_anchor_kernel(
   // anchors before the bucket have smaller codes, those after it larger.
   int lo = 0, hi = 0;
   if (D.n_anchors > 0) {
-    const uint32_t pre = code >> (3 * kGridBits - kAnchorTabBits);
+    const uint32_t pre = code >> (kCodeBits - kAnchorTabBits);
     lo = static_cast<int>(D.anchor_tab[pre]);
     hi = static_cast<int>(D.anchor_tab[pre + 1]);
   }
@@ -445,7 +450,7 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
   for (int c = 0; c < 3; ++c) {
     const float ext = hb[3 + c] - hb[c];
     map->qlo[c] = hb[c];
-    map->qscale[c] = ext > 0.0f ? 1024.0f / ext : 1.0f;
+    map->qscale[c] = ext > 0.0f ? static_cast<float>(1u << kGridBits) / ext : 1.0f;
   }
 
   // 2-3. Hilbert codes, stable sort
@@ -463,7 +468,7 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
                      pos, n, qz, k0, v0);
   PM_HIP(hipGetLastError());
   int which = 0;
-  if (int rc = radix_sort_u32(&ctx->ws, k0, v0, k1, v1, n, 3 * kGridBits, s, &which)) return rc;
+  if (int rc = radix_sort_u32(&ctx->ws, k0, v0, k1, v1, n, kCodeBits, s, &which)) return rc;
   const uint32_t* sorted_codes = which ? k1 : k0;
   const uint32_t* order = which ? v1 : v0;
 
@@ -609,7 +614,8 @@ int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
     hipLaunchKernelGGL(query_keys_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
                        dev_q_prefix, n_inst, total_q, k0, v0);
     PM_HIP(hipGetLastError());
-    if (int rc = radix_sort_u32(ws, k0, v0, k1, v1, total_q, 3 * kGridBits, s, &which))
+    if (int rc = radix_sort_u32(ws, k0, v0, k1, v1, total_q, kCodeBits, s, &which,
+                                kQuerySortLoBit))
       return rc;
     hipLaunchKernelGGL(query_anchor_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
                        dev_q_prefix, n_inst, total_q, which ? k1 : k0,
@@ -622,7 +628,9 @@ int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
     hipLaunchKernelGGL(query_keys_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
                        dev_q_prefix, n_inst, total_q, k0, v0);
     PM_HIP(hipGetLastError());
-    if (int rc = radix_sort_u64(ws, k0, v0, k1, v1, total_q, 32 + ib, s, &which)) return rc;
+    if (int rc = radix_sort_u64(ws, k0, v0, k1, v1, total_q, 32 + ib, s, &which,
+                                kQuerySortLoBit))
+      return rc;
     hipLaunchKernelGGL(query_anchor_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
                        dev_q_prefix, n_inst, total_q, which ? k1 : k0,
                        ws->q_anchor.as<uint32_t>());

@@ -18,7 +18,17 @@ constexpr int kBins = 32;           // selection histogram bins per query
 #define PM_STASH_CAP 11
 #endif
 constexpr int kStashCap = PM_STASH_CAP;  // per-query candidates kept past the cut
-constexpr int kGridBits = 10;       // space-filling-curve grid bits per axis
+#ifndef PM_GRID_BITS
+#define PM_GRID_BITS 10
+#endif
+constexpr int kGridBits = PM_GRID_BITS;  // space-filling-curve grid bits per axis
+// bits of a curve code: the top 32 bits of the 3 * kGridBits-bit index
+constexpr int kCodeBits = 3 * kGridBits < 32 ? 3 * kGridBits : 32;
+static_assert(kGridBits >= 6 && kGridBits <= 21, "curve grid bits");
+#ifndef PM_QUERY_SORT_LO_BIT
+#define PM_QUERY_SORT_LO_BIT 0
+#endif
+constexpr int kQuerySortLoBit = PM_QUERY_SORT_LO_BIT;  // query keys sort from this bit up
 #ifndef PM_ANCHOR_WINDOW
 #define PM_ANCHOR_WINDOW 4
 #endif
@@ -213,16 +223,17 @@ void* stage_put(Context* ctx, const void* src, size_t bytes);  // returns device
 int stage_commit(Context* ctx, hipStream_t s);
 
 // ---- launchers (defined in the .hip files) -----------------------------------------
-// Radix sort of (key, value) pairs on the low `bits` bits of the key.
+// Stable radix sort of (key, value) pairs on bits [lo_bit, bits) of the key
+// (8-bit digits from lo_bit up; bits below lo_bit keep their input order).
 // Returns the index (0 or 1) of the buffer pair that holds the result.
 int radix_sort_u32(Workspace* ws, uint32_t* keys, uint32_t* vals,
                    uint32_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
-                   hipStream_t s, int* which);
+                   hipStream_t s, int* which, int lo_bit = 0);
 // Reserves the sort histograms and scan partials for n keys.
 int sort_reserve(Workspace* ws, int64_t n);
 int radix_sort_u64(Workspace* ws, uint64_t* keys, uint32_t* vals,
                    uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
-                   hipStream_t s, int* which);
+                   hipStream_t s, int* which, int lo_bit = 0);
 
 int build_photon_map(PhotonMapImpl* map, hipStream_t s);
 int update_flux(PhotonMapImpl* map, const float* flux_dev, hipStream_t s);

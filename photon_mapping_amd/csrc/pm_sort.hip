@@ -236,7 +236,7 @@ int exclusive_scan(Workspace* ws, uint32_t* data, int64_t n, hipStream_t s) {
 template <typename K>
 int radix_sort_impl(Workspace* ws, K* keys, uint32_t* vals, K* keys_alt,
                     uint32_t* vals_alt, int64_t n, int bits, hipStream_t s,
-                    int* which) {
+                    int* which, int lo_bit) {
   *which = 0;
   if (n <= 1 || bits <= 0) return PM_OK;
   const int64_t n_tiles = (n + kSortTile - 1) / kSortTile;
@@ -246,7 +246,7 @@ int radix_sort_impl(Workspace* ws, K* keys, uint32_t* vals, K* keys_alt,
   K* kb[2] = {keys, keys_alt};
   uint32_t* vb[2] = {vals, vals_alt};
   int cur = 0;
-  for (int shift = 0; shift < bits; shift += 8) {
+  for (int shift = lo_bit; shift < bits; shift += 8) {
     hipLaunchKernelGGL(digit_hist_kernel<K>, dim3(n_tiles), dim3(kSortThreads), 0, s,
                        kb[cur], n, shift, counts, n_tiles);
     if (int rc = exclusive_scan(ws, counts, n_tiles * 256, s)) return rc;
@@ -271,13 +271,17 @@ int sort_reserve(Workspace* ws, int64_t n) {
 }
 
 int radix_sort_u32(Workspace* ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
-                   uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which) {
-  return radix_sort_impl<uint32_t>(ws, keys, vals, keys_alt, vals_alt, n, bits, s, which);
+                   uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which,
+                   int lo_bit) {
+  return radix_sort_impl<uint32_t>(ws, keys, vals, keys_alt, vals_alt, n, bits, s, which,
+                                   lo_bit);
 }
 
 int radix_sort_u64(Workspace* ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
-                   uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which) {
-  return radix_sort_impl<uint64_t>(ws, keys, vals, keys_alt, vals_alt, n, bits, s, which);
+                   uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which,
+                   int lo_bit) {
+  return radix_sort_impl<uint64_t>(ws, keys, vals, keys_alt, vals_alt, n, bits, s, which,
+                                   lo_bit);
 }
 
 }  // namespace pm

@@ -63,11 +63,17 @@ namespace {
 constexpr float kPiF = 3.14159265359f;  // include/core.h:7
 constexpr uint64_t kKeyMax = ~0ull;
 constexpr int kChunk = 64;        // candidate photons staged per LDS chunk
-constexpr int kMaxRanges = 256;   // candidate subtrees per list
+#ifndef PM_MAX_RANGES
+#define PM_MAX_RANGES 192
+#endif
+#ifndef PM_SLOT_CAP
+#define PM_SLOT_CAP 384
+#endif
+constexpr int kMaxRanges = PM_MAX_RANGES;  // candidate subtrees per list
 constexpr int kFront = 256;       // breadth-first frontier capacity
 constexpr int kDrhoSlots = 64;    // adjoint: materials accumulated in LDS
 constexpr int kCentreBins = 256;  // centre histogram bins
-constexpr int kSlotCap = 512;     // photons of an explicit slot list
+constexpr int kSlotCap = PM_SLOT_CAP;  // photons of an explicit slot list
 // A candidate list is either nr subtrees (sh.rl.rnode / sh.rl.rpre) or, when nr
 // is this value, an explicit list of photon slots (sh.cslot).
 constexpr uint32_t kSlotList = 0xffffffffu;

@@ -13,7 +13,10 @@ namespace pm {
 
 // ---- tuning constants ------------------------------------------------------
 constexpr int kWave = 64;           // CDNA wavefront width
-constexpr int kBins = 32;           // selection histogram bins per query
+#ifndef PM_BINS
+#define PM_BINS 32
+#endif
+constexpr int kBins = PM_BINS;      // selection histogram bins per query
 #ifndef PM_STASH_CAP
 #define PM_STASH_CAP 11
 #endif

@@ -1064,6 +1064,21 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
   bool lambert = false;
   int32_t mat = -1;
   float rho_r = 0.f, rho_g = 0.f, rho_b = 0.f;
+  const bool reuse = (MODE == kModeAdjoint) && D.key_in != nullptr;
+  const bool all = D.all_members != 0;
+  // The anchors around the lane's query depend only on its sorted position,
+  // so they are loaded before its record: both chains of loads are then in
+  // flight together.
+  constexpr int kAn = 2 * kAnchorWindow + 1;
+  float4 anq[kAn];
+  if (!reuse && !all) {
+    const float4* __restrict__ A = D.anchors;
+    const int32_t center = static_cast<int32_t>(D.q_anchor[active ? qloc : lg * kWave]);
+    const int32_t a0 = max(center - kAnchorWindow, 0);
+    const int32_t a1 = min(center + kAnchorWindow, D.n_anchors - 1);
+#pragma unroll
+    for (int o = 0; o < kAn; ++o) anq[o] = A[min(max(center + o - kAnchorWindow, a0), a1)];
+  }
   {
     const int64_t sp = D.q_base + (active ? qloc : lg * kWave);
     qi = static_cast<int64_t>(perm[sp]) - D.q_base;
@@ -1086,8 +1101,6 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
   }
 
   // ---- radius bound: r0sq >= the lane's k-th FP32 d^2 --------------------------
-  const bool reuse = (MODE == kModeAdjoint) && D.key_in != nullptr;
-  const bool all = D.all_members != 0;
   uint64_t tau_in = 0;
   float r0sq = -1.0f;  // inactive lanes never match anything
   if (reuse) {
@@ -1100,15 +1113,10 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
     // Triangle inequality on precomputed anchors: the k nearest photons of
     // anchor a lie within r_a of its centroid c_a, hence within
     // r_a + |q - c_a| of q.  The anchor radii are already rounded upward.
-    const float4* __restrict__ A = D.anchors;
-    const int32_t center = static_cast<int32_t>(D.q_anchor[active ? qloc : lg * kWave]);
-    const int32_t a0 = max(center - kAnchorWindow, 0);
-    const int32_t a1 = min(center + kAnchorWindow, D.n_anchors - 1);
     float best = INFINITY;
 #pragma unroll
-    for (int32_t o = -kAnchorWindow; o <= kAnchorWindow; ++o) {
-      const int32_t a = min(max(center + o, a0), a1);
-      const float4 an = A[a];
+    for (int o = 0; o < kAn; ++o) {
+      const float4 an = anq[o];
       const float dx = qx - an.x, dy = qy - an.y, dz = qz - an.z;
       best = fminf(best, an.w + sqrt_up(dx * dx + dy * dy + dz * dz));
     }

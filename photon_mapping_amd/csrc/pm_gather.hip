@@ -70,6 +70,10 @@ constexpr int kChunk = 64;        // candidate photons staged per LDS chunk
 #define PM_SLOT_CAP 384
 #endif
 constexpr int kMaxRanges = PM_MAX_RANGES;  // candidate subtrees per list
+// deepest level whose nodes always fit a list: 2^kCapDepth <= kMaxRanges
+constexpr int kCapDepth = kMaxRanges >= 256 ? 8 : kMaxRanges >= 128 ? 7 : kMaxRanges >= 64 ? 6 : 5;
+static_assert(kMaxRanges >= 32, "a depth-capped walk must fit the list");
+static_assert((1 << kCapDepth) <= kMaxRanges, "depth cap");
 constexpr int kFront = 256;       // breadth-first frontier capacity
 constexpr int kDrhoSlots = 64;    // adjoint: materials accumulated in LDS
 constexpr int kCentreBins = 256;  // centre histogram bins
@@ -369,8 +373,9 @@ __device__ uint32_t list_prefix(DescRef D, GatherShared& sh, int lane, uint32_t 
 // sh.rl.rpre; returns the number of subtrees, the total slot count in *total.
 // When the list would overflow, the walk restarts with whole inner nodes
 // allowed, then with a depth cap above the level that overflowed (at most
-// 2^8 subtrees, so it fits); a frontier that would overflow emits its nodes
-// whole.  *fine (when given) tells whether every entry is a single leaf
+// kCapDepth levels: 2^kCapDepth <= kMaxRanges subtrees, so it fits); a
+// frontier that would overflow emits its nodes whole, and a walk that still
+// overflows lists the root (never a truncated list).  *fine (when given) tells whether every entry is a single leaf
 // (asked with leaf_only and no coarsening was needed); coarsenings are
 // counted in *stat when given.
 __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2,
@@ -443,11 +448,13 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
     }
     if (!full) break;
     if (stat && lane == 0) atomicAdd(stat, 1000u);
-    if (leaf_only) {
+    if (attempt == 2) {
+      whole = true;  // cannot happen with the depth cap; never return a cut list
+    } else if (leaf_only) {
       leaf_only = false;
     } else {
       coarse = true;
-      max_depth = min(full_depth - 1, 8);
+      max_depth = min(full_depth - 1, kCapDepth);
       if (max_depth < 1) whole = true;
     }
   }

@@ -65,6 +65,20 @@ def test_knn_dense_cluster_beside_sparse_region(gpu, oracle, rng, k):
     np.testing.assert_array_equal(r2.view(np.uint32), br2.view(np.uint32))
 
 
+@pytest.mark.parametrize("k", [600, 1000])
+def test_knn_spheres_covering_most_of_a_small_map(gpu, oracle, rng, k):
+    """k close to n on a 256-leaf map: candidate spheres cover most leaves,
+    so leaf-only list walks overflow and the restarted walks (whole inner
+    nodes, then a depth cap) must still list every photon."""
+    pos, wi, flux = photons_cube(rng, 2048)
+    q = np.concatenate([queries_for(rng, pos, 300), rng.random((200, 3)).astype(np.float32)])
+    print(f"knn covering spheres k={k}", flush=True)
+    idx, r2 = _map(gpu, pos, wi, flux, leaf=8).knn(q, k)
+    bidx, br2 = oracle.brute_knn(pos, q, k)
+    np.testing.assert_array_equal(idx, bidx)
+    np.testing.assert_array_equal(r2.view(np.uint32), br2.view(np.uint32))
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 31, 63, 64, 65, 100, 101, 257])
 @pytest.mark.parametrize("k", [1, 64, 100, 130])
 def test_knn_small_maps_and_k_over_n(gpu, oracle, rng, n, k):

@@ -12,14 +12,16 @@
 //   centre  a wave-parallel histogram of d^2(c, p) over the photons of a
 //           guaranteed bound gives r_c, a radius holding at least k photons
 //           around c.  By the triangle inequality the k nearest photons of
-//           every lane q lie within R_q = r_c + |q - c| of q, and all of them
-//           within r_c + 2 delta of c: that sphere's subtrees are the run's
-//           candidate list (a few hundred photons at leaf size 8);
+//           every lane q lie within R_q = min(r_c + |q - c|, r0_q) of q (r0_q
+//           the lane's own anchor bound), and all of them within the largest
+//           R_q + |q - c| of c: that sphere's photons are the run's candidate
+//           list (a few hundred at leaf size 8);
 //   pass A  per lane, a histogram of d^2 over [0, R_q^2] in 32 linear bins
-//           (LDS) finds the bin holding the k-th key;
-//   pass B  photons below that bin are neighbours and are consumed on the
-//           spot (radiance sum, index write, ...); photons inside the bin are
-//           stashed in registers (a handful), and a rank selection inside the
+//           (LDS) finds the bin holding the k-th key, i.e. a key window;
+//           the list then shrinks to the sphere that holds every window;
+//   pass B  photons below the window are neighbours and are consumed on the
+//           spot (radiance sum, index write, ...); photons inside it are
+//           stashed in LDS (a handful), and a rank selection inside the
 //           stash finishes the set.
 // The neighbour set is {key <= tau}; tau's d^2 is r^2.  Bins that hold more
 // than kStashCap keys are split further (rare) until they fit.

@@ -79,6 +79,34 @@ def test_knn_spheres_covering_most_of_a_small_map(gpu, oracle, rng, k):
     np.testing.assert_array_equal(r2.view(np.uint32), br2.view(np.uint32))
 
 
+@pytest.mark.parametrize("stride", [1, 3, 64, 4096])
+def test_knn_any_anchor_stride(gpu, oracle, rng, stride):
+    """Radius anchors only bound the search: neighbour sets stay exact for
+    any photons-per-anchor setting, including one anchor per photon and a
+    single anchor for the whole map, at k = 1 and k = 100."""
+    from photon_mapping_amd import _lib
+
+    ctx = gpu.Context(0)
+    try:
+        with pytest.raises(_lib.PmError):
+            ctx.set_option(_lib.PM_OPT_ANCHOR_STRIDE, 4097)
+        ctx.set_option(_lib.PM_OPT_ANCHOR_STRIDE, stride)
+        pos, wi, flux, _ = photons_surfaces(rng, 4000)
+        q = queries_for(rng, pos, 600)
+        pmap = gpu.PhotonMap(ctx)
+        pmap.set_photons_soa(pos, wi, flux)
+        pmap.build()
+        print(f"knn anchor stride {stride}", flush=True)
+        for k in (1, 100):
+            idx, r2 = pmap.knn(q, k)
+            bidx, br2 = oracle.brute_knn(pos, q, k)
+            np.testing.assert_array_equal(idx, bidx)
+            np.testing.assert_array_equal(r2.view(np.uint32), br2.view(np.uint32))
+        del pmap
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 31, 63, 64, 65, 100, 101, 257])
 @pytest.mark.parametrize("k", [1, 64, 100, 130])
 def test_knn_small_maps_and_k_over_n(gpu, oracle, rng, n, k):

@@ -327,7 +327,10 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
               share[0] = 1.0f;  // total internal reflection
             }
           }
-          for (int b = 0; b < nb; ++b) {
+          // (both slots unrolled, so dirs and share stay in registers)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            if (b >= nb) break;
             const V3 dl = dirs[b];
             const V3 dir = ft * dl.x + ns * dl.y + fb * dl.z;
             const float c = share[b] / fabsf(dl.y) * fabsf(dot(dir, ns));

@@ -209,7 +209,7 @@ struct ShadeOut {
 __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
                                                          const Segment* __restrict__ in,
                                                          uint32_t n_in, ShadeOut O) {
-  __shared__ int stk_all[kBvhStack * kThreads];
+  extern __shared__ int stk_all[];  // SceneImpl::stack_bytes
   int* stk = stk_all + threadIdx.x;
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n_in) return;
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(kThreads) void path_trace_kernel(RenderDev R, pmx_c
                                                               unsigned long long* __restrict__ acc,
                                                               uint32_t* __restrict__ bad,
                                                               unsigned long long* rays) {
-  __shared__ int stk_all[kBvhStack * kThreads];
+  extern __shared__ int stk_all[];  // SceneImpl::stack_bytes
   int* stk = stk_all + threadIdx.x;
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
@@ -646,7 +646,8 @@ int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
       ShadeOut O{nxt, c32 + 0, 2 * n_cur, qg.as<pm_query>(), wg.as<Weighted>(), c32 + 1,
                  qc.as<pm_query>(), wc.as<Weighted>(), c32 + 2, n_cur,
                  acc.as<unsigned long long>(), bad.as<uint32_t>(), c32 + 3, rays};
-      hipLaunchKernelGGL(shade_kernel, dim3(grid_of(n_cur)), dim3(kThreads), 0, nullptr, R, cur,
+      hipLaunchKernelGGL(shade_kernel, dim3(grid_of(n_cur)), dim3(kThreads), s->stack_bytes(0),
+                         nullptr, R, cur,
                          n_cur, O);
       PM_HIP(hipGetLastError());
       uint32_t cnt[4];
@@ -732,7 +733,8 @@ int pmx_render_path_tracing(const pmx_scene* scene, const pmx_camera* cam,
     const int64_t np = std::min(bp, n_pix - pix0);
     const uint32_t ns = static_cast<uint32_t>(np * p->spp);
     if (int e = zero_words(bad.ptr, static_cast<size_t>(ns) * sizeof(uint32_t))) return e;
-    hipLaunchKernelGGL(path_trace_kernel, dim3(grid_of(ns)), dim3(kThreads), 0, nullptr, R, *cam,
+    hipLaunchKernelGGL(path_trace_kernel, dim3(grid_of(ns)), dim3(kThreads), s->stack_bytes(0),
+                       nullptr, R, *cam,
                        pix0 * p->spp, ns, acc.as<unsigned long long>(), bad.as<uint32_t>(), rays);
     PM_HIP(hipGetLastError());
     hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(np)), dim3(kThreads), 0, nullptr,

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kThreads) void intersect_kernel(
     SceneDev S, int64_t n, const float* __restrict__ org, const float* __restrict__ dir,
     int32_t* __restrict__ out_tri, float* __restrict__ out_t, float* __restrict__ out_u,
     float* __restrict__ out_v) {
-  __shared__ int stk[kBvhStack * kThreads];
+  extern __shared__ int stk[];  // SceneImpl::stack_bytes
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= n) return;
   const V3 o = v3(org[i], org[n + i], org[2 * n + i]);
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kThreads) void intersect_kernel(
 __global__ __launch_bounds__(kThreads) void fg_queries_kernel(
     SceneDev S, pmx_camera cam, int width, int height, int spp, int depth, uint64_t seed,
     pm_query* __restrict__ out, int32_t* __restrict__ fg_mat) {
-  __shared__ int stk_all[kBvhStack * kThreads];
+  extern __shared__ int stk_all[];  // SceneImpl::stack_bytes
   int* stk = stk_all + threadIdx.x;
   const int64_t id = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t total = static_cast<int64_t>(width) * height * spp;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kThreads) void trace_kernel(
     SceneDev S, int64_t path0, int64_t n_paths, uint64_t seed, int max_depth,
     uint32_t* __restrict__ counts, const int64_t* __restrict__ offsets, int64_t n_store,
     float* __restrict__ pos, float* __restrict__ wi, float* __restrict__ flux) {
-  __shared__ int stk_all[kBvhStack * kThreads];
+  extern __shared__ int stk_all[];  // SceneImpl::stack_bytes
   int* stk = stk_all + threadIdx.x;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= n_paths) return;
@@ -247,10 +247,10 @@ int launch_trace_count(const SceneImpl* s, int kind, int64_t path0, int64_t n, u
   const dim3 grid(static_cast<unsigned>((n + kThreads - 1) / kThreads)), block(kThreads);
   const SceneDev S = s->dev(0);
   if (kind == PMX_PHOTONS_CAUSTICS)
-    hipLaunchKernelGGL((trace_kernel<true, true>), grid, block, 0, nullptr, S, path0, n, seed,
+    hipLaunchKernelGGL((trace_kernel<true, true>), grid, block, s->stack_bytes(0), nullptr, S, path0, n, seed,
                        max_depth, counts, nullptr, int64_t{0}, nullptr, nullptr, nullptr);
   else
-    hipLaunchKernelGGL((trace_kernel<true, false>), grid, block, 0, nullptr, S, path0, n, seed,
+    hipLaunchKernelGGL((trace_kernel<true, false>), grid, block, s->stack_bytes(0), nullptr, S, path0, n, seed,
                        max_depth, counts, nullptr, int64_t{0}, nullptr, nullptr, nullptr);
   PM_HIP(hipGetLastError());
   return PM_OK;
@@ -262,10 +262,10 @@ int launch_trace_write(const SceneImpl* s, int kind, int64_t n_paths, uint64_t s
   const dim3 grid(static_cast<unsigned>((n_paths + kThreads - 1) / kThreads)), block(kThreads);
   const SceneDev S = s->dev(0);
   if (kind == PMX_PHOTONS_CAUSTICS)
-    hipLaunchKernelGGL((trace_kernel<false, true>), grid, block, 0, nullptr, S, int64_t{0},
+    hipLaunchKernelGGL((trace_kernel<false, true>), grid, block, s->stack_bytes(0), nullptr, S, int64_t{0},
                        n_paths, seed, max_depth, nullptr, offsets, n_store, pos, wi, flux);
   else
-    hipLaunchKernelGGL((trace_kernel<false, false>), grid, block, 0, nullptr, S, int64_t{0},
+    hipLaunchKernelGGL((trace_kernel<false, false>), grid, block, s->stack_bytes(0), nullptr, S, int64_t{0},
                        n_paths, seed, max_depth, nullptr, offsets, n_store, pos, wi, flux);
   PM_HIP(hipGetLastError());
   return PM_OK;
@@ -432,7 +432,7 @@ int pmx_scene_intersect(const pmx_scene* scene, int64_t n, const float* org_soa,
   const auto hs = static_cast<hipStream_t>(stream);
   const dim3 grid(static_cast<unsigned>((n + kThreads - 1) / kThreads));
   if (mem == PM_MEM_DEVICE) {
-    hipLaunchKernelGGL(intersect_kernel, grid, dim3(kThreads), 0, hs, s->dev(method), n,
+    hipLaunchKernelGGL(intersect_kernel, grid, dim3(kThreads), s->stack_bytes(method), hs, s->dev(method), n,
                        org_soa, dir_soa, out_tri, out_t, out_u, out_v);
     PM_HIP(hipGetLastError());
     return PM_OK;
@@ -450,7 +450,7 @@ int pmx_scene_intersect(const pmx_scene* scene, int64_t n, const float* org_soa,
   if (rc) return rc;
   PM_HIP(hipMemcpyAsync(d_org.ptr, org_soa, v3, hipMemcpyHostToDevice, hs));
   PM_HIP(hipMemcpyAsync(d_dir.ptr, dir_soa, v3, hipMemcpyHostToDevice, hs));
-  hipLaunchKernelGGL(intersect_kernel, grid, dim3(kThreads), 0, hs, s->dev(method), n,
+  hipLaunchKernelGGL(intersect_kernel, grid, dim3(kThreads), s->stack_bytes(method), hs, s->dev(method), n,
                      d_org.as<float>(), d_dir.as<float>(), d_tri.as<int32_t>(), d_t.as<float>(),
                      out_u ? d_u.as<float>() : nullptr, out_v ? d_v.as<float>() : nullptr);
   PM_HIP(hipGetLastError());
@@ -471,7 +471,7 @@ int pmx_fg_queries(const pmx_scene* scene, const pmx_camera* cam, int width, int
   PM_HIP(hipSetDevice(s->ctx->device));
   const int64_t total = static_cast<int64_t>(width) * height * spp;
   hipLaunchKernelGGL(fg_queries_kernel, dim3(static_cast<unsigned>((total + kThreads - 1) / kThreads)),
-                     dim3(kThreads), 0, nullptr, s->dev(0), *cam, width, height, spp, depth, seed,
+                     dim3(kThreads), s->stack_bytes(0), nullptr, s->dev(0), *cam, width, height, spp, depth, seed,
                      out, fg_mat_opt);
   PM_HIP(hipGetLastError());
   PM_HIP(hipDeviceSynchronize());

@@ -266,6 +266,15 @@ struct SceneImpl {
   bool has_ior = false;
   DevBuf tris, tri4, nodes, mats, ior, lights, le;
 
+  // LDS of a traversal stack for a workgroup of ray kernels: a ray pushes at
+  // most one entry per internal level (bvh_depth + 1 of them; the builder
+  // keeps bvh_depth <= kBvhStack - 2), entry-major across the kThreads lanes.
+  // Sized to the scene, not to kBvhStack, so shallow BVHs leave LDS for
+  // more resident waves.  Brute force uses none.
+  size_t stack_bytes(int brute) const {
+    return brute ? 0 : static_cast<size_t>(bvh_depth + 2) * scn::kThreads * sizeof(int);
+  }
+
   scn::SceneDev dev(int brute) const {
     return scn::SceneDev{tris.as<pmx_triangle>(), tri4.as<float4>(), nodes.as<BvhNode>(),
                     n_tri, brute, mats.as<pm_material>(),

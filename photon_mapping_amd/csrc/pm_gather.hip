@@ -1339,31 +1339,33 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
           if (valid && rank < st.need) members |= 1u << i;
         }
         if (kAccum) {
-          // members' photons, two per round so their loads overlap
+          // members' photons, four per round so their loads overlap
+          constexpr int kRound = 4;
           uint32_t mm = members;
           while (wave_any(mm != 0u)) {
-            const uint32_t i0 = mm ? static_cast<uint32_t>(__builtin_ctz(mm)) : 32u;
-            mm &= mm - 1u;
-            const uint32_t i1 = mm ? static_cast<uint32_t>(__builtin_ctz(mm)) : 32u;
-            mm &= mm - 1u;
-            const bool v0 = i0 < 32u, v1 = i1 < 32u;
-            int32_t s0 = 0, s1 = 0;
-            if (v0) s0 = static_cast<int32_t>(sh.sb.slot[i0 * kWave + lane]);
-            if (v1) s1 = static_cast<int32_t>(sh.sb.slot[i1 * kWave + lane]);
-            float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f), w1 = w0;
-            float2 f0 = make_float2(0.f, 0.f), f1 = f0;
-            if (v0) { w0 = D.wif4[s0]; f0 = D.flux2[s0]; }
-            if (v1) { w1 = D.wif4[s1]; f1 = D.flux2[s1]; }
-            const float c0s = dot_ref(w0.x, w0.y, w0.z, nx, ny, nz);
-            const float c1s = dot_ref(w1.x, w1.y, w1.z, nx, ny, nz);
-            const float sel0 = (v0 && lane_lambert && !(c0s < 0.0f)) ? 1.0f : 0.0f;
-            const float sel1 = (v1 && lane_lambert && !(c1s < 0.0f)) ? 1.0f : 0.0f;
-            acc_r = fmaf(sel0, w0.w, acc_r);
-            acc_g = fmaf(sel0, f0.x, acc_g);
-            acc_b = fmaf(sel0, f0.y, acc_b);
-            acc_r = fmaf(sel1, w1.w, acc_r);
-            acc_g = fmaf(sel1, f1.x, acc_g);
-            acc_b = fmaf(sel1, f1.y, acc_b);
+            uint32_t im[kRound];
+            float4 wm[kRound];
+            float2 fm[kRound];
+#pragma unroll
+            for (int u = 0; u < kRound; ++u) {
+              im[u] = mm ? static_cast<uint32_t>(__builtin_ctz(mm)) : 32u;
+              mm &= mm - 1u;
+              wm[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+              fm[u] = make_float2(0.f, 0.f);
+              if (im[u] < 32u) {
+                const int32_t sl = static_cast<int32_t>(sh.sb.slot[im[u] * kWave + lane]);
+                wm[u] = D.wif4[sl];
+                fm[u] = D.flux2[sl];
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < kRound; ++u) {
+              const float cs = dot_ref(wm[u].x, wm[u].y, wm[u].z, nx, ny, nz);
+              const float sel = (im[u] < 32u && lane_lambert && !(cs < 0.0f)) ? 1.0f : 0.0f;
+              acc_r = fmaf(sel, wm[u].w, acc_r);
+              acc_g = fmaf(sel, fm[u].x, acc_g);
+              acc_b = fmaf(sel, fm[u].y, acc_b);
+            }
           }
         } else if (kRows) {
 #pragma unroll

@@ -1191,8 +1191,18 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         st.klo = tau_in + 1; st.khi = tau_in + 1; st.need = 0; st.cnt = 0;
         const float xs = part ? sqrt_up(cull_bound(r0sq)) * 1.00002f + dq : 0.0f;
         const float rs = wave_max(xs) * 1.0002f;
-        nr = build_list(D, cx, cy, cz, rs * rs + 1e-30f, sh, lane, &total,
-                        count_stats ? &counters[kStatOverflow] : nullptr);
+        const float rs2 = rs * rs + 1e-30f;
+        bool fine = false;
+        nr = build_list(D, cx, cy, cz, rs2, sh, lane, &total,
+                        count_stats ? &counters[kStatOverflow] : nullptr, true, &fine);
+        // both scatter passes read the list: keep exactly the photons of
+        // the run's sphere when they fit the slot list
+        const uint32_t ns = collect_slots(D, sh, lane, nr, total, fine, cx, cy, cz,
+                                          rs2 * 1.00001f);
+        if (ns != kSlotList) {
+          nr = kSlotList;
+          total = ns;
+        }
         pt.mark(2, lane);
       } else if (all) {
         // every photon is a neighbour; pass B tracks the largest key
@@ -1494,10 +1504,26 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         float4* qb = qa + kWave;
         float4* qc = qb + kWave;
         static_assert(3 * kWave * 4 <= (kBins + 2) * kWave, "query staging fits the bins");
-        qa[lane] = make_float4(qx, qy, qz, __uint_as_float(t_hi));
-        qb[lane] = make_float4(nx, ny, nz, __uint_as_float(t_lo));
-        qc[lane] = make_float4(c_r, c_g, c_b, 0.0f);
+        // The contributing queries are packed in lane order and padded to a
+        // multiple of four with entries whose coefficients are zero (adding
+        // sel * 0 leaves a sum unchanged), so the loop below reads four
+        // queries per step.
         const uint64_t cm = __ballot(contributes);
+        const uint32_t ncm = static_cast<uint32_t>(__popcll(cm));
+        const uint32_t npad = (ncm + 3u) & ~3u;
+        {
+          uint32_t j = kWave;
+          if (contributes) j = lanes_below(cm, lane);
+          else if (ncm + lanes_below(~cm, lane) < npad) j = ncm + lanes_below(~cm, lane);
+          if (j < static_cast<uint32_t>(kWave)) {
+            qa[j] = contributes ? make_float4(qx, qy, qz, __uint_as_float(t_hi))
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+            qb[j] = contributes ? make_float4(nx, ny, nz, __uint_as_float(t_lo))
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+            qc[j] = contributes ? make_float4(c_r, c_g, c_b, 0.0f)
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+        }
         __syncthreads();
         if (cm != 0ull) {
           const float4* __restrict__ P = D.pos4;
@@ -1526,23 +1552,24 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
             float a_r = 0.f, a_g = 0.f, a_b = 0.f;
             auto queries = [&](auto test_tag) {
               constexpr bool kTest = decltype(test_tag)::value;
-              uint64_t mm = cm;
-              while (mm != 0ull) {
-                const int i = __builtin_ctzll(mm);
-                mm &= mm - 1ull;
-                const float4 A = qa[i];
-                const float4 B = qb[i];
-                const float4 C = qc[i];
-                const float d2 = d2_ref(A.x, A.y, A.z, p.x, p.y, p.z);
-                const uint32_t db = __float_as_uint(d2);
-                const uint32_t th = __float_as_uint(A.w);
-                const bool nb = db < th || (db == th && idx <= __float_as_uint(B.w));
-                bool on = nb && inc;
-                if (kTest && per_query) on = nb && !(dot_ref(w.x, w.y, w.z, B.x, B.y, B.z) < 0.0f);
-                const float sel = on ? 1.0f : 0.0f;
-                a_r = fmaf(sel, C.x, a_r);
-                a_g = fmaf(sel, C.y, a_g);
-                a_b = fmaf(sel, C.z, a_b);
+              for (uint32_t i0 = 0; i0 < npad; i0 += 4) {
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                  const float4 A = qa[i0 + u];
+                  const float4 B = qb[i0 + u];
+                  const float4 C = qc[i0 + u];
+                  const float d2 = d2_ref(A.x, A.y, A.z, p.x, p.y, p.z);
+                  const uint32_t db = __float_as_uint(d2);
+                  const uint32_t th = __float_as_uint(A.w);
+                  const bool nb = db < th || (db == th && idx <= __float_as_uint(B.w));
+                  bool on = nb && inc;
+                  if (kTest && per_query)
+                    on = nb && !(dot_ref(w.x, w.y, w.z, B.x, B.y, B.z) < 0.0f);
+                  const float sel = on ? 1.0f : 0.0f;
+                  a_r = fmaf(sel, C.x, a_r);
+                  a_g = fmaf(sel, C.y, a_g);
+                  a_b = fmaf(sel, C.z, a_b);
+                }
               }
             };
             if (__ballot(ok && per_query) != 0ull) queries(std::true_type{});

@@ -597,7 +597,7 @@ int prepare_reserve(Workspace* ws, int n_inst, int64_t total_q, int64_t total_gr
 int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
                     const int64_t* dev_q_prefix, const int64_t* dev_g_prefix,
                     int64_t total_q, int64_t total_groups, hipStream_t s,
-                    const uint32_t** perm_out) {
+                    const uint32_t** perm_out, bool need_anchors) {
   if (total_q == 0) {
     *perm_out = nullptr;
     return PM_OK;
@@ -617,9 +617,10 @@ int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
     if (int rc = radix_sort_u32(ws, k0, v0, k1, v1, total_q, kCodeBits, s, &which,
                                 kQuerySortLoBit))
       return rc;
-    hipLaunchKernelGGL(query_anchor_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, n_inst, total_q, which ? k1 : k0,
-                       ws->q_anchor.as<uint32_t>());
+    if (need_anchors)
+      hipLaunchKernelGGL(query_anchor_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
+                         dev_q_prefix, n_inst, total_q, which ? k1 : k0,
+                         ws->q_anchor.as<uint32_t>());
   } else {
     uint64_t* k0 = ws->keys.as<uint64_t>();
     uint64_t* k1 = ws->keys2.as<uint64_t>();
@@ -631,9 +632,10 @@ int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
     if (int rc = radix_sort_u64(ws, k0, v0, k1, v1, total_q, 32 + ib, s, &which,
                                 kQuerySortLoBit))
       return rc;
-    hipLaunchKernelGGL(query_anchor_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, n_inst, total_q, which ? k1 : k0,
-                       ws->q_anchor.as<uint32_t>());
+    if (need_anchors)
+      hipLaunchKernelGGL(query_anchor_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
+                         dev_q_prefix, n_inst, total_q, which ? k1 : k0,
+                         ws->q_anchor.as<uint32_t>());
   }
   PM_HIP(hipGetLastError());
   *perm_out = which ? v1 : v0;

@@ -161,8 +161,15 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
 
   CallTimer tm(ctx, s);
   tm.mark(0);
+  // the anchors serve the radius bounds only: an adjoint with the forward
+  // keys and an instance whose set is the whole map read none
+  bool need_anchors = false;
+  for (const GatherDesc& d : ds)
+    need_anchors = need_anchors ||
+                   (!(mode == kModeAdjoint && d.key_in != nullptr) && d.all_members == 0);
   const uint32_t* perm = nullptr;
-  if (int rc = prepare_queries(&ctx->ws, n_inst, dd, dqp, dgp, total_q, total_groups, s, &perm))
+  if (int rc = prepare_queries(&ctx->ws, n_inst, dd, dqp, dgp, total_q, total_groups, s, &perm,
+                               need_anchors))
     return rc;
   tm.mark(1);
   GatherLaunch L{};
@@ -806,7 +813,10 @@ int pm_gather_adjoint(pm_photonmap* h, int64_t m, const pm_query* q, const pm_ma
   }
   const int k_eff = static_cast<int>(std::min<int64_t>(k, n));
   const float4* anchors = nullptr;
-  if (int rc = ensure_anchors(map, k_eff, s, &anchors)) return rc;
+  // with the forward keys the radius of every set is known: no anchors
+  if (!dkey) {
+    if (int rc = ensure_anchors(map, k_eff, s, &anchors)) return rc;
+  }
   GatherDesc d{};
   fill_desc_map_part(map, &d);
   d.anchors = anchors;

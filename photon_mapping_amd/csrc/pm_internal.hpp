@@ -245,15 +245,16 @@ int update_flux(PhotonMapImpl* map, const float* flux_dev, hipStream_t s);
 int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** out);
 void fill_desc_map_part(const PhotonMapImpl* map, GatherDesc* d);
 
-// Hilbert keys for all queries of all instances, stable sort, and the
-// nearest radius anchor of every sorted query (ws->q_anchor, global sorted
-// order; instance i starts at q_prefix[i] there, as in query order).
+// Hilbert keys for all queries of all instances, stable sort, and (when
+// need_anchors) the nearest radius anchor of every sorted query
+// (ws->q_anchor, global sorted order; instance i starts at q_prefix[i] there,
+// as in query order).
 // Reserves every buffer prepare_queries and the gather launch use.
 int prepare_reserve(Workspace* ws, int n_inst, int64_t total_q, int64_t total_groups);
 int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
                     const int64_t* dev_q_prefix, const int64_t* dev_g_prefix,
                     int64_t total_q, int64_t total_groups, hipStream_t s,
-                    const uint32_t** perm_out);
+                    const uint32_t** perm_out, bool need_anchors = true);
 
 int launch_gather(Context* ctx, const GatherLaunch& L, hipStream_t s);
 int launch_knn_rowsort(const GatherDesc* dev_desc, int64_t m, int32_t* out_idx,

@@ -49,6 +49,25 @@ def test_adjoint_matches_fp64_oracle(gpu, oracle, rng, k):
     assert not dflux_r[:, ~used].any()
 
 
+def test_adjoint_reused_keys_on_fresh_map(gpu, oracle, rng):
+    """Given the forward keys the adjoint needs no radius anchors: a second
+    map over the same photons that never ran a gather gives the oracle's
+    gradients too."""
+    pos, wi, flux, q, mats, pmap = _setup(gpu, rng)
+    k, n_emit = 32, 3e4
+    g = rng.normal(size=(len(q), 3)).astype(np.float32)
+    print("adjoint, reused keys, fresh map", flush=True)
+    _, _, key = pmap.gather_radiance(q, mats, k, n_emit)
+    fresh = gpu.PhotonMap()
+    fresh.set_photons_soa(pos, wi, flux)
+    fresh.build()
+    dflux, drho = fresh.gather_adjoint(q, mats, k, n_emit, g, key=key)
+    bidx, br2 = oracle.brute_knn(pos, q[:, :3], k)
+    ef, er = oracle.adjoint_fp64(wi, flux, q, mats, bidx, br2, n_emit, g)
+    np.testing.assert_allclose(dflux, ef, rtol=1e-4, atol=1e-5 * np.abs(ef).max())
+    np.testing.assert_allclose(drho, er, rtol=1e-4, atol=1e-6 * np.abs(er).max())
+
+
 def test_adjoint_directional_finite_difference(gpu, rng):
     """<dL/dflux, v> against (L(flux + h v) - L(flux - h v)) / 2h on the GPU
     forward pass, and the same for the albedos (target: 1e-3 relative)."""

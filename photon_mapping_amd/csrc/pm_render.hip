@@ -261,9 +261,9 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
             const float r = sqrtf(dot(to, to));
             const V3 wi = to * (1.0f / r);
             const float pdf_dir = (1.0f / area) * r * r / fabsf(dot(wi * -1.0f, lns));
-            const Hit hs = closest_hit(S, x, wi, r - kRayEps, stk);
+            const bool blocked = occluded(S, x, wi, r - kRayEps, stk);
             ++n_rays;
-            if (hs.tri < 0) {
+            if (!blocked) {
               const float f = lambert_on(wo, wi, ns);
               const float c = f * fabsf(dot(wi, ns)) /
                               ((1.0f / static_cast<float>(S.n_lights)) * pdf_dir);

@@ -177,6 +177,65 @@ __device__ inline Hit closest_hit(const SceneDev& S, V3 o, V3 d, float tmax, int
   return h;
 }
 
+// Whether some triangle meets the ray at kRayEps < t < tmax: the shadow test
+// of include/integrator.h:198-204 reads only whether scene.intersect found a
+// hit.  It stops at the first such triangle.  Its answer equals
+// closest_hit(...).tri >= 0: both run mt_test on the same operands, and every
+// leaf that closest_hit reaches (boxes entered before its shrinking limit)
+// is reachable here under the fixed limit tmax.
+__device__ inline bool occluded(const SceneDev& S, V3 o, V3 d, float tmax, int* stk) {
+  if (S.brute) {
+    for (int i = 0; i < S.n_tri; ++i) {
+      const V3 p0 = ld3(S.tris[i].v[0]);
+      float t, u, v;
+      if (mt_test(p0, ld3(S.tris[i].v[1]) - p0, ld3(S.tris[i].v[2]) - p0, o, d, t, u, v) &&
+          t < tmax)
+        return true;
+    }
+    return false;
+  }
+  auto leaf_any = [&](int code) {
+    const int first = code >> 4, count = code & 15;
+    for (int j = 0; j < count; ++j) {
+      const float4* q = S.tri4 + 3 * (first + j);
+      const float4 a = q[0], b = q[1], c = q[2];
+      float t, u, v;
+      if (mt_test(xyz(a), xyz(b), xyz(c), o, d, t, u, v) && t < tmax) return true;
+    }
+    return false;
+  };
+  const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  int node = 0, sp = 0;
+  for (;;) {
+    const BvhNode nd = S.nodes[node];
+    float tl = box_entry(nd.bx.x, nd.bx.y, nd.by.x, nd.by.y, nd.bz.x, nd.bz.y, o, inv, tmax);
+    float tr = box_entry(nd.bx.z, nd.bx.w, nd.by.z, nd.by.w, nd.bz.z, nd.bz.w, o, inv, tmax);
+    if (tl != INFINITY && nd.ref.x < 0) {
+      if (leaf_any(~nd.ref.x)) return true;
+      tl = INFINITY;
+    }
+    if (tr != INFINITY && nd.ref.y < 0) {
+      if (leaf_any(~nd.ref.y)) return true;
+      tr = INFINITY;
+    }
+    if (tl != INFINITY && tr != INFINITY) {
+      const bool left_first = tl <= tr;
+      stk[sp * kThreads] = left_first ? nd.ref.y : nd.ref.x;
+      ++sp;
+      node = left_first ? nd.ref.x : nd.ref.y;
+    } else if (tl != INFINITY) {
+      node = nd.ref.x;
+    } else if (tr != INFINITY) {
+      node = nd.ref.y;
+    } else {
+      if (sp == 0) break;
+      --sp;
+      node = stk[sp * kThreads];
+    }
+  }
+  return false;
+}
+
 __device__ __forceinline__ V3 shading_normal(const pmx_triangle& tr, float u, float v) {
   return ld3(tr.n[0]) * (1.0f - u - v) + ld3(tr.n[1]) * u + ld3(tr.n[2]) * v;
 }

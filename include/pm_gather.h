@@ -98,7 +98,7 @@ typedef struct pm_stats {
 #define PM_OPT_LEAF_SIZE 4      /* default kd-tree leaf size (default 8)    */
 #define PM_OPT_XCD_AWARE 5      /* 0/1: per-XCD work queues (default 1)     */
 #define PM_OPT_COUNT_STATS 6    /* 0/1: count fallbacks/refinements         */
-#define PM_OPT_RUN_SPLIT 7      /* query-run split factor x1000 (def. 750)  */
+#define PM_OPT_RUN_SPLIT 7      /* query-run split factor x1000 (def. 900)  */
 
 /* ---- errors / version ----------------------------------------------------- */
 /* Thread-local text of the last failing call ("" when none). */

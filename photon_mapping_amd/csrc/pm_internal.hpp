@@ -200,7 +200,7 @@ struct Context {
   int anchor_stride = kAnchorStride;
   int leaf_size = 8;
   int xcd_aware = 1;
-  float run_split = 0.75f;
+  float run_split = 0.9f;
   int timing = 0;
   int count_stats = 0;
   pm_stats stats{};

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
 // calls this file; only the tests, the smoke check and the CPU-baseline leg of
-// the benchmark do, and only as the checker / the timed CPU reference.
+// the benchmark do, and only to check results / as the timed CPU reference.
 //
 // What it restates (paths relative to the reference checkout):
 //   * distance2                       include/photon_map.h:33-41
@@ -14,7 +14,7 @@
 //     of include/core.h:206-209,234-237: only the y = shading-normal component
 //     of the local direction is ever read)
 //   * computeRadianceWithPhotonMap    include/integrator.h:133-152
-// and adds what has no reference counterpart but is needed as a checker:
+// and adds what has no reference counterpart but is needed to check results:
 //   * brute-force exact kNN under the (FP32 d^2, photon index) order,
 //   * an FP64 radiance estimate and its FP64 adjoint (gradients with respect
 //     to per-photon flux and per-material albedo).

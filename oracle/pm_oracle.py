@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/pm_oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
-CPU-baseline leg of bench.py, as the checker.  The product library never
+CPU-baseline leg of bench.py, to check results.  The product library never
 loads it.
 """
 

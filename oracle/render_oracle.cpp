@@ -1,5 +1,5 @@
-// CPU restatement of the reference's two integrators and render loop, the
-// checker of the device renderer's images.
+// CPU restatement of the reference's two integrators and render loop, against
+// which the device renderer's images are tested.
 //
 // TEST INFRASTRUCTURE ONLY: built into libpm_oracle.so and called by tests/
 // alone; the product library never links or calls it.

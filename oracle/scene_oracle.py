@@ -1,6 +1,6 @@
 """CPU restatement of the scene's closest-hit query, in numpy float32.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of
+TEST INFRASTRUCTURE ONLY: imported by tests/ to check the results of
 pmx_scene_intersect.  The product library never loads it.
 
 The reference intersects rays with Embree (include/scene.h:307-348: rays

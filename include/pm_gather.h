@@ -164,6 +164,33 @@ int pm_gather_radiance(pm_photonmap* map, int64_t m, const pm_query* q,
                        double n_emit, float* out_rgb, float* out_r2,
                        uint64_t* out_key, int mem, void* stream);
 
+/* ---- allocation-free radiance gather (no reference counterpart) ----------------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments, semantics and results (bit for bit) as pm_gather_radiance with
+ * PM_MEM_DEVICE: every pointer is device memory and the call is
+ * asynchronous on the stream.  The call allocates nothing and does not
+ * synchronise: its scratch (descriptor, query sort, nearest anchors,
+ * counters, and radius bounds when needed) lives in a fixed device array of
+ * the library, PM_NOALLOC_SCRATCH_BYTES on each GPU it runs on, set up at
+ * pm_context_create and held until the process ends.  Limits:
+ *   - m <= PM_NOALLOC_MAX_QUERIES per call (else PM_ERR_UNSUPPORTED);
+ *   - the map's radius bounds for min(k, N) are read from its cache when an
+ *     earlier gather at that k computed them; otherwise they are computed
+ *     into the scratch array on every call, which needs at most
+ *     PM_NOALLOC_MAX_ANCHORS anchors (maps of up to 1,048,576 photons at the
+ *     default anchor stride of 4), else PM_ERR_UNSUPPORTED.
+ * All calls on one GPU share its scratch array, which keeps the last call's
+ * contents: calls that overlap on one GPU (other streams, other threads,
+ * other contexts) race, and must be ordered by the caller. */
+#define PM_NOALLOC_MAX_QUERIES (1 << 20)
+#define PM_NOALLOC_MAX_ANCHORS (1 << 18)
+#define PM_NOALLOC_SCRATCH_BYTES (25 << 20)
+int pm_gather_radiance_noalloc(pm_photonmap* map, int64_t m, const pm_query* q,
+                               const pm_material* mats, int n_mats, int k, double n_emit,
+                               float* out_rgb, float* out_r2, uint64_t* out_key,
+                               void* stream);
+
 /* ---- adjoint (no reference counterpart) ----------------------------------------
  * Reverse-mode derivative of pm_gather_radiance for dl_drgb (m*3):
  *   dl_dflux[c*N + i]  += sum_q s_qi (rho_c/PI) dl_drgb[q,c] / (n_emit PI r_q^2)

@@ -39,6 +39,9 @@ PM_OPT_XCD_AWARE = 5
 PM_OPT_COUNT_STATS = 6
 PM_OPT_RUN_SPLIT = 7
 
+PM_NOALLOC_MAX_QUERIES = 1 << 20
+PM_NOALLOC_MAX_ANCHORS = 1 << 18
+
 
 class PmError(RuntimeError):
     """A C-ABI call returned a non-zero status."""
@@ -104,6 +107,7 @@ SIGNATURES = {
     "pm_photonmap_get_photon": (None, [VP, I64, VP]),
     "pm_knn": (None, [VP, I64, VP, I32, VP, VP, I32, VP]),
     "pm_gather_radiance": (None, [VP, I64, VP, VP, I32, I32, DBL, VP, VP, VP, I32, VP]),
+    "pm_gather_radiance_noalloc": (None, [VP, I64, VP, VP, I32, I32, DBL, VP, VP, VP, VP]),
     "pm_gather_adjoint": (None, [VP, I64, VP, VP, I32, I32, DBL, VP, VP, VP, VP, VP, I32,
                                  VP]),
     "pm_gather_batched": (None, [VP, I32, VP, VP]),

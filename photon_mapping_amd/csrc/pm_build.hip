@@ -394,6 +394,28 @@ int launch_fill_f32(float* p, float v, int64_t n, hipStream_t s) {
   return PM_OK;
 }
 
+// Writes one descriptor (passed by value in the kernel arguments) and its
+// query and group prefixes {0, q_total}, {0, g_total} to device memory.
+__global__ void store_desc_kernel(GatherDesc d, GatherDesc* __restrict__ dst,
+                                  int64_t* __restrict__ prefix, int64_t q_total,
+                                  int64_t g_total) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *dst = d;
+    prefix[0] = 0;
+    prefix[1] = q_total;
+    prefix[2] = 0;
+    prefix[3] = g_total;
+  }
+}
+
+int launch_store_desc(const GatherDesc& d, GatherDesc* dst, int64_t* prefix, int64_t q_total,
+                      int64_t g_total, hipStream_t s) {
+  hipLaunchKernelGGL(store_desc_kernel, dim3(1), dim3(64), 0, s, d, dst, prefix, q_total,
+                     g_total);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
 void fill_desc_map_part(const PhotonMapImpl* map, GatherDesc* d) {
   d->pos4 = map->pos4.as<float4>();
   d->wif4 = map->wif4.as<float4>();
@@ -531,24 +553,7 @@ int update_flux(PhotonMapImpl* map, const float* flux_dev, hipStream_t s) {
   return PM_OK;
 }
 
-int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** out) {
-  *out = map->anchors.as<float4>();
-  if (map->n == 0 || k_eff >= map->n) return PM_OK;  // every photon is a neighbour
-  PhotonMapImpl::AnchorSet* slot = &map->aset[0];
-  for (auto& a : map->aset) {
-    if (a.k == k_eff) {
-      a.stamp = ++map->anchor_clock;
-      *out = a.buf.as<float4>();
-      return PM_OK;
-    }
-    if (a.k < 0 || a.stamp < slot->stamp) slot = &a;
-    if (slot->k < 0) break;
-  }
-  Context* ctx = map->ctx;
-  const size_t bytes = static_cast<size_t>(map->n_anchors) * sizeof(float4);
-  slot->k = -1;
-  if (int rc = slot->buf.reserve(bytes)) return rc;
-  PM_HIP(hipMemcpyAsync(slot->buf.ptr, map->anchors.ptr, bytes, hipMemcpyDeviceToDevice, s));
+GatherDesc anchor_desc(const PhotonMapImpl* map, int k_eff, float* out) {
   GatherDesc d{};
   fill_desc_map_part(map, &d);
   d.m = map->n_anchors;
@@ -558,24 +563,62 @@ int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** 
   d.k = k_eff;
   d.k_eff = k_eff;
   d.all_members = 0;
-  d.anchor_out = slot->buf.as<float>();
-  if (int rc = ctx->ws.counters.reserve(64 * sizeof(uint32_t))) return rc;
-  int64_t gp[2] = {0, d.n_groups};
-  if (int rc = stage_begin(ctx, sizeof(GatherDesc) + sizeof(gp) + 256, s)) return rc;
+  d.anchor_out = out;
+  return d;
+}
+
+const float4* cached_anchors(PhotonMapImpl* map, int k_eff) {
+  for (auto& a : map->aset)
+    if (a.k == k_eff) {
+      a.stamp = ++map->anchor_clock;
+      return a.buf.as<float4>();
+    }
+  return nullptr;
+}
+
+int launch_anchor_bounds(Context* ctx, const GatherDesc* dev_desc, const int64_t* dev_gp,
+                         int64_t n_anchors, uint32_t* counters, hipStream_t s) {
   GatherLaunch L{};
-  L.descs = static_cast<const GatherDesc*>(stage_put(ctx, &d, sizeof(d)));
-  L.group_prefix = static_cast<const int64_t*>(stage_put(ctx, gp, sizeof(gp)));
-  if (int rc = stage_commit(ctx, s)) return rc;
+  L.descs = dev_desc;
+  L.group_prefix = dev_gp;
   L.perm = nullptr;
   L.q_prefix = nullptr;
-  L.counters = ctx->ws.counters.as<uint32_t>();
-  L.total_groups = d.m;  // the anchor kernel takes one anchor per wave at a time
+  L.counters = counters;
+  L.total_groups = n_anchors;  // the anchor kernel takes one anchor per wave at a time
   L.n_inst = 1;
   L.xcd_aware = 0;
   L.count_stats = 0;
   L.run_split = ctx->run_split;
   L.mode = kModeAnchor;
-  if (int rc = launch_gather(ctx, L, s)) return rc;
+  return launch_gather(ctx, L, s);
+}
+
+int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** out) {
+  *out = map->anchors.as<float4>();
+  if (map->n == 0 || k_eff >= map->n) return PM_OK;  // every photon is a neighbour
+  if (const float4* hit = cached_anchors(map, k_eff)) {
+    *out = hit;
+    return PM_OK;
+  }
+  PhotonMapImpl::AnchorSet* slot = &map->aset[0];
+  for (auto& a : map->aset) {
+    if (a.k < 0 || a.stamp < slot->stamp) slot = &a;
+    if (slot->k < 0) break;
+  }
+  Context* ctx = map->ctx;
+  const size_t bytes = static_cast<size_t>(map->n_anchors) * sizeof(float4);
+  slot->k = -1;
+  if (int rc = slot->buf.reserve(bytes)) return rc;
+  PM_HIP(hipMemcpyAsync(slot->buf.ptr, map->anchors.ptr, bytes, hipMemcpyDeviceToDevice, s));
+  const GatherDesc d = anchor_desc(map, k_eff, slot->buf.as<float>());
+  if (int rc = ctx->ws.counters.reserve(64 * sizeof(uint32_t))) return rc;
+  int64_t gp[2] = {0, d.n_groups};
+  if (int rc = stage_begin(ctx, sizeof(GatherDesc) + sizeof(gp) + 256, s)) return rc;
+  const auto* dd = static_cast<const GatherDesc*>(stage_put(ctx, &d, sizeof(d)));
+  const auto* dgp = static_cast<const int64_t*>(stage_put(ctx, gp, sizeof(gp)));
+  if (int rc = stage_commit(ctx, s)) return rc;
+  if (int rc = launch_anchor_bounds(ctx, dd, dgp, d.m, ctx->ws.counters.as<uint32_t>(), s))
+    return rc;
   slot->k = k_eff;
   slot->stamp = ++map->anchor_clock;
   *out = slot->buf.as<float4>();
@@ -598,44 +641,62 @@ int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
                     const int64_t* dev_q_prefix, const int64_t* dev_g_prefix,
                     int64_t total_q, int64_t total_groups, hipStream_t s,
                     const uint32_t** perm_out, bool need_anchors) {
+  (void)dev_g_prefix;
+  if (total_q == 0) {
+    *perm_out = nullptr;
+    return PM_OK;
+  }
+  if (int rc = prepare_reserve(ws, n_inst, total_q, total_groups)) return rc;
+  QueryScratch qs;
+  qs.keys = ws->keys.ptr;
+  qs.keys2 = ws->keys2.ptr;
+  qs.vals = ws->vals.as<uint32_t>();
+  qs.vals2 = ws->vals2.as<uint32_t>();
+  qs.q_anchor = ws->q_anchor.as<uint32_t>();
+  qs.counts = ws->counts.as<uint32_t>();
+  qs.partials = ws->partials.as<uint32_t>();
+  return prepare_queries_raw(qs, n_inst, dev_descs, dev_q_prefix, total_q, s, perm_out,
+                             need_anchors);
+}
+
+int prepare_queries_raw(const QueryScratch& qs, int n_inst, const GatherDesc* dev_descs,
+                        const int64_t* dev_q_prefix, int64_t total_q, hipStream_t s,
+                        const uint32_t** perm_out, bool need_anchors) {
   if (total_q == 0) {
     *perm_out = nullptr;
     return PM_OK;
   }
   const bool wide = n_inst > 1;
-  if (int rc = prepare_reserve(ws, n_inst, total_q, total_groups)) return rc;
-  uint32_t* v0 = ws->vals.as<uint32_t>();
-  uint32_t* v1 = ws->vals2.as<uint32_t>();
+  uint32_t* v0 = qs.vals;
+  uint32_t* v1 = qs.vals2;
   int which = 0;
   const dim3 grid(ceil_div(total_q, kThreads));
   if (!wide) {
-    uint32_t* k0 = ws->keys.as<uint32_t>();
-    uint32_t* k1 = ws->keys2.as<uint32_t>();
+    uint32_t* k0 = static_cast<uint32_t*>(qs.keys);
+    uint32_t* k1 = static_cast<uint32_t*>(qs.keys2);
     hipLaunchKernelGGL(query_keys_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
                        dev_q_prefix, n_inst, total_q, k0, v0);
     PM_HIP(hipGetLastError());
-    if (int rc = radix_sort_u32(ws, k0, v0, k1, v1, total_q, kCodeBits, s, &which,
-                                kQuerySortLoBit))
+    if (int rc = radix_sort_u32_raw(qs.counts, qs.partials, k0, v0, k1, v1, total_q,
+                                    kCodeBits, s, &which, kQuerySortLoBit))
       return rc;
     if (need_anchors)
       hipLaunchKernelGGL(query_anchor_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                         dev_q_prefix, n_inst, total_q, which ? k1 : k0,
-                         ws->q_anchor.as<uint32_t>());
+                         dev_q_prefix, n_inst, total_q, which ? k1 : k0, qs.q_anchor);
   } else {
-    uint64_t* k0 = ws->keys.as<uint64_t>();
-    uint64_t* k1 = ws->keys2.as<uint64_t>();
+    uint64_t* k0 = static_cast<uint64_t*>(qs.keys);
+    uint64_t* k1 = static_cast<uint64_t*>(qs.keys2);
     int ib = 1;
     while ((1 << ib) < n_inst) ++ib;
     hipLaunchKernelGGL(query_keys_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
                        dev_q_prefix, n_inst, total_q, k0, v0);
     PM_HIP(hipGetLastError());
-    if (int rc = radix_sort_u64(ws, k0, v0, k1, v1, total_q, 32 + ib, s, &which,
-                                kQuerySortLoBit))
+    if (int rc = radix_sort_u64_raw(qs.counts, qs.partials, k0, v0, k1, v1, total_q, 32 + ib,
+                                    s, &which, kQuerySortLoBit))
       return rc;
     if (need_anchors)
       hipLaunchKernelGGL(query_anchor_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                         dev_q_prefix, n_inst, total_q, which ? k1 : k0,
-                         ws->q_anchor.as<uint32_t>());
+                         dev_q_prefix, n_inst, total_q, which ? k1 : k0, qs.q_anchor);
   }
   PM_HIP(hipGetLastError());
   *perm_out = which ? v1 : v0;

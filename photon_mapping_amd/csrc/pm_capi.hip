@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -17,6 +18,75 @@ namespace pm {
 namespace {
 thread_local std::string g_last_error;
 constexpr int kVersion = 1;
+}  // namespace
+
+// ---- allocation-free gather: a fixed scratch array per device ------------------
+// The scratch of pm_gather_radiance_noalloc (descriptors, query sort, nearest
+// anchors, counters, radius bounds) lives in this file-scope device array.
+// Each GPU that loads the library holds its own copy until the process ends.
+// Host code never passes `scratch` itself to a kernel or a copy: its name is a
+// host-side stand-in, not a device address.  Every device's address comes
+// from hipGetSymbolAddress at context creation (noalloc_init) and is kept in
+// g_scratch_dev.
+constexpr size_t kNoAllocScratchBytes = PM_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Byte offsets of the scratch regions (256-byte aligned).
+struct NoAllocLayout {
+  size_t desc, prefix, counters, keys, keys2, vals, vals2, q_anchor, counts, partials, anchors,
+      end;
+};
+
+NoAllocLayout noalloc_layout() {
+  auto al = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
+  const size_t mq = static_cast<size_t>(PM_NOALLOC_MAX_QUERIES);
+  NoAllocLayout L{};
+  size_t o = 0;
+  L.desc = o;      o += al(2 * sizeof(GatherDesc));          // gather, anchor pass
+  L.prefix = o;    o += al(8 * sizeof(int64_t));             // {0, m, 0, groups} x 2
+  L.counters = o;  o += al(kCounterWords * sizeof(uint32_t));
+  L.keys = o;      o += al(mq * sizeof(uint32_t));
+  L.keys2 = o;     o += al(mq * sizeof(uint32_t));
+  L.vals = o;      o += al(mq * sizeof(uint32_t));
+  L.vals2 = o;     o += al(mq * sizeof(uint32_t));
+  L.q_anchor = o;  o += al(mq * sizeof(uint32_t));
+  L.counts = o;    o += al(static_cast<size_t>(sort_count_words(mq)) * sizeof(uint32_t));
+  L.partials = o;  o += al(static_cast<size_t>(sort_partial_words(mq)) * sizeof(uint32_t));
+  L.anchors = o;   o += al(static_cast<size_t>(PM_NOALLOC_MAX_ANCHORS) * sizeof(float4));
+  L.end = o;
+  return L;
+}
+
+// Once per device, with that device current: the device address of
+// `scratch` on it.
+int noalloc_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(PM_ERR_UNSUPPORTED, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return PM_OK;
+  if (noalloc_layout().end > kNoAllocScratchBytes)
+    return fail(PM_ERR_HIP, "scratch layout exceeds the scratch array");
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                    "hipGetSymbolAddress(scratch)");
+  }
+  g_scratch_dev[device] = ptr;
+  return PM_OK;
+}
+
+char* noalloc_base(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return static_cast<char*>(g_scratch_dev[device]);
+}
 }  // namespace
 
 void set_error(const std::string& msg) { g_last_error = msg; }
@@ -422,6 +492,10 @@ int pm_context_create(int device, pm_context** out_ctx) {
   auto* ctx = new (std::nothrow) Context();
   if (!ctx) return fail(PM_ERR_OOM, "host allocation failed");
   ctx->device = device;
+  if (int rc = noalloc_init(device)) {
+    delete ctx;
+    return rc;
+  }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     ctx->n_cu = prop.multiProcessorCount;
@@ -755,6 +829,105 @@ int pm_gather_radiance(pm_photonmap* h, int64_t m, const pm_query* q, const pm_m
     PM_HIP(hipStreamSynchronize(s));
   }
   return PM_OK;
+}
+
+int pm_gather_radiance_noalloc(pm_photonmap* h, int64_t m, const pm_query* q,
+                               const pm_material* mats, int n_mats, int k, double n_emit,
+                               float* out_rgb, float* out_r2, uint64_t* out_key,
+                               void* stream) {
+  if (int rc = check_map(h)) return rc;
+  auto* map = reinterpret_cast<PhotonMapImpl*>(h);
+  Context* ctx = map->ctx;
+  if (k < 1) return fail(PM_ERR_INVALID_ARG, "k must be >= 1");
+  if (m < 0) return fail(PM_ERR_INVALID_ARG, "negative query count");
+  if (m > PM_NOALLOC_MAX_QUERIES)
+    return fail(PM_ERR_UNSUPPORTED, "pm_gather_radiance_noalloc takes at most " +
+                                        std::to_string(PM_NOALLOC_MAX_QUERIES) +
+                                        " queries per call");
+  if (m == 0) return PM_OK;
+  if (!q || !out_rgb) return fail(PM_ERR_INVALID_ARG, "null query or output");
+  if (n_mats < 0 || (n_mats > 0 && !mats)) return fail(PM_ERR_INVALID_ARG, "bad materials");
+  char* base = noalloc_base(ctx->device);
+  if (!base) return fail(PM_ERR_HIP, "scratch array of this device is not set up");
+  const NoAllocLayout lay = noalloc_layout();
+  hipStream_t s = as_stream(stream);
+  const int64_t n = map->n;
+  if (n == 0) {
+    ZeroFill z;
+    z.rgb = out_rgb;
+    z.r2 = out_r2;
+    z.key = out_key;
+    z.m = m;
+    return z.launch(s);
+  }
+  auto* desc = reinterpret_cast<GatherDesc*>(base + lay.desc);
+  auto* prefix = reinterpret_cast<int64_t*>(base + lay.prefix);
+  auto* counters = reinterpret_cast<uint32_t*>(base + lay.counters);
+  const int k_eff = static_cast<int>(std::min<int64_t>(k, n));
+  const bool all_members = k_eff >= n;
+  const float4* anchors = map->anchors.as<float4>();
+  if (!all_members) {
+    anchors = cached_anchors(map, k_eff);
+    if (!anchors) {
+      // radius bounds for this k into the scratch array (valid for this call)
+      if (map->n_anchors > PM_NOALLOC_MAX_ANCHORS)
+        return fail(PM_ERR_UNSUPPORTED,
+                    "the map's radius bounds for this k are not cached and its " +
+                        std::to_string(map->n_anchors) + " anchors exceed the scratch (" +
+                        std::to_string(PM_NOALLOC_MAX_ANCHORS) + ")");
+      auto* dst = reinterpret_cast<float4*>(base + lay.anchors);
+      PM_HIP(hipMemcpyAsync(dst, map->anchors.ptr,
+                            static_cast<size_t>(map->n_anchors) * sizeof(float4),
+                            hipMemcpyDeviceToDevice, s));
+      const GatherDesc ad = anchor_desc(map, k_eff, reinterpret_cast<float*>(dst));
+      if (int rc = launch_store_desc(ad, desc + 1, prefix + 4, ad.m, ad.n_groups, s)) return rc;
+      if (int rc = launch_anchor_bounds(ctx, desc + 1, prefix + 6, ad.m, counters, s))
+        return rc;
+      anchors = dst;
+    }
+  }
+  GatherDesc d{};
+  fill_desc_map_part(map, &d);
+  d.anchors = anchors;
+  d.q = q;
+  d.mats = mats;
+  d.n_mats = n_mats;
+  d.out_rgb = out_rgb;
+  d.out_r2 = out_r2;
+  d.out_key = out_key;
+  d.m = m;
+  d.k = k;
+  d.k_eff = k_eff;
+  d.n_emit = static_cast<float>(n_emit);
+  d.all_members = all_members ? 1 : 0;
+  d.q_base = 0;
+  d.group_base = 0;
+  d.n_groups = ceil_div(m, kWave);
+  d.q_anchor = reinterpret_cast<uint32_t*>(base + lay.q_anchor);
+  if (int rc = launch_store_desc(d, desc, prefix, m, d.n_groups, s)) return rc;
+  QueryScratch qs;
+  qs.keys = base + lay.keys;
+  qs.keys2 = base + lay.keys2;
+  qs.vals = reinterpret_cast<uint32_t*>(base + lay.vals);
+  qs.vals2 = reinterpret_cast<uint32_t*>(base + lay.vals2);
+  qs.q_anchor = reinterpret_cast<uint32_t*>(base + lay.q_anchor);
+  qs.counts = reinterpret_cast<uint32_t*>(base + lay.counts);
+  qs.partials = reinterpret_cast<uint32_t*>(base + lay.partials);
+  const uint32_t* perm = nullptr;
+  if (int rc = prepare_queries_raw(qs, 1, desc, prefix, m, s, &perm, !all_members)) return rc;
+  GatherLaunch L{};
+  L.descs = desc;
+  L.group_prefix = prefix + 2;
+  L.perm = perm;
+  L.q_prefix = prefix;
+  L.counters = counters;
+  L.total_groups = d.n_groups;
+  L.n_inst = 1;
+  L.xcd_aware = ctx->xcd_aware;
+  L.count_stats = 0;
+  L.run_split = ctx->run_split;
+  L.mode = kModeRadiance;
+  return launch_gather(ctx, L, s);
 }
 
 int pm_gather_adjoint(pm_photonmap* h, int64_t m, const pm_query* q, const pm_material* mats,

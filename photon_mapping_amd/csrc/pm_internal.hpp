@@ -237,12 +237,36 @@ int sort_reserve(Workspace* ws, int64_t n);
 int radix_sort_u64(Workspace* ws, uint64_t* keys, uint32_t* vals,
                    uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
                    hipStream_t s, int* which, int lo_bit = 0);
+// uint32 words of the sort histograms and of the scan partials for n keys.
+int64_t sort_count_words(int64_t n);
+int64_t sort_partial_words(int64_t n);
+// The same sorts over caller-provided histogram and partial storage (at
+// least the words above): they allocate nothing.
+int radix_sort_u32_raw(uint32_t* counts, uint32_t* partials, uint32_t* keys, uint32_t* vals,
+                       uint32_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
+                       hipStream_t s, int* which, int lo_bit = 0);
+int radix_sort_u64_raw(uint32_t* counts, uint32_t* partials, uint64_t* keys, uint32_t* vals,
+                       uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
+                       hipStream_t s, int* which, int lo_bit = 0);
 
 int build_photon_map(PhotonMapImpl* map, hipStream_t s);
 int update_flux(PhotonMapImpl* map, const float* flux_dev, hipStream_t s);
 // Anchor radius bounds of the map for k_eff (computed on first use, then
 // cached); *out receives the anchor array the gather must read.
 int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** out);
+// The cached anchor set of k_eff (marked most recently used), or nullptr.
+const float4* cached_anchors(PhotonMapImpl* map, int k_eff);
+// Descriptor of the anchor-bound pass of the map for k_eff writing to out
+// (4 floats per anchor; the centroids must already be there).
+GatherDesc anchor_desc(const PhotonMapImpl* map, int k_eff, float* out);
+// Launches the anchor-bound pass of a descriptor already in device memory
+// (dev_gp: its group prefix {0, groups}).
+int launch_anchor_bounds(Context* ctx, const GatherDesc* dev_desc, const int64_t* dev_gp,
+                         int64_t n_anchors, uint32_t* counters, hipStream_t s);
+// Stores d at dst and the prefixes {0, q_total, 0, g_total} at prefix (4
+// int64) with a one-thread kernel: no host staging.
+int launch_store_desc(const GatherDesc& d, GatherDesc* dst, int64_t* prefix, int64_t q_total,
+                      int64_t g_total, hipStream_t s);
 void fill_desc_map_part(const PhotonMapImpl* map, GatherDesc* d);
 
 // Hilbert keys for all queries of all instances, stable sort, and (when
@@ -255,6 +279,22 @@ int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
                     const int64_t* dev_q_prefix, const int64_t* dev_g_prefix,
                     int64_t total_q, int64_t total_groups, hipStream_t s,
                     const uint32_t** perm_out, bool need_anchors = true);
+// Device storage of prepare_queries: keys / keys2 hold total_q keys (uint64
+// when n_inst > 1, else uint32), vals / vals2 / q_anchor total_q words,
+// counts and partials the sort_count_words / sort_partial_words of total_q.
+struct QueryScratch {
+  void* keys;
+  void* keys2;
+  uint32_t* vals;
+  uint32_t* vals2;
+  uint32_t* q_anchor;
+  uint32_t* counts;
+  uint32_t* partials;
+};
+// prepare_queries over caller-provided storage: allocates nothing.
+int prepare_queries_raw(const QueryScratch& qs, int n_inst, const GatherDesc* dev_descs,
+                        const int64_t* dev_q_prefix, int64_t total_q, hipStream_t s,
+                        const uint32_t** perm_out, bool need_anchors);
 
 int launch_gather(Context* ctx, const GatherLaunch& L, hipStream_t s);
 int launch_knn_rowsort(const GatherDesc* dev_desc, int64_t m, int32_t* out_idx,

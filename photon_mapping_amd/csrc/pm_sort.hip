@@ -218,11 +218,8 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   }
 }
 
-int exclusive_scan(Workspace* ws, uint32_t* data, int64_t n, hipStream_t s) {
+int exclusive_scan(uint32_t* partial, uint32_t* data, int64_t n, hipStream_t s) {
   const int64_t n_chunks = (n + kScanChunk - 1) / kScanChunk;
-  if (int rc = ws->partials.reserve(static_cast<size_t>(n_chunks + 1) * sizeof(uint32_t)))
-    return rc;
-  uint32_t* partial = ws->partials.as<uint32_t>();
   hipLaunchKernelGGL(scan_reduce_kernel, dim3(n_chunks), dim3(kSortThreads), 0, s, data,
                      n, partial);
   hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kSortThreads), 0, s, partial,
@@ -233,23 +230,22 @@ int exclusive_scan(Workspace* ws, uint32_t* data, int64_t n, hipStream_t s) {
   return PM_OK;
 }
 
+// The sort proper over caller-provided histogram and scan storage
+// (sort_count_words / sort_partial_words words): no allocation.
 template <typename K>
-int radix_sort_impl(Workspace* ws, K* keys, uint32_t* vals, K* keys_alt,
-                    uint32_t* vals_alt, int64_t n, int bits, hipStream_t s,
+int radix_sort_impl(uint32_t* counts, uint32_t* partials, K* keys, uint32_t* vals,
+                    K* keys_alt, uint32_t* vals_alt, int64_t n, int bits, hipStream_t s,
                     int* which, int lo_bit) {
   *which = 0;
   if (n <= 1 || bits <= 0) return PM_OK;
   const int64_t n_tiles = (n + kSortTile - 1) / kSortTile;
-  if (int rc = ws->counts.reserve(static_cast<size_t>(n_tiles) * 256 * sizeof(uint32_t)))
-    return rc;
-  uint32_t* counts = ws->counts.as<uint32_t>();
   K* kb[2] = {keys, keys_alt};
   uint32_t* vb[2] = {vals, vals_alt};
   int cur = 0;
   for (int shift = lo_bit; shift < bits; shift += 8) {
     hipLaunchKernelGGL(digit_hist_kernel<K>, dim3(n_tiles), dim3(kSortThreads), 0, s,
                        kb[cur], n, shift, counts, n_tiles);
-    if (int rc = exclusive_scan(ws, counts, n_tiles * 256, s)) return rc;
+    if (int rc = exclusive_scan(partials, counts, n_tiles * 256, s)) return rc;
     hipLaunchKernelGGL(digit_scatter_kernel<K>, dim3(n_tiles), dim3(kSortThreads), 0, s,
                        kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, shift, counts,
                        n_tiles);
@@ -262,26 +258,49 @@ int radix_sort_impl(Workspace* ws, K* keys, uint32_t* vals, K* keys_alt,
 
 }  // namespace
 
-int sort_reserve(Workspace* ws, int64_t n) {
+int64_t sort_count_words(int64_t n) {
   const int64_t n_tiles = (std::max<int64_t>(n, 1) + kSortTile - 1) / kSortTile;
-  const int64_t n_chunks = (n_tiles * 256 + kScanChunk - 1) / kScanChunk;
-  if (int rc = ws->counts.reserve(static_cast<size_t>(n_tiles) * 256 * sizeof(uint32_t)))
+  return n_tiles * 256;
+}
+
+int64_t sort_partial_words(int64_t n) {
+  return (sort_count_words(n) + kScanChunk - 1) / kScanChunk + 1;
+}
+
+int sort_reserve(Workspace* ws, int64_t n) {
+  if (int rc = ws->counts.reserve(static_cast<size_t>(sort_count_words(n)) * sizeof(uint32_t)))
     return rc;
-  return ws->partials.reserve(static_cast<size_t>(n_chunks + 1) * sizeof(uint32_t));
+  return ws->partials.reserve(static_cast<size_t>(sort_partial_words(n)) * sizeof(uint32_t));
+}
+
+int radix_sort_u32_raw(uint32_t* counts, uint32_t* partials, uint32_t* keys, uint32_t* vals,
+                       uint32_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
+                       hipStream_t s, int* which, int lo_bit) {
+  return radix_sort_impl<uint32_t>(counts, partials, keys, vals, keys_alt, vals_alt, n, bits,
+                                   s, which, lo_bit);
+}
+
+int radix_sort_u64_raw(uint32_t* counts, uint32_t* partials, uint64_t* keys, uint32_t* vals,
+                       uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
+                       hipStream_t s, int* which, int lo_bit) {
+  return radix_sort_impl<uint64_t>(counts, partials, keys, vals, keys_alt, vals_alt, n, bits,
+                                   s, which, lo_bit);
 }
 
 int radix_sort_u32(Workspace* ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                    uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which,
                    int lo_bit) {
-  return radix_sort_impl<uint32_t>(ws, keys, vals, keys_alt, vals_alt, n, bits, s, which,
-                                   lo_bit);
+  if (int rc = sort_reserve(ws, n)) return rc;
+  return radix_sort_u32_raw(ws->counts.as<uint32_t>(), ws->partials.as<uint32_t>(), keys, vals,
+                            keys_alt, vals_alt, n, bits, s, which, lo_bit);
 }
 
 int radix_sort_u64(Workspace* ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                    uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which,
                    int lo_bit) {
-  return radix_sort_impl<uint64_t>(ws, keys, vals, keys_alt, vals_alt, n, bits, s, which,
-                                   lo_bit);
+  if (int rc = sort_reserve(ws, n)) return rc;
+  return radix_sort_u64_raw(ws->counts.as<uint32_t>(), ws->partials.as<uint32_t>(), keys, vals,
+                            keys_alt, vals_alt, n, bits, s, which, lo_bit);
 }
 
 }  // namespace pm

@@ -43,6 +43,8 @@ def load() -> C.CDLL:
         "orc_render": (I64, [I32, VP, I32, VP, VP, I32, VP, VP, VP, I32, I32, I32, I32, I32, I32,
                              I64, VP, VP, VP, C.c_float, I64, VP, VP, VP, C.c_float,
                              C.c_uint64, I32, VP, VP]),
+        "orc_fg_queries": (None, [VP, I32, VP, VP, I32, VP, VP, VP, I32, I32, I32, I32,
+                                  C.c_uint64, I32, VP, VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -194,3 +196,25 @@ def render(mode, tris, mats, ior, le, cam, width, height, spp, max_depth=100, fg
                                 nc, _p(cp), _p(cw), _p(cf), float(n_emit_caustics), int(seed),
                                 _threads(threads), _p(mean), _p(var))
     return mean, var, int(dropped)
+
+
+def fg_queries(tris, mats, ior, le, cam, width, height, spp, depth, seed=0, threads: int = 0,
+               mat_le=None):
+    """CPU restatement of the camera pass's gather query points
+    (render_oracle.cpp orc_fg_queries): (m, 10) float32 records (pos, shading
+    normal, wo, material as int32 bits) and the primary-hit material of each,
+    m = width * height * spp."""
+    t = np.ascontiguousarray(tris)
+    mt = np.ascontiguousarray(mats, np.float32)
+    io = None if ior is None else np.ascontiguousarray(ior, np.float32)
+    l = None if le is None else np.ascontiguousarray(le, np.float32)
+    ml = None if mat_le is None else np.ascontiguousarray(mat_le, np.float32)
+    c = np.ascontiguousarray(cam, np.float32)
+    assert c.shape == (13,)
+    m = width * height * spp
+    out = np.empty((m, 10), np.float32)
+    fg = np.empty(m, np.int32)
+    load().orc_fg_queries(_p(t), len(t), _p(mt), _p(io), mt.shape[0], _p(l), _p(ml), _p(c),
+                          width, height, spp, depth, int(seed), _threads(threads), _p(out),
+                          _p(fg))
+    return out, fg

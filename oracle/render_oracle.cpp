@@ -19,6 +19,10 @@
 //   * orthonormalBasis, sampleCosineHemisphere   include/core.h:224-247,
 //                                                include/sampler.h:84-91
 //   * Scene::sampleLight                         include/scene.h:350-355
+//   * the gather query points of a camera pass: the primary diffuse hit
+//     (include/integrator.h:305-308) or the hit of one cosine-sampled
+//     final-gathering ray followed through specular surfaces (:215-252),
+//     with the benchmark producer's redraw rules (orc_fg_queries)
 // Ray casting is a brute-force two-sided Moller-Trumbore test over every
 // triangle (the reference calls Embree; its hit points are unpinned), with
 // hits in (1e-5, tmax).  The photon-map estimates use the reference kd-tree
@@ -448,6 +452,89 @@ int64_t orc_render(int mode, const void* tris, int n_tri, const void* mats, cons
     }
   }
   return dropped;
+}
+
+// Gather query points of a camera pass, restated for checking the device
+// producer's distribution.  Per sample: draw a camera ray through the pixel
+// (examples/main.cpp:40-41); a miss, a light or a non-Lambert primary hit is
+// redrawn (at most 64 draws, then a placeholder of material -1).  depth 0:
+// the primary hit is the query (include/integrator.h:305-308).  depth 1: up
+// to 16 cosine-sampled final-gathering rays from the primary hit, each
+// followed through at most 17 specular hits sampled as camera paths sample
+// them, until one reaches a Lambert surface (include/integrator.h:215-252).
+// out: m records of 10 floats {pos, shading normal, wo, material as int32
+// bits}; fg_mat (may be null): the primary hit's material, -1 if none.
+void orc_fg_queries(const void* tris, int n_tri, const void* mats, const float* ior, int n_mat,
+                    const float* le, const float* mat_le, const float* cam, int width,
+                    int height, int spp, int depth, uint64_t seed, int nthreads, float* out,
+                    int32_t* fg_mat) {
+  Scene S{static_cast<const Tri*>(tris), n_tri, static_cast<const MaterialRec*>(mats), ior,
+          n_mat, {}, {}};
+  S.tri_le.assign(n_tri, V{0, 0, 0});
+  for (int i = 0; i < n_tri; ++i) {
+    const int m = S.mat_of(i);
+    if (mat_le && (mat_le[3 * m] > 0 || mat_le[3 * m + 1] > 0 || mat_le[3 * m + 2] > 0))
+      S.tri_le[i] = V{mat_le[3 * m], mat_le[3 * m + 1], mat_le[3 * m + 2]};
+    else if (le && S.tris[i].is_light)
+      S.tri_le[i] = V{le[0], le[1], le[2]};
+  }
+  const V pos{cam[0], cam[1], cam[2]}, fwd{cam[3], cam[4], cam[5]}, rgt{cam[6], cam[7], cam[8]},
+      up{cam[9], cam[10], cam[11]};
+  const float focal = cam[12];
+  const int64_t total = static_cast<int64_t>(width) * height * spp;
+  const int threads = resolve_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+  for (int64_t id = 0; id < total; ++id) {
+    const int64_t pix = id / spp;
+    const int i = static_cast<int>(pix / width), j = static_cast<int>(pix % width);
+    Rng rng{seed * 0x9e3779b97f4a7c15ull + static_cast<uint64_t>(id) * 0xd1b54a32d192ed03ull};
+    V qp{0, 0, 0}, qn{0, 1, 0}, qwo{0, 1, 0};
+    int qmat = -1, pmat = -1;
+    for (int attempt = 0; attempt < 64; ++attempt) {
+      const float u = (2.0f * (j + rng.next()) - width) / height;
+      const float v = (2.0f * (i + rng.next()) - height) / height;
+      const V sensor = pos + u * rgt + v * up;
+      const V d0 = normalize(pos + focal * fwd - sensor);
+      const Hit h0 = closest(S, sensor, d0, std::numeric_limits<float>::max());
+      if (h0.tri < 0 || S.tris[h0.tri].is_light || S.is_light(h0.tri)) continue;
+      const Surf s0 = surface(S, sensor, d0, h0);
+      if (S.mats[s0.mat].type != 0) continue;
+      if (depth == 0) {
+        qp = s0.pos; qn = s0.ns; qwo = d0 * -1.0f; qmat = S.tris[h0.tri].material;
+        break;
+      }
+      bool found = false;
+      for (int bounce = 0; bounce < 16 && !found; ++bounce) {
+        float pdf;
+        const V dl = cosine_hemisphere(rng.next(), rng.next(), pdf);
+        V d = to_world(dl, s0);
+        V o = s0.pos;
+        for (int chain = 0; chain <= 16; ++chain) {
+          const Hit h1 = closest(S, o, d, std::numeric_limits<float>::max());
+          if (h1.tri < 0) break;
+          const Surf s1 = surface(S, o, d, h1);
+          if (S.mats[s1.mat].type == 0) {
+            qp = s1.pos; qn = s1.ns; qwo = d * -1.0f; qmat = S.tris[h1.tri].material;
+            pmat = S.tris[h0.tri].material;
+            found = true;
+            break;
+          }
+          V wi;
+          float spdf;
+          (void)sample_bxdf(S, s1, d * -1.0f, FROM_CAMERA, rng, wi, spdf);
+          o = s1.pos;
+          d = wi;
+        }
+      }
+      if (found) break;
+    }
+    float* r = out + 10 * id;
+    r[0] = qp.x; r[1] = qp.y; r[2] = qp.z;
+    r[3] = qn.x; r[4] = qn.y; r[5] = qn.z;
+    r[6] = qwo.x; r[7] = qwo.y; r[8] = qwo.z;
+    std::memcpy(r + 9, &qmat, sizeof(int32_t));
+    if (fg_mat) fg_mat[id] = pmat;
+  }
 }
 
 }  // extern "C"

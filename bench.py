@@ -240,28 +240,14 @@ def batched_bench(ctx, dev, n_inst=64, frames=3, seed=0):
     timed three ways per frame: one plain pm_gather_radiance launch per
     instance, one pm_gather_batched launch, and the prepared plan replayed as
     a captured HIP graph."""
-    import numpy as np
     import torch
 
     import photon_mapping_amd as pm
     from photon_mapping_amd import scene as sc
 
-    rng = np.random.default_rng(seed)
-    inst, sizes = [], []
-    for i in range(n_inst):
-        s = sc.batch_scene_instance(rng)
-        n = int(round(math.exp(rng.uniform(math.log(1e4), math.log(4e6)))))
-        k = int(rng.choice([32, 64, 100, 200]))
-        res = int(rng.choice([128, 256, 512, 1024]))
-        with sc.NativeScene(s, ctx) as ns:
-            pos, wi, flux, n_paths = sc.trace_photons(ns, n, seed=1000 + i)
-            q = sc.camera_queries(ns, res, res, 16, depth=1, seed=1000 + i)
-        pmap = pm.PhotonMap(ctx)
-        pmap.set_photons_soa(pos, wi, flux)
-        pmap.build()
-        del pos, wi, flux
-        inst.append((pmap, q, torch.from_numpy(s.materials()).to(dev), k, float(n_paths)))
-        sizes.append((n, k, res))
+    work = sc.batching_workload(n_inst, seed, ctx=ctx)
+    inst = [(w["map"], w["queries"], w["materials"], w["k"], w["n_emit"]) for w in work]
+    sizes = [(w["n"], w["k"], w["res"]) for w in work]
     total_q = sum(int(q.shape[0]) for _, q, *_ in inst)
     torch.cuda.synchronize()
 

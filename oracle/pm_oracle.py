@@ -43,6 +43,7 @@ def load() -> C.CDLL:
         "orc_render": (I64, [I32, VP, I32, VP, VP, I32, VP, VP, VP, I32, I32, I32, I32, I32, I32,
                              I64, VP, VP, VP, C.c_float, I64, VP, VP, VP, C.c_float,
                              C.c_uint64, I32, VP, VP]),
+        "orc_closest_hits": (None, [VP, I32, I64, VP, VP, C.c_float, I32, I32, VP, VP]),
         "orc_fg_queries": (None, [VP, I32, VP, VP, I32, VP, VP, VP, I32, I32, I32, I32,
                                   C.c_uint64, I32, VP, VP]),
     }
@@ -218,3 +219,17 @@ def fg_queries(tris, mats, ior, le, cam, width, height, spp, depth, seed=0, thre
                           width, height, spp, depth, int(seed), _threads(threads), _p(out),
                           _p(fg))
     return out, fg
+
+
+def closest_hits(tris, org, dirs, brute_force=False, tmax=3.0e38, threads: int = 0):
+    """Closest hits of the render restatement's ray casting (BVH, or the
+    brute-force loop): (triangle int32, -1 on a miss; t float32)."""
+    t = np.ascontiguousarray(tris)
+    o = np.ascontiguousarray(org, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    tri = np.empty(n, np.int32)
+    tt = np.empty(n, np.float32)
+    load().orc_closest_hits(_p(t), len(t), n, _p(o), _p(d), float(tmax), int(bool(brute_force)),
+                            _threads(threads), _p(tri), _p(tt))
+    return tri, tt

@@ -23,9 +23,11 @@
 //     (include/integrator.h:305-308) or the hit of one cosine-sampled
 //     final-gathering ray followed through specular surfaces (:215-252),
 //     with the benchmark producer's redraw rules (orc_fg_queries)
-// Ray casting is a brute-force two-sided Moller-Trumbore test over every
-// triangle (the reference calls Embree; its hit points are unpinned), with
-// hits in (1e-5, tmax).  The photon-map estimates use the reference kd-tree
+// Ray casting is a two-sided Moller-Trumbore test (the reference calls
+// Embree; its hit points are unpinned), with hits in (1e-5, tmax), the
+// smallest t winning and ties going to the lower triangle index; a
+// median-split BVH returns exactly the brute-force loop's hits
+// (orc_closest_hits checks one against the other).  The photon-map estimates use the reference kd-tree
 // and FP32 estimate of oracle_core.hpp over the photon arrays the caller
 // passes.  Random numbers come from this file's own stream per pixel
 // (SplitMix64), independent of the device renderer's, so the two images are
@@ -64,6 +66,8 @@ struct Tri {
 };
 static_assert(sizeof(Tri) == 80, "triangle layout");
 
+struct Bvh;
+
 struct Scene {
   const Tri* tris;
   int nt;
@@ -72,6 +76,7 @@ struct Scene {
   int nm;
   std::vector<V> tri_le;   // emission per triangle (zero: not a light)
   std::vector<int> lights;
+  const Bvh* bvh = nullptr;  // when set, closest() traverses it (same results)
   bool is_light(int t) const { return tri_le[t].x > 0 || tri_le[t].y > 0 || tri_le[t].z > 0; }
   int mat_of(int t) const {
     const int m = tris[t].material;
@@ -97,25 +102,168 @@ struct Hit {
   float u, v;
 };
 
-Hit closest(const Scene& S, V o, V d, float tmax) {
+// The two-sided Moller-Trumbore test of one triangle: t (or -1 on a miss).
+inline float tri_hit(const Tri& tr, V o, V d, float& u_out, float& v_out) {
+  const V p0 = ld(tr.v[0]);
+  const V e1 = ld(tr.v[1]) - p0, e2 = ld(tr.v[2]) - p0;
+  const V pv = cross(d, e2);
+  const float det = dot(e1, pv);
+  if (std::fabs(det) < 1e-12f) return -1.0f;
+  const float inv = 1.0f / det;
+  const V tv = o - p0;
+  const float u = dot(tv, pv) * inv;
+  if (u < 0.0f || u > 1.0f) return -1.0f;
+  const V qv = cross(tv, e1);
+  const float v = dot(d, qv) * inv;
+  if (v < 0.0f || u + v > 1.0f) return -1.0f;
+  u_out = u;
+  v_out = v;
+  return dot(e2, qv) * inv;
+}
+
+// Brute force: every triangle in index order, the smallest t in
+// (1e-5, tmax), ties to the lower index.
+Hit closest_brute(const Scene& S, V o, V d, float tmax) {
   Hit h{tmax, -1, 0.f, 0.f};
   for (int i = 0; i < S.nt; ++i) {
-    const V p0 = ld(S.tris[i].v[0]);
-    const V e1 = ld(S.tris[i].v[1]) - p0, e2 = ld(S.tris[i].v[2]) - p0;
-    const V pv = cross(d, e2);
-    const float det = dot(e1, pv);
-    if (std::fabs(det) < 1e-12f) continue;
-    const float inv = 1.0f / det;
-    const V tv = o - p0;
-    const float u = dot(tv, pv) * inv;
-    if (u < 0.0f || u > 1.0f) continue;
-    const V qv = cross(tv, e1);
-    const float v = dot(d, qv) * inv;
-    if (v < 0.0f || u + v > 1.0f) continue;
-    const float t = dot(e2, qv) * inv;
+    float u, v;
+    const float t = tri_hit(S.tris[i], o, d, u, v);
     if (t > kRayEps && t < h.t) h = Hit{t, i, u, v};
   }
   return h;
+}
+
+// A median-split bounding-volume hierarchy over triangle centroids (leaves
+// of at most 4 triangles), boxes padded by 1e-4 of the scene extent so that
+// they contain every hit the triangle test reports.  Traversal keeps the
+// smallest (t, index) over the triangles of every box the ray meets no later
+// than the best t so far, so it returns the brute-force result exactly.
+struct Bvh {
+  struct Node {
+    float lo[3], hi[3];
+    int left = -1, right = -1;  // children (inner node)
+    int first = 0, count = 0;   // triangle range of idx (leaf when count > 0)
+  };
+  std::vector<Node> nodes;
+  std::vector<int> idx;
+
+  void build(const Tri* tris, int n) {
+    idx.resize(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::vector<V> c(n);
+    float slo[3] = {std::numeric_limits<float>::max(), std::numeric_limits<float>::max(),
+                    std::numeric_limits<float>::max()};
+    float shi[3] = {-slo[0], -slo[1], -slo[2]};
+    for (int i = 0; i < n; ++i) {
+      V a = ld(tris[i].v[0]), b = ld(tris[i].v[1]), e = ld(tris[i].v[2]);
+      c[i] = (a + b + e) / 3.0f;
+      for (const V& p : {a, b, e}) {
+        const float q[3] = {p.x, p.y, p.z};
+        for (int k = 0; k < 3; ++k) {
+          slo[k] = std::min(slo[k], q[k]);
+          shi[k] = std::max(shi[k], q[k]);
+        }
+      }
+    }
+    const float ext = std::max({shi[0] - slo[0], shi[1] - slo[1], shi[2] - slo[2], 1e-6f});
+    pad = 1e-4f * ext + 1e-6f;
+    nodes.reserve(2 * n / 3 + 2);
+    if (n > 0) make(tris, c, 0, n);
+  }
+
+  float pad = 0.0f;
+
+  int make(const Tri* tris, const std::vector<V>& c, int first, int count) {
+    const int id = static_cast<int>(nodes.size());
+    nodes.emplace_back();
+    Node nd;
+    for (int k = 0; k < 3; ++k) {
+      nd.lo[k] = std::numeric_limits<float>::max();
+      nd.hi[k] = -std::numeric_limits<float>::max();
+    }
+    for (int j = first; j < first + count; ++j)
+      for (int vtx = 0; vtx < 3; ++vtx)
+        for (int k = 0; k < 3; ++k) {
+          nd.lo[k] = std::min(nd.lo[k], tris[idx[j]].v[vtx][k]);
+          nd.hi[k] = std::max(nd.hi[k], tris[idx[j]].v[vtx][k]);
+        }
+    for (int k = 0; k < 3; ++k) {
+      nd.lo[k] -= pad;
+      nd.hi[k] += pad;
+    }
+    if (count <= 4) {
+      nd.first = first;
+      nd.count = count;
+      nodes[id] = nd;
+      return id;
+    }
+    int ax = 0;
+    float best = -1.0f;
+    for (int k = 0; k < 3; ++k)
+      if (nd.hi[k] - nd.lo[k] > best) {
+        best = nd.hi[k] - nd.lo[k];
+        ax = k;
+      }
+    const int mid = first + count / 2;
+    auto key = [&](int t) { return ax == 0 ? c[t].x : ax == 1 ? c[t].y : c[t].z; };
+    std::nth_element(idx.begin() + first, idx.begin() + mid, idx.begin() + first + count,
+                     [&](int a, int b) { return key(a) < key(b) || (key(a) == key(b) && a < b); });
+    nodes[id] = nd;
+    const int l = make(tris, c, first, mid - first);
+    const int r = make(tris, c, mid, first + count - mid);
+    nodes[id].left = l;
+    nodes[id].right = r;
+    return id;
+  }
+
+  // Entry distance of the ray into the node's box, or +inf when it misses
+  // the box within [0, tmax].
+  static float enter(const Node& nd, V o, V d, float tmax) {
+    const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    float t0 = 0.0f, t1 = tmax;
+    for (int k = 0; k < 3; ++k) {
+      if (dd[k] == 0.0f) {
+        if (oo[k] < nd.lo[k] || oo[k] > nd.hi[k]) return std::numeric_limits<float>::infinity();
+        continue;
+      }
+      float a = (nd.lo[k] - oo[k]) / dd[k], b = (nd.hi[k] - oo[k]) / dd[k];
+      if (a > b) std::swap(a, b);
+      t0 = std::max(t0, a);
+      t1 = std::min(t1, b);
+    }
+    // a relative slack keeps the entry at or before any hit inside the box
+    t0 = t0 - 1e-5f * std::fabs(t0);
+    return t0 <= t1 * (1.0f + 1e-5f) ? t0 : std::numeric_limits<float>::infinity();
+  }
+
+  Hit closest(const Tri* tris, V o, V d, float tmax) const {
+    Hit h{tmax, -1, 0.f, 0.f};
+    if (nodes.empty()) return h;
+    int stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+      const Node& nd = nodes[stack[--sp]];
+      if (enter(nd, o, d, tmax) > h.t) continue;
+      if (nd.count > 0) {
+        for (int j = nd.first; j < nd.first + nd.count; ++j) {
+          const int i = idx[j];
+          float u, v;
+          const float t = tri_hit(tris[i], o, d, u, v);
+          if (t > kRayEps && t < tmax && (t < h.t || (t == h.t && h.tri >= 0 && i < h.tri)))
+            h = Hit{t, i, u, v};
+        }
+      } else {
+        stack[sp++] = nd.left;
+        stack[sp++] = nd.right;
+      }
+    }
+    return h;
+  }
+};
+
+Hit closest(const Scene& S, V o, V d, float tmax) {
+  return S.bvh ? S.bvh->closest(S.tris, o, d, tmax) : closest_brute(S, o, d, tmax);
 }
 
 struct Surf {
@@ -399,6 +547,9 @@ int64_t orc_render(int mode, const void* tris, int n_tri, const void* mats, cons
       S.tri_le[i] = V{le[0], le[1], le[2]};
     if (S.is_light(i)) S.lights.push_back(i);
   }
+  Bvh bvh;
+  bvh.build(S.tris, n_tri);
+  S.bvh = &bvh;
   KdTreeOracle gt, ct;
   auto build = [](KdTreeOracle& t, int64_t n, const float* p) {
     t.n = static_cast<int>(n);
@@ -478,6 +629,9 @@ void orc_fg_queries(const void* tris, int n_tri, const void* mats, const float* 
     else if (le && S.tris[i].is_light)
       S.tri_le[i] = V{le[0], le[1], le[2]};
   }
+  Bvh bvh;
+  bvh.build(S.tris, n_tri);
+  S.bvh = &bvh;
   const V pos{cam[0], cam[1], cam[2]}, fwd{cam[3], cam[4], cam[5]}, rgt{cam[6], cam[7], cam[8]},
       up{cam[9], cam[10], cam[11]};
   const float focal = cam[12];
@@ -534,6 +688,27 @@ void orc_fg_queries(const void* tris, int n_tri, const void* mats, const float* 
     r[6] = qwo.x; r[7] = qwo.y; r[8] = qwo.z;
     std::memcpy(r + 9, &qmat, sizeof(int32_t));
     if (fg_mat) fg_mat[id] = pmat;
+  }
+}
+
+// Closest hits of n rays (origins and directions as n x 3 floats), by the
+// BVH (method 0) or the brute-force loop (method 1): triangle (-1 on a miss)
+// and t (tmax on a miss).
+void orc_closest_hits(const void* tris, int n_tri, int64_t n, const float* org,
+                      const float* dir, float tmax, int method, int nthreads, int32_t* out_tri,
+                      float* out_t) {
+  Scene S{static_cast<const Tri*>(tris), n_tri, nullptr, nullptr, 0, {}, {}};
+  Bvh bvh;
+  if (method == 0) {
+    bvh.build(S.tris, n_tri);
+    S.bvh = &bvh;
+  }
+  const int threads = resolve_threads(nthreads);
+#pragma omp parallel for schedule(static) num_threads(threads)
+  for (int64_t i = 0; i < n; ++i) {
+    const Hit h = closest(S, ld(org + 3 * i), ld(dir + 3 * i), tmax);
+    out_tri[i] = h.tri;
+    out_t[i] = h.t;
   }
 }
 

@@ -317,6 +317,48 @@ def batch_scene_instance(rng) -> Scene:
     return s
 
 
+def batching_workload(n_inst: int = 64, seed: int = 0, spp: int = 16, ctx=None,
+                      keep_photons: bool = False, photon_range=(1e4, 4e6),
+                      resolutions=(128, 256, 512, 1024), ks=(32, 64, 100, 200)):
+    """The batching configuration: n_inst box-scene instances of uneven size
+    drawn with numpy seed `seed`, in this order per instance: the scene
+    (batch_scene_instance: wall albedos uniform in [0.1, 0.9], 0-3 mirror or
+    glass spheres), the stored photon count (log-uniform over photon_range),
+    k (uniform over ks) and the square resolution (uniform over
+    resolutions).  Photons are traced from the light (seed 1000 + i; the
+    emitted paths are the divisor) and the queries are the final-gathering
+    hits at spp samples per pixel (seed 1000 + i).  Returns one dict per
+    instance: map (built PhotonMap), queries (device records), materials
+    (device records), k, n_emit, n, res, and with keep_photons the host
+    (pos, wi, flux) arrays."""
+    import torch
+
+    from .photon_map import Context, PhotonMap
+
+    ctx = ctx or Context.default()
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_inst):
+        s = batch_scene_instance(rng)
+        lo, hi = math.log(photon_range[0]), math.log(photon_range[1])
+        n = int(round(math.exp(rng.uniform(lo, hi))))
+        k = int(rng.choice(list(ks)))
+        res = int(rng.choice(list(resolutions)))
+        with NativeScene(s, ctx) as ns:
+            pos, wi, flux, n_paths = trace_photons(ns, n, seed=1000 + i)
+            q = camera_queries(ns, res, res, spp, depth=1, seed=1000 + i)
+        pmap = PhotonMap(ctx)
+        pmap.set_photons_soa(pos, wi, flux)
+        pmap.build()
+        rec = dict(map=pmap, queries=q, materials=torch.from_numpy(s.materials()).to(q.device),
+                   k=k, n_emit=float(n_paths), n=n, res=res, scene=s)
+        if keep_photons:
+            rec["photons"] = tuple(a.cpu().numpy() for a in (pos, wi, flux))
+        del pos, wi, flux
+        out.append(rec)
+    return out
+
+
 def _camera_struct(cam: dict) -> pmx_camera:
     c = pmx_camera()
     for k in ("position", "forward", "right", "up"):

@@ -200,3 +200,27 @@ def test_oracle_fg_queries_lie_on_lambert_faces(oracle):
             assert (fg >= 0).all()
         else:
             assert (fg == -1).all()
+
+
+@pytest.mark.parametrize("name", ["box", "caustics", "water"])
+def test_oracle_bvh_equals_brute_force(oracle, name):
+    """The restatement's BVH returns exactly the brute-force loop's hits
+    (triangle and t bits): random rays from inside the box, axis-aligned
+    rays, and rays aimed at triangle vertices (ties between neighbours)."""
+    from photon_mapping_amd import scene as sc
+
+    s = {"box": sc.box_scene, "caustics": sc.caustics_scene,
+         "water": lambda: sc.water_scene(0)}[name]()
+    rng = np.random.default_rng(4)
+    n = 4000
+    org = rng.random((n, 3), dtype=np.float32)
+    d = rng.normal(size=(n, 3))
+    d[: n // 8] = np.eye(3)[rng.integers(0, 3, n // 8)] * rng.choice([-1.0, 1.0], (n // 8, 1))
+    vtx = s.tris["v"][rng.integers(0, len(s.tris), n // 8), rng.integers(0, 3, n // 8)]
+    d[n // 8: n // 4] = vtx - org[n // 8: n // 4]
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    a_tri, a_t = oracle.closest_hits(s.tris, org, d)
+    b_tri, b_t = oracle.closest_hits(s.tris, org, d, brute_force=True)
+    np.testing.assert_array_equal(a_tri, b_tri)
+    np.testing.assert_array_equal(a_t.view(np.uint32), b_t.view(np.uint32))
+    assert (a_tri >= 0).mean() > 0.5

@@ -12,11 +12,18 @@ pm_gather_radiance call over all queries with inputs resident in HBM: Morton
 keys + radix sort of the queries, group anchors, and the persistent gather
 kernel.  Prints one JSON line.
 
-The roofline object prices the gather kernel at its algorithmic bytes per
-query, 40 (query record) + 36 k (position, wi, flux of each neighbour) + 16
-(rgb + r^2) = 3,656 B at k = 100, against 8 TB/s of HBM3E; its duration is the
-HIP-event average of the kernel over the timed steps (stream-synchronous
-timing pass after the throughput pass).  The CPU baseline is the oracle's
+The roofline object describes the gather kernel (gather_kernel<0>) by the
+resource that binds it, VALU issue: its VALU wave-instructions per launch
+(rocprofv3 counter SQ_INSTS_VALU) over its HIP-event duration, against
+1,024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction.  Beside it: the
+HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, each scaled by what it reads
+for a plain 1 GiB copy kernel in the same run) and their fraction of 8 TB/s,
+the L2 hit rate (TCC_HIT / (TCC_HIT + TCC_MISS)), and the algorithmic bytes
+per query (40 + 36 k + 16), which the kernel does not read from HBM once per
+query, so their rate is reported but is no roofline fraction.  The counters
+come from rocprofv3 --pmc child runs of the same workload started by this
+script before it touches the GPU (one counter set per run); --no-pmc falls
+back to the committed profile summaries.  The CPU baseline is the oracle's
 restatement of the reference gather (kd-tree search + FP32 estimate, OpenMP
 over queries) on a bounded sample of the same queries.
 """
@@ -24,10 +31,16 @@ over queries) on a bounded sample of the same queries.
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import math
 import os
+import re
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -46,20 +59,191 @@ def bytes_per_query(k: int) -> int:
     return 40 + 36 * k + 16
 
 
-TRAFFIC_FILE = ROOT / "profiles" / "gather_kernel_radiance_box1M_k100_fg512x64_traffic.md"
+# MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock; a wave64 VALU
+# instruction occupies a 16-lane SIMD for 4 cycles.
+VALU_PEAK_G = 1024 * 2.4 / 4   # G wave-instructions/s
+
+GATHER_KERNEL = re.compile(r"gather_kernel<0>")
+COPY_KERNEL = re.compile(r"neg")
+COPY_BYTES = 1 << 30           # the calibration copy reads and writes 1 GiB
+PMC_PASSES = [
+    ("valu", ["SQ_INSTS_VALU", "SQ_WAVES", "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES"]),
+    ("fetch", ["FETCH_SIZE"]),
+    ("write", ["WRITE_SIZE", "TCC_HIT_sum", "TCC_MISS_sum"]),
+]
+PROFILE_SQ = ROOT / "profiles" / "gather_kernel_radiance_box1M_k100_fg512x64_sq_counters.md"
+PROFILE_TRAFFIC = ROOT / "profiles" / "gather_kernel_radiance_box1M_k100_fg512x64_traffic.md"
 
 
-def traffic_per_launch():
-    """HBM bytes per gather launch measured with rocprofv3 FETCH_SIZE and
-    WRITE_SIZE (scaled by the same counters on a plain copy kernel), read
-    from the committed profile summary; None when it has not been measured."""
+def _read_counter_csv(path):
+    """{(kernel name, dispatch): {counter: value summed over its rows}} from a
+    rocprofv3 counter-collection CSV (columns found by name)."""
+    out = {}
+    with open(path, newline="") as fh:
+        rd = csv.DictReader(fh)
+        cols = rd.fieldnames or []
+
+        def col(*names):
+            for nm in names:
+                for c in cols:
+                    if c.lower() == nm.lower():
+                        return c
+            raise KeyError(f"{names} not among {cols}")
+
+        kn = col("Kernel_Name", "Kernel-Name")
+        cn = col("Counter_Name", "Counter-Name")
+        cv = col("Counter_Value", "Counter-Value")
+        di = col("Dispatch_Id", "Dispatch-Id", "Correlation_Id")
+        for row in rd:
+            key = (row[kn], int(row[di]))
+            d = out.setdefault(key, {})
+            d[row[cn]] = d.get(row[cn], 0.0) + float(row[cv])
+    return out
+
+
+def _per_dispatch(rows, kernel_re, counter, skip_first=0):
+    vals = [v[counter] for (kn, _), v in sorted(rows.items(), key=lambda kv: kv[0][1])
+            if kernel_re.search(kn) and counter in v]
+    vals = vals[skip_first:] if len(vals) > skip_first else vals
+    return sum(vals) / len(vals) if vals else None
+
+
+def run_pmc_passes(args, save_dir=None):
+    """rocprofv3 --pmc child runs of this script's gather workload (one
+    counter set per run, before this process touches the GPU).  Returns the
+    per-launch counter means of gather_kernel<0> and the copy calibration,
+    or a dict with an 'error'."""
+    rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not Path(rocprof).exists():
+        return {"error": "rocprofv3 not found"}
+    child = [sys.executable, str(ROOT / "bench.py"), "--pmc-child",
+             "--photons", str(args.photons), "--k", str(args.k), "--width", str(args.width),
+             "--height", str(args.height), "--spp", str(args.spp), "--leaf", str(args.leaf),
+             "--photon-source", args.photon_source, "--split", str(args.split),
+             "--waves", str(args.waves)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    res = {}
+    for name, counters in PMC_PASSES:
+        d = tempfile.mkdtemp(prefix=f"pm_pmc_{name}_", dir="/tmp")
+        cmd = [rocprof, "--pmc", *counters, "-f", "csv", "-d", d, "-o", "run", "--", *child]
+        log(f"pmc pass {name}: {' '.join(counters)}")
+        t0 = time.perf_counter()
+        try:
+            proc = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE,
+                                  stderr=subprocess.STDOUT, text=True, timeout=180)
+        except subprocess.TimeoutExpired:
+            return {"error": f"pmc pass {name} timed out"}
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            tail = "\n".join(proc.stdout.splitlines()[-5:])
+            return {"error": f"pmc pass {name}: rc {proc.returncode}, no counter CSV: {tail}"}
+        if save_dir:
+            os.makedirs(save_dir, exist_ok=True)
+            shutil.copy(files[0], os.path.join(save_dir, f"{name}_counter_collection.csv"))
+        rows = _read_counter_csv(files[0])
+        shutil.rmtree(d, ignore_errors=True)
+        for c in counters:
+            # the first gather dispatch also warms caches; the launches after
+            # it are the measured kind
+            res[c] = _per_dispatch(rows, GATHER_KERNEL, c, skip_first=1)
+            if c in ("FETCH_SIZE", "WRITE_SIZE"):
+                res[c + "_copy"] = _per_dispatch(rows, COPY_KERNEL, c)
+        res[name + "_dispatches"] = sum(1 for (kn, _) in rows if GATHER_KERNEL.search(kn))
+        log(f"pmc pass {name}: {time.perf_counter() - t0:.1f} s")
+    return res
+
+
+def pmc_from_profiles():
+    """The committed counter summaries of the same workload (when the live
+    passes are off or fail)."""
+    res = {}
     try:
-        for line in TRAFFIC_FILE.read_text().splitlines():
+        for line in PROFILE_SQ.read_text().splitlines():
+            m = re.match(r"\|\s*(SQ_INSTS_VALU)\s*\|\s*([0-9.e+]+)", line)
+            if m:
+                res[m.group(1)] = float(m.group(2))
+        for line in PROFILE_TRAFFIC.read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
+                res["traffic_bytes"] = float(line.split(":", 1)[1])
+            if line.startswith("l2_hit_rate:"):
+                res["l2_hit_rate"] = float(line.split(":", 1)[1])
     except OSError:
         pass
-    return None
+    return res
+
+
+def pmc_child(args) -> int:
+    """The gather workload alone, for a counter pass: the inputs, the map,
+    one call that also computes the radius bounds, two more calls, and three
+    plain 1 GiB copies (torch.neg) that calibrate FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    import photon_mapping_amd as pm
+    from photon_mapping_amd import _lib
+    from photon_mapping_amd import scene as sc
+
+    ctx = pm.Context.default()
+    s = sc.box_scene()
+    ns = sc.NativeScene(s)
+    if args.photon_source == "traced":
+        pos, wi, flux, n_emit = sc.trace_photons(ns, args.photons, seed=0)
+    else:
+        pos, wi, flux = sc.surrogate_photons(s, args.photons, seed=0)
+        n_emit = args.photons
+    q = sc.camera_queries(ns, args.width, args.height, args.spp, depth=1, seed=0)
+    if args.split > 0:
+        ctx.set_option(_lib.PM_OPT_RUN_SPLIT, int(round(args.split * 1000)))
+    if args.waves > 0:
+        ctx.set_option(_lib.PM_OPT_WAVES_PER_CU, args.waves)
+    pmap = pm.PhotonMap(ctx)
+    pmap.set_photons_soa(pos, wi, flux)
+    pmap.build(args.leaf)
+    mats = torch.from_numpy(s.materials()).cuda()
+    for _ in range(3):
+        pmap.gather_radiance(q, mats, args.k, float(n_emit))
+    torch.cuda.synchronize()
+    a = torch.ones((COPY_BYTES // 4,), dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print("pmc child done", flush=True)
+    return 0
+
+
+def roofline(k, m, gather_ms, pmc):
+    """The roofline object of gather_kernel<0> from its HIP-event time and
+    the counters (see the module docstring)."""
+    sec = gather_ms * 1e-3
+    out = {"bound": "valu", "kernel": "gather_kernel<0>", "kernel_ms": gather_ms,
+           "unit": "G VALU wave-instructions/s", "peak": VALU_PEAK_G}
+    valu = pmc.get("SQ_INSTS_VALU")
+    out["valu_instructions_per_launch"] = valu
+    out["achieved"] = valu / sec / 1e9 if valu else None
+    out["frac"] = out["achieved"] / VALU_PEAK_G if valu else None
+    traffic = pmc.get("traffic_bytes")
+    if traffic is None and pmc.get("FETCH_SIZE") and pmc.get("WRITE_SIZE"):
+        f_scale = COPY_BYTES / (pmc["FETCH_SIZE_copy"] * 1024) if pmc.get("FETCH_SIZE_copy") else 1.0
+        w_scale = COPY_BYTES / (pmc["WRITE_SIZE_copy"] * 1024) if pmc.get("WRITE_SIZE_copy") else 1.0
+        traffic = pmc["FETCH_SIZE"] * 1024 * f_scale + pmc["WRITE_SIZE"] * 1024 * w_scale
+        out["traffic_fetch_bytes"] = pmc["FETCH_SIZE"] * 1024 * f_scale
+        out["traffic_write_bytes"] = pmc["WRITE_SIZE"] * 1024 * w_scale
+        out["counter_scale"] = {"FETCH_SIZE": f_scale, "WRITE_SIZE": w_scale}
+    out["traffic"] = traffic
+    out["hbm"] = {"achieved_GBs": traffic / sec / 1e9 if traffic else None,
+                  "peak_GBs": HBM_PEAK_GBS,
+                  "frac": traffic / sec / 1e9 / HBM_PEAK_GBS if traffic else None}
+    hit, miss = pmc.get("TCC_HIT_sum"), pmc.get("TCC_MISS_sum")
+    out["l2_hit_rate"] = (hit / (hit + miss) if hit is not None and miss is not None
+                          and hit + miss > 0 else pmc.get("l2_hit_rate"))
+    bq = bytes_per_query(k)
+    out["algorithmic"] = {
+        "bytes_per_query": bq, "rate_GBs": m * bq / sec / 1e9,
+        "note": "40 + 36 k + 16 B per query over the kernel time; the kernel reads each "
+                "candidate photon once per run of up to 64 queries, so this is not a "
+                "fraction of the HBM roofline"}
+    out["counters_source"] = pmc.get("source")
+    return out
 
 
 def cpu_baseline(pos, wi, flux, q, mats, k, n_emit, budget_s=12.0, m_total=None):
@@ -152,14 +336,18 @@ def plain_config_bench(ctx, with_cpu, budget_s=10.0):
     return out
 
 
-def water_diff_bench(ctx, dev, k=100, reps=3):
-    """The differentiation configuration's gather on the water scene (131,084
-    triangles, glass IOR 1.33): 1,000,000 traced global photons (seed 0),
-    10,000,000 emitted caustic paths (their map built and sized), and the
-    final-gathering queries of 512 x 512 x 64 spp with the primary hit's
-    material (the final-gathering albedo factor of the adjoint).  Times the
-    forward estimate and the adjoint in both modes (keys reused, kNN
-    recomputed) with HIP events, like the main measurement."""
+def water_diff_bench(ctx, dev, k=100, reps=3, frame=True):
+    """The differentiation configuration on the water scene (131,072-triangle
+    height field, glass IOR 1.33, walls cut into 64 albedo patches, 66
+    materials): 1,000,000 traced global photons (seed 0) and the caustics map
+    of 10,000,000 emitted caustic paths.  The forward pass is the two
+    estimates of a 512 x 512 x 64 frame: the global map at the
+    final-gathering hits (times the primary-hit albedo) and the caustics map
+    at the primary diffuse hits (include/integrator.h:155-175, 239-244, 314);
+    the adjoint runs both, with the forward keys reused and with the kNN
+    recomputed.  HIP-event times of whole calls, like the main measurement.
+    With frame, also the rendered photon-mapping frame of the scene (both
+    maps, final gathering, k = 100)."""
     import numpy as np
     import torch
 
@@ -168,66 +356,133 @@ def water_diff_bench(ctx, dev, k=100, reps=3):
     from photon_mapping_amd import scene as sc
 
     s = sc.water_scene(0)
-    out = {"workload": "water scene (131,084 triangles), 1M global photons, 10M caustic paths, "
-                       f"k={k}, 512x512x64 final-gathering queries"}
-    with sc.NativeScene(s, ctx) as ns:
-        t0 = time.perf_counter()
-        pos, wi, flux, n_emit = sc.trace_photons(ns, 1_000_000, seed=0)
-        torch.cuda.synchronize()
-        out["global_trace_ms"] = 1e3 * (time.perf_counter() - t0)
-        out["global_paths"] = int(n_emit)
-        t0 = time.perf_counter()
-        cmap, n_c = sc.trace_photon_map(ns, 10_000_000, seed=0, kind="caustics")
-        cmap.build()
-        torch.cuda.synchronize()
-        out["caustics_trace_build_ms"] = 1e3 * (time.perf_counter() - t0)
-        out["caustic_photons"] = int(n_c)
-        q, fg = sc.camera_queries(ns, 512, 512, 64, depth=1, seed=0, want_fg_mat=True)
+    out = {"workload": f"water scene ({len(s.tris)} triangles, {len(s.albedo)} materials: 64 "
+                       f"wall patches), 1M global photons, 10M caustic paths, k={k}, "
+                       "512x512x64 final-gathering queries + 512x512x64 caustics lookups"}
+    ns = sc.NativeScene(s, ctx)
+    t0 = time.perf_counter()
+    pos, wi, flux, n_emit = sc.trace_photons(ns, 1_000_000, seed=0)
+    torch.cuda.synchronize()
+    out["global_trace_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["global_paths"] = int(n_emit)
+    t0 = time.perf_counter()
+    n_cpaths = 10_000_000
+    cmap, n_c = sc.trace_photon_map(ns, n_cpaths, seed=0, kind="caustics")
+    cmap.build()
+    torch.cuda.synchronize()
+    out["caustics_trace_build_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["caustic_photons"] = int(n_c)
+    q, fg = sc.camera_queries(ns, 512, 512, 64, depth=1, seed=0, want_fg_mat=True)
+    qp = sc.camera_queries(ns, 512, 512, 64, depth=0, seed=1)
     pmap = pm.PhotonMap(ctx)
     pmap.set_photons_soa(pos, wi, flux)
     pmap.build()
     mats = torch.from_numpy(s.materials()).to(dev)
+    nm = mats.shape[0]
     m = int(q.shape[0])
     stream = torch.cuda.current_stream(dev)
     sptr = _lib.C.c_void_p(stream.cuda_stream)
-    rgb = torch.empty((m, 3), dtype=torch.float32, device=dev)
-    r2 = torch.empty((m,), dtype=torch.float32, device=dev)
-    key = torch.empty((m,), dtype=torch.int64, device=dev)
-    g = torch.randn((m, 3), dtype=torch.float32, device=dev,
-                    generator=torch.Generator(device=dev).manual_seed(0))
-    dflux = torch.zeros((3, 1_000_000), dtype=torch.float32, device=dev)
-    dalb = torch.zeros((mats.shape[0], 3), dtype=torch.float32, device=dev)
+    gen = torch.Generator(device=dev).manual_seed(0)
+    bufs = {}
+    for name, mp, qq, n_ph in (("global", pmap, q, 1_000_000), ("caustics", cmap, qp, n_c)):
+        bufs[name] = dict(
+            map=mp, q=qq, rgb=torch.empty((m, 3), dtype=torch.float32, device=dev),
+            r2=torch.empty((m,), dtype=torch.float32, device=dev),
+            key=torch.empty((m,), dtype=torch.int64, device=dev),
+            g=torch.randn((m, 3), dtype=torch.float32, device=dev, generator=gen),
+            dflux=torch.zeros((3, max(int(n_ph), 1)), dtype=torch.float32, device=dev))
+    bufs["global"]["n_emit"], bufs["caustics"]["n_emit"] = float(n_emit), float(n_cpaths)
+    bufs["global"]["fg"], bufs["caustics"]["fg"] = fg, None
+    dalb = torch.zeros((nm, 3), dtype=torch.float32, device=dev)
 
-    def fwd():
-        _lib.call("pm_gather_radiance", pmap.handle, m, _lib.ptr(q), _lib.ptr(mats),
-                  mats.shape[0], k, float(n_emit), _lib.ptr(rgb), _lib.ptr(r2), _lib.ptr(key),
+    def fwd(b):
+        _lib.call("pm_gather_radiance", b["map"].handle, m, _lib.ptr(b["q"]), _lib.ptr(mats),
+                  nm, k, b["n_emit"], _lib.ptr(b["rgb"]), _lib.ptr(b["r2"]), _lib.ptr(b["key"]),
                   _lib.PM_MEM_DEVICE, sptr)
 
-    def adj(kk):
-        _lib.call("pm_gather_adjoint", pmap.handle, m, _lib.ptr(q), _lib.ptr(mats),
-                  mats.shape[0], k, float(n_emit), _lib.ptr(g), _lib.ptr(kk), _lib.ptr(fg),
-                  _lib.ptr(dflux), _lib.ptr(dalb), _lib.PM_MEM_DEVICE, sptr)
+    def adj(b, reuse):
+        _lib.call("pm_gather_adjoint", b["map"].handle, m, _lib.ptr(b["q"]), _lib.ptr(mats), nm,
+                  k, b["n_emit"], _lib.ptr(b["g"]), _lib.ptr(b["key"]) if reuse else None,
+                  _lib.ptr(b["fg"]), _lib.ptr(b["dflux"]), _lib.ptr(dalb), _lib.PM_MEM_DEVICE,
+                  sptr)
 
-    fwd()
-    adj(key)
+    def call_ms(fn):
+        fn()
+        st = ctx.stats()
+        return st["sort_ms"] + st["gather_ms"] + st["post_ms"]
+
+    for b in bufs.values():
+        fwd(b)
+        adj(b, True)
     torch.cuda.synchronize()
     ctx.set_option(_lib.PM_OPT_TIMING, 1)
-    times = {"forward": [], "reuse": [], "recompute": []}
+    times = {f"{n}_{w}": [] for n in bufs for w in ("forward", "reuse", "recompute")}
     for _ in range(reps):
-        fwd()
-        st = ctx.stats()
-        times["forward"].append(st["sort_ms"] + st["gather_ms"] + st["post_ms"])
-        for name, kk in (("reuse", key), ("recompute", None)):
-            adj(kk)
-            st = ctx.stats()
-            times[name].append(st["sort_ms"] + st["gather_ms"] + st["post_ms"])
+        for name, b in bufs.items():
+            times[f"{name}_forward"].append(call_ms(lambda: fwd(b)))
+            times[f"{name}_reuse"].append(call_ms(lambda: adj(b, True)))
+            times[f"{name}_recompute"].append(call_ms(lambda: adj(b, False)))
     ctx.set_option(_lib.PM_OPT_TIMING, 0)
-    f = float(np.mean(times["forward"]))
-    out.update({"forward_ms": f, "forward_queries_per_s": m / f * 1e3,
-                "adjoint_reuse_ms": float(np.mean(times["reuse"])),
-                "adjoint_recompute_ms": float(np.mean(times["recompute"]))})
+    t = {kk: float(np.mean(v)) for kk, v in times.items()}
+    out.update({f"{kk}_ms": v for kk, v in t.items()})
+    f = t["global_forward"] + t["caustics_forward"]
+    out["forward_ms"] = f
+    out["forward_queries_per_s"] = 2 * m / f * 1e3
+    out["adjoint_reuse_ms"] = t["global_reuse"] + t["caustics_reuse"]
+    out["adjoint_recompute_ms"] = t["global_recompute"] + t["caustics_recompute"]
     out["fwd_plus_adj_over_fwd_reuse"] = (f + out["adjoint_reuse_ms"]) / f
     out["fwd_plus_adj_over_fwd_recompute"] = (f + out["adjoint_recompute_ms"]) / f
+    if frame:
+        kw = dict(k=k, caustics_map=cmap, n_emit_caustics=float(n_cpaths),
+                  final_gathering_depth=1, seed=0)
+        sc.render_photon_mapping(ns, pmap, float(n_emit), 64, 64, 2, **kw)  # warm
+        torch.cuda.synchronize()
+        img, _, st = sc.render_photon_mapping(ns, pmap, float(n_emit), 512, 512, 64, **kw)
+        torch.cuda.synchronize()
+        out["frame"] = _frame_row(st, img, 512, 512, 64, k)
+    ns.close()
+    return out
+
+
+def _frame_row(st, img, w, h, spp, k):
+    return {"resolution": f"{w}x{h}", "spp": spp, "k": k, "ms": st["ms"],
+            "frames_per_s": 1e3 / st["ms"], "mrays_per_s": st["rays"] / st["ms"] / 1e3,
+            "gathers_global": st["gathers_global"], "gathers_caustics": st["gathers_caustics"],
+            "gathers_per_s": (st["gathers_global"] + st["gathers_caustics"]) / st["ms"] * 1e3,
+            "rounds": st["iterations"], "batches": st["batches"],
+            "dropped_samples": st["dropped_samples"],
+            "image_mean": [float(x) for x in img.double().mean((0, 1)).cpu()]}
+
+
+def caustics_frame_bench(ctx, k=100):
+    """The caustics configuration's frame: the box walls and light with a
+    mirror and an IOR-1.5 glass icosphere (5120 triangles each), 1,000,000
+    emitted global paths and 10,000,000 emitted caustic paths (every stored
+    photon kept, the reference's count semantics), 512 x 512 at 100 spp,
+    final gathering, k = 100 for both maps.  Reports the trace + build time
+    of the maps and the frame time."""
+    import torch
+
+    from photon_mapping_amd import scene as sc
+
+    s = sc.caustics_scene()
+    out = {"workload": "caustics scene (two 5120-triangle icospheres), 1M global + 10M caustic "
+                       f"paths, k={k}, 512x512x100 spp, final gathering"}
+    with sc.NativeScene(s, ctx) as ns:
+        t0 = time.perf_counter()
+        gmap, n_g = sc.trace_photon_map(ns, 1_000_000, seed=0)
+        gmap.build()
+        cmap, n_c = sc.trace_photon_map(ns, 10_000_000, seed=0, kind="caustics")
+        cmap.build()
+        torch.cuda.synchronize()
+        out["maps_trace_build_ms"] = 1e3 * (time.perf_counter() - t0)
+        out["global_photons"], out["caustic_photons"] = int(n_g), int(n_c)
+        kw = dict(k=k, caustics_map=cmap, n_emit_caustics=1e7, final_gathering_depth=1, seed=0)
+        sc.render_photon_mapping(ns, gmap, 1e6, 64, 64, 2, **kw)  # warm
+        torch.cuda.synchronize()
+        img, _, st = sc.render_photon_mapping(ns, gmap, 1e6, 512, 512, 100, **kw)
+        torch.cuda.synchronize()
+        out["frame"] = _frame_row(st, img, 512, 512, 100, k)
     return out
 
 
@@ -308,7 +563,7 @@ def ray_bench(ctx, dev, seed=0):
     uniform points in the unit box, isotropic directions; HIP events on the
     null stream) and caustic photons traced/s (1M stored), per scene: the box
     (36 triangles), the caustics scene (two 5120-triangle icospheres) and the
-    water height field (131,084 triangles)."""
+    water scene (131,202 triangles: the height field, 64 wall patches, the light)."""
     import torch
 
     from photon_mapping_amd import _lib
@@ -427,12 +682,32 @@ def main() -> int:
                     help="also run a plain 1 GiB read + 1 GiB write kernel (torch.neg) three "
                          "times, the calibration of the HBM traffic counters")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the rocprofv3 counter passes (the roofline then reads the "
+                         "committed profile summaries)")
+    ap.add_argument("--pmc-dir", default="",
+                    help="also keep the counter passes' CSV files in this directory")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the caustics-scene and water-scene frames")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the gather shards "
               "by query ranges across GPUs but no multi-GPU path is built.", file=sys.stderr)
         return 2
+    if args.pmc_child:
+        return pmc_child(args)
+
+    # counter passes first: child processes, before this one touches the GPU
+    pmc = {"error": "skipped (--no-pmc)"} if args.no_pmc else run_pmc_passes(args, args.pmc_dir)
+    if "error" in pmc:
+        log(f"counter passes: {pmc['error']}; reading the committed profile summaries")
+        reason = pmc["error"]
+        pmc = pmc_from_profiles()
+        pmc["source"] = f"committed profiles/ summaries ({reason})"
+    else:
+        pmc["source"] = "rocprofv3 --pmc child runs of this workload in this invocation"
 
     import numpy as np
     import torch
@@ -574,7 +849,7 @@ def main() -> int:
         adj["fwd_plus_adj_over_fwd_reuse"] = (fwd_ms + adj["adjoint_reuse_ms"]) / fwd_ms
         adj["fwd_plus_adj_over_fwd_recompute"] = (fwd_ms + adj["adjoint_recompute_ms"]) / fwd_ms
         log(f"adjoint: {adj}")
-        adj["water_scene"] = water_diff_bench(ctx, dev, k)
+        adj["water_scene"] = water_diff_bench(ctx, dev, k, frame=not args.no_configs)
         log(f"adjoint (water scene): {adj['water_scene']}")
 
     batched = None
@@ -589,6 +864,8 @@ def main() -> int:
     if not args.no_render:
         render = render_bench(ns, pmap, n_emit, args.width, args.height, args.spp, k)
         render["plain_config"] = plain_config_bench(ctx, not args.no_cpu)
+        if not args.no_configs:
+            render["caustics_config"] = caustics_frame_bench(ctx, k)
         log(f"render: {render}")
 
     tuned = None
@@ -610,8 +887,6 @@ def main() -> int:
         torch.cuda.synchronize()
         log("copy calibration: 3 x torch.neg over 2^28 float32 (1 GiB read + 1 GiB write)")
         del a, b
-    bq = bytes_per_query(k)
-    achieved = m * bq / (gather_ms * 1e-3) / 1e9
     log(f"step {ms_per_step:.3f} ms | gather kernel {gather_ms:.3f} ms | sort+anchors "
         f"{sort_ms:.3f} ms | build {build_ms:.2f} ms | work {work}")
 
@@ -648,17 +923,7 @@ def main() -> int:
             "leaf_size": args.leaf or 8,
             "parallelism": "1 GPU",
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_per_launch(),
-            "kernel": "gather_kernel<0>",
-            "kernel_ms": gather_ms,
-            "bytes_per_query": bq,
-        },
+        "roofline": roofline(k, m, gather_ms, pmc),
         "cpu_baseline": cpu,
         "detail": {"sort_ms": sort_ms, "build_ms": build_ms, **work, **(adj or {}),
                    "batched": batched, "autotune": tuned, "tracing": tracing, "rays": rays,

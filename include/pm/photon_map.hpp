@@ -34,6 +34,27 @@ struct Vec3f {
   Vec3f(float x, float y, float z) : v{x, y, z} {}
   float operator[](int i) const { return v[i]; }
   float& operator[](int i) { return v[i]; }
+  // component-wise arithmetic of the reference's Vec3 (include/core.h:114-218)
+  Vec3f operator+(const Vec3f& o) const { return {v[0] + o.v[0], v[1] + o.v[1], v[2] + o.v[2]}; }
+  Vec3f operator-(const Vec3f& o) const { return {v[0] - o.v[0], v[1] - o.v[1], v[2] - o.v[2]}; }
+  Vec3f operator*(const Vec3f& o) const { return {v[0] * o.v[0], v[1] * o.v[1], v[2] * o.v[2]}; }
+  Vec3f operator*(float s) const { return {v[0] * s, v[1] * s, v[2] * s}; }
+  Vec3f operator/(float s) const { return {v[0] / s, v[1] / s, v[2] / s}; }
+  Vec3f& operator+=(const Vec3f& o) {
+    for (int i = 0; i < 3; ++i) v[i] += o.v[i];
+    return *this;
+  }
+};
+inline Vec3f operator*(float s, const Vec3f& a) { return a * s; }
+
+// Two floats with operator[] (the reference's Vec2f, include/core.h:25-112).
+struct Vec2f {
+  float v[2];
+  Vec2f() : v{0.f, 0.f} {}
+  Vec2f(float x) : v{x, x} {}
+  Vec2f(float x, float y) : v{x, y} {}
+  float operator[](int i) const { return v[i]; }
+  float& operator[](int i) { return v[i]; }
 };
 
 struct Error : std::runtime_error {
@@ -166,8 +187,13 @@ class BasicPhotonMap {
 
 }  // namespace pm
 
+#ifndef PM_VEC2F_TYPE
+#define PM_VEC2F_TYPE pm::Vec2f
+#define PM_VEC2F_IS_DEFAULT 1
+#endif
 #ifndef PM_VEC3F_TYPE
 #define PM_VEC3F_TYPE pm::Vec3f
+#define PM_VEC3F_IS_DEFAULT 1
 #endif
 
 #ifndef PM_NO_GLOBAL_NAMES

@@ -3,11 +3,12 @@
 // PhotonMapping integrators (include/integrator.h) and Image
 // (include/image.h), backed by libpm_gather.so.
 //
-// Same class names, method names and argument meaning, with one change of
-// shape: the reference renders by calling Integrator::integrate(ray, scene,
-// sampler) for every pixel sample from an OpenMP loop (examples/main.cpp:
-// 33-70); here Integrator::render(scene, camera, width, height, n_samples,
-// seed) runs that whole loop on the GPU and returns the averaged Image
+// Same class names, method names and argument meaning.  The reference
+// renders by calling Integrator::integrate(ray, scene, sampler) for every
+// pixel sample from an OpenMP loop (examples/main.cpp:33-70); that call is
+// here (one ray through the device renderer per call, serialised: the slow
+// path), and Integrator::render(scene, camera, width, height, n_samples,
+// seed) runs the whole loop on the GPU and returns the averaged Image
 // (dropped NaN / negative samples and the division by n_samples as in the
 // reference's loop).  Scene::loadModel reads OBJ + MTL with the reference's
 // material semantics; Scene::build uploads the scene and builds its BVH.
@@ -22,8 +23,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <fstream>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -51,6 +55,164 @@ template <typename V>
 struct BasicRay {
   V origin;
   V direction;
+  BasicRay() = default;
+  BasicRay(const V& origin, const V& direction) : origin(origin), direction(direction) {}
+  V operator()(float t) const {
+    V p;
+    for (int i = 0; i < 3; ++i) p[i] = origin[i] + t * direction[i];
+    return p;
+  }
+};
+
+namespace detail {
+// PCG32 with the XSH-RR output function (M. E. O'Neill, "PCG: A family of
+// simple fast space-efficient statistically good algorithms for random
+// number generation", 2014): 64-bit LCG state, increment 1, the way the
+// reference's RNG seeds it (include/sampler.h:27-48).
+struct Pcg32 {
+  uint64_t state = 1;
+  uint32_t next() {
+    const uint64_t old = state;
+    state = old * 6364136223846793005ull + 1ull;
+    const uint32_t xs = static_cast<uint32_t>(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = static_cast<uint32_t>(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+  }
+};
+
+// Every device call of the per-ray path goes through one lock: the
+// renderer's workspaces belong to the scene's context.
+inline std::mutex& device_mutex() {
+  static std::mutex m;
+  return m;
+}
+}  // namespace detail
+
+// Sampler interface (include/sampler.h:51-66): a seeded random stream of
+// floats in [0, 1].
+template <typename V2>
+class BasicSampler {
+ public:
+  BasicSampler() = default;
+  explicit BasicSampler(uint64_t seed) { rng.state = seed; }
+  virtual ~BasicSampler() = default;
+  uint64_t getSeed() const { return rng.state; }
+  void setSeed(uint64_t seed) { rng.state = seed; }
+  virtual std::unique_ptr<BasicSampler> clone() const = 0;
+  virtual float getNext1D() = 0;
+  virtual V2 getNext2D() = 0;
+
+ protected:
+  detail::Pcg32 rng;
+};
+
+// UniformSampler (include/sampler.h:68-80): u32 * (1 / (2^32 - 1)).
+template <typename V2>
+class BasicUniformSampler : public BasicSampler<V2> {
+ public:
+  BasicUniformSampler() = default;
+  BasicUniformSampler(uint64_t seed) : BasicSampler<V2>(seed) {}
+  std::unique_ptr<BasicSampler<V2>> clone() const override {
+    return std::make_unique<BasicUniformSampler>();
+  }
+  float getNext1D() override {
+    constexpr float divider = 1.0f / 4294967295.0f;
+    return static_cast<float>(this->rng.next()) * divider;
+  }
+  V2 getNext2D() override {
+    const float a = getNext1D();
+    const float b = getNext1D();
+    return V2(a, b);
+  }
+};
+
+// Surface point of a light sample (include/core.h:268-276 SurfaceInfo, the
+// members a light sample fills).
+template <typename V, typename V2>
+struct BasicSurfaceInfo {
+  V position;
+  V geometricNormal;
+  V shadingNormal;  // interpolated, not renormalised
+  V dpdu;
+  V dpdv;
+  V2 barycentric;
+};
+
+// AreaLight (include/light.h:17-45): one emitting triangle of constant
+// radiance on both sides, sampled uniformly by area, emitting
+// cosine-distributed about its shading normal.
+template <typename V, typename V2>
+class BasicAreaLight {
+ public:
+  using SurfaceInfo = BasicSurfaceInfo<V, V2>;
+  BasicAreaLight(const pmx_triangle& tri, const float le[3]) : tri(tri) {
+    for (int i = 0; i < 3; ++i) this->le[i] = le[i];
+    float e1[3], e2[3], c[3];
+    for (int a = 0; a < 3; ++a) {
+      e1[a] = tri.v[1][a] - tri.v[0][a];
+      e2[a] = tri.v[2][a] - tri.v[0][a];
+    }
+    detail::cross3(e1, e2, c);
+    area = 0.5f * std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  }
+  V Le(const SurfaceInfo&, const V&) const { return le; }
+  float surfaceArea() const { return area; }
+
+  // Triangle::samplePoint (include/triangle.h:84-114): barycentric
+  // (1 - sqrt(u0), u1 sqrt(u0)), pdf 1 / area
+  SurfaceInfo samplePoint(BasicSampler<V2>& sampler, float& pdf) const {
+    const V2 uv = sampler.getNext2D();
+    const float su0 = std::sqrt(uv[0]);
+    const float b0 = 1.0f - su0, b1 = uv[1] * su0;
+    SurfaceInfo s;
+    float ng[3], e1[3], e2[3], n[3], t[3], b[3];
+    for (int a = 0; a < 3; ++a) {
+      s.position[a] = (1.0f - b0 - b1) * tri.v[0][a] + b0 * tri.v[1][a] + b1 * tri.v[2][a];
+      n[a] = (1.0f - b0 - b1) * tri.n[0][a] + b0 * tri.n[1][a] + b1 * tri.n[2][a];
+      s.shadingNormal[a] = n[a];
+      e1[a] = tri.v[1][a] - tri.v[0][a];
+      e2[a] = tri.v[2][a] - tri.v[0][a];
+    }
+    detail::cross3(e1, e2, ng);
+    detail::normalize3(ng);
+    basis(n, t, b);
+    for (int a = 0; a < 3; ++a) {
+      s.geometricNormal[a] = ng[a];
+      s.dpdu[a] = t[a];
+      s.dpdv[a] = b[a];
+    }
+    s.barycentric = V2(b0, b1);
+    pdf = 1.0f / area;
+    return s;
+  }
+
+  // AreaLight::sampleDirection: cosine about the shading normal, pdf cos / pi
+  V sampleDirection(const SurfaceInfo& s, BasicSampler<V2>& sampler, float& pdf) const {
+    const V2 uv = sampler.getNext2D();
+    const float theta = 0.5f * std::acos(std::min(1.0f, std::max(-1.0f, 1.0f - 2.0f * uv[0])));
+    const float phi = 2.0f * kPi * uv[1];
+    const float l[3] = {std::cos(phi) * std::sin(theta), std::cos(theta),
+                        std::sin(phi) * std::sin(theta)};
+    pdf = std::cos(theta) / kPi;
+    V d;
+    for (int a = 0; a < 3; ++a)
+      d[a] = l[0] * s.dpdu[a] + l[1] * s.shadingNormal[a] + l[2] * s.dpdv[a];
+    return d;
+  }
+
+ private:
+  // orthonormalBasis (include/core.h:224-231)
+  static void basis(const float n[3], float t[3], float b[3]) {
+    const float up[3] = {0.f, 1.f, 0.f}, back[3] = {0.f, 0.f, -1.f};
+    detail::cross3(n, std::fabs(n[1]) < 0.9f ? up : back, t);
+    detail::normalize3(t);
+    detail::cross3(t, n, b);
+    detail::normalize3(b);
+  }
+
+  pmx_triangle tri;
+  V le;
+  float area = 0.0f;
 };
 
 // Pinhole camera (include/camera.h:10-47): right = normalize(forward x y),
@@ -72,8 +234,10 @@ class BasicCamera {
     cam.focal = 1.0f / std::tan(0.5f * FOV);
   }
 
-  // uv in [-1, 1]^2 on the sensor; pdf 1 (include/camera.h:41-47).
-  bool sampleRay(const float uv[2], BasicRay<V>& ray, float& pdf) const {
+  // uv in [-1, 1]^2 on the sensor (a Vec2f, or any two floats with
+  // operator[]); pdf 1 (include/camera.h:41-47).
+  template <typename UV>
+  bool sampleRay(const UV& uv, BasicRay<V>& ray, float& pdf) const {
     float s[3], d[3];
     for (int i = 0; i < 3; ++i) {
       s[i] = cam.position[i] + uv[0] * cam.right[i] + uv[1] * cam.up[i];
@@ -284,6 +448,31 @@ class Scene {
           "pmx_scene_intersect");
   }
 
+  // Scene::sampleLight (include/scene.h:350-355): one of the emitting
+  // triangles chosen uniformly, pdf 1 / (number of lights).
+  template <typename V = PM_VEC3F_TYPE, typename V2>
+  std::shared_ptr<BasicAreaLight<V, V2>> sampleLight(BasicSampler<V2>& sampler,
+                                                     float& pdf) const {
+    const std::vector<int> lights = lightFaces();
+    if (lights.empty()) throw Error(PM_ERR_INVALID_ARG, "Scene::sampleLight: no light");
+    unsigned int idx = static_cast<unsigned int>(lights.size() * sampler.getNext1D());
+    if (idx == lights.size()) --idx;
+    pdf = 1.0f / static_cast<float>(lights.size());
+    const pmx_triangle& t = tris[lights[idx]];
+    return std::make_shared<BasicAreaLight<V, V2>>(t, &mat_le[3 * t.material]);
+  }
+
+  // the emitting triangles: those whose material has a positive Ke
+  // (include/scene.h:41-52, 249-265)
+  std::vector<int> lightFaces() const {
+    std::vector<int> out;
+    for (size_t f = 0; f < tris.size(); ++f) {
+      const float* le = &mat_le[3 * tris[f].material];
+      if (le[0] > 0.f || le[1] > 0.f || le[2] > 0.f) out.push_back(static_cast<int>(f));
+    }
+    return out;
+  }
+
   uint32_t nVertices() const { return static_cast<uint32_t>(3 * tris.size()); }
   uint32_t nFaces() const { return static_cast<uint32_t>(tris.size()); }
   const std::vector<pmx_triangle>& triangles() const { return tris; }
@@ -392,25 +581,23 @@ class BasicImage {
   std::vector<float> pixels;
 };
 
-// The seed source of an integrator's random streams (the reference's
-// UniformSampler, include/sampler.h:61-80).
-class UniformSampler {
- public:
-  explicit UniformSampler(uint64_t seed = 0) : seed(seed) {}
-  uint64_t getSeed() const { return seed; }
-  void setSeed(uint64_t s) { seed = s; }
-
- private:
-  uint64_t seed;
-};
-
-template <typename V>
+template <typename V, typename V2 = PM_VEC2F_TYPE>
 class BasicIntegrator {
  public:
   using Image = BasicImage<V>;
   using Camera = BasicCamera<V>;
+  using Ray = BasicRay<V>;
+  using Sampler = BasicSampler<V2>;
   virtual ~BasicIntegrator() = default;
-  virtual void build(const Scene& scene, UniformSampler& sampler) = 0;
+  // Integrator::build (include/integrator.h:13): the photon maps' random
+  // streams start from sampler.getSeed()
+  virtual void build(const Scene& scene, Sampler& sampler) = 0;
+  // Integrator::integrate (include/integrator.h:16-17): one radiance sample
+  // along the ray.  Each call runs one ray through the device renderer (its
+  // random stream seeded from two of the sampler's numbers) and waits for
+  // it; calls from several threads take turns.  A sample the renderer drops
+  // (NaN, negative) comes back as 0, which the reference's loop skips.
+  virtual V integrate(const Ray& ray, const Scene& scene, Sampler& sampler) const = 0;
   // every pixel sample of the reference's render loop, averaged
   virtual Image render(const Scene& scene, const Camera& camera, int width, int height,
                        int n_samples, uint64_t seed = 0) const = 0;
@@ -427,20 +614,50 @@ class BasicIntegrator {
     p.out_mem = PM_MEM_HOST;
     return p;
   }
+
+  static V integrate_one(const Scene& scene, int integrator, pm_photonmap* global,
+                         pm_photonmap* caustics, pmx_render_params p, const Ray& ray,
+                         Sampler& sampler) {
+    uint32_t w[2];
+    for (uint32_t& x : w) {
+      const float f = sampler.getNext1D();
+      std::memcpy(&x, &f, sizeof(x));
+    }
+    p.seed = (static_cast<uint64_t>(w[0]) << 32) | w[1];
+    p.out_mem = PM_MEM_HOST;
+    const float r[6] = {ray.origin[0],    ray.origin[1],    ray.origin[2],
+                        ray.direction[0], ray.direction[1], ray.direction[2]};
+    float rgb[3] = {0.f, 0.f, 0.f};
+    {
+      std::lock_guard<std::mutex> lock(detail::device_mutex());
+      check(pmx_integrate_rays(scene.handle(), integrator, global, caustics, &p, 1, r, rgb),
+            "pmx_integrate_rays");
+    }
+    V out;
+    for (int i = 0; i < 3; ++i) out[i] = rgb[i];
+    return out;
+  }
 };
 
 // PathTracing (include/integrator.h:49-101).
-template <typename V>
-class BasicPathTracing : public BasicIntegrator<V> {
+template <typename V, typename V2 = PM_VEC2F_TYPE>
+class BasicPathTracing : public BasicIntegrator<V, V2> {
  public:
-  using typename BasicIntegrator<V>::Image;
-  using typename BasicIntegrator<V>::Camera;
+  using typename BasicIntegrator<V, V2>::Image;
+  using typename BasicIntegrator<V, V2>::Camera;
+  using typename BasicIntegrator<V, V2>::Ray;
+  using typename BasicIntegrator<V, V2>::Sampler;
   explicit BasicPathTracing(int maxDepth = 100) : maxDepth(maxDepth) {}
-  void build(const Scene&, UniformSampler&) override {}
+  void build(const Scene&, Sampler&) override {}
+  V integrate(const Ray& ray, const Scene& scene, Sampler& sampler) const override {
+    return this->integrate_one(scene, PMX_INTEGRATOR_PATH_TRACING, nullptr, nullptr,
+                               this->params(1, 1, 1, maxDepth, 0), ray, sampler);
+  }
   Image render(const Scene& scene, const Camera& camera, int width, int height, int n_samples,
                uint64_t seed = 0) const override {
     Image img(width, height);
     const pmx_render_params p = this->params(width, height, n_samples, maxDepth, seed);
+    std::lock_guard<std::mutex> lock(detail::device_mutex());
     check(pmx_render_path_tracing(scene.handle(), &camera.desc(), &p, img.data().data(), nullptr,
                                   nullptr),
           "pmx_render_path_tracing");
@@ -452,11 +669,13 @@ class BasicPathTracing : public BasicIntegrator<V> {
 };
 
 // PhotonMapping (include/integrator.h:104-563).
-template <typename V>
-class BasicPhotonMapping : public BasicIntegrator<V> {
+template <typename V, typename V2 = PM_VEC2F_TYPE>
+class BasicPhotonMapping : public BasicIntegrator<V, V2> {
  public:
-  using typename BasicIntegrator<V>::Image;
-  using typename BasicIntegrator<V>::Camera;
+  using typename BasicIntegrator<V, V2>::Image;
+  using typename BasicIntegrator<V, V2>::Camera;
+  using typename BasicIntegrator<V, V2>::Ray;
+  using typename BasicIntegrator<V, V2>::Sampler;
   BasicPhotonMapping(int nPhotonsGlobal, int nEstimationGlobal, float nPhotonsCausticsMultiplier,
                      int nEstimationCaustics, int strictCalcDepth, int maxDepth)
       : nPhotonsGlobal(nPhotonsGlobal),
@@ -470,8 +689,9 @@ class BasicPhotonMapping : public BasicIntegrator<V> {
   ~BasicPhotonMapping() override { release(); }
 
   // photon tracing of both maps and their builds (include/integrator.h:393-563)
-  void build(const Scene& scene, UniformSampler& sampler) override {
+  void build(const Scene& scene, Sampler& sampler) override {
     release();
+    std::lock_guard<std::mutex> lock(detail::device_mutex());
     check(pmx_trace_photon_map(scene.handle(), PMX_PHOTONS_GLOBAL, nPhotonsGlobal, maxDepth,
                                sampler.getSeed(), &global, &nGlobalStored),
           "pmx_trace_photon_map");
@@ -484,19 +704,34 @@ class BasicPhotonMapping : public BasicIntegrator<V> {
     }
   }
 
+  // PhotonMapping::integrate (include/integrator.h:566-569)
+  V integrate(const Ray& ray, const Scene& scene, Sampler& sampler) const override {
+    if (!global) throw Error(PM_ERR_NOT_BUILT, "PhotonMapping used before build()");
+    return this->integrate_one(scene, PMX_INTEGRATOR_PHOTON_MAPPING, global, caustics,
+                               frameParams(1, 1, 1, 0), ray, sampler);
+  }
+
   Image render(const Scene& scene, const Camera& camera, int width, int height, int n_samples,
                uint64_t seed = 0) const override {
     if (!global) throw Error(PM_ERR_NOT_BUILT, "PhotonMapping used before build()");
     Image img(width, height);
+    const pmx_render_params p = frameParams(width, height, n_samples, seed);
+    std::lock_guard<std::mutex> lock(detail::device_mutex());
+    check(pmx_render_photon_mapping(scene.handle(), &camera.desc(), global, caustics, &p,
+                                    img.data().data(), nullptr, nullptr),
+          "pmx_render_photon_mapping");
+    return img;
+  }
+
+  // the render parameters of this integrator (what render() passes to
+  // pmx_render_photon_mapping), for callers of the C ABI
+  pmx_render_params frameParams(int width, int height, int n_samples, uint64_t seed) const {
     pmx_render_params p = this->params(width, height, n_samples, maxDepth, seed);
     p.final_gathering_depth = finalGatheringDepth;
     p.k = nEstimationGlobal;  // both maps, as include/integrator.h:139, 161 do
     p.n_emit_global = nPhotonsGlobal;
     p.n_emit_caustics = nPhotonsCaustics;
-    check(pmx_render_photon_mapping(scene.handle(), &camera.desc(), global, caustics, &p,
-                                    img.data().data(), nullptr, nullptr),
-          "pmx_render_photon_mapping");
-    return img;
+    return p;
   }
 
   // the device photon maps (the reference's getPhotonMapPtr returns the global one)
@@ -526,15 +761,25 @@ class BasicPhotonMapping : public BasicIntegrator<V> {
 }  // namespace pm
 
 #ifndef PM_NO_GLOBAL_NAMES
+#ifdef PM_VEC2F_IS_DEFAULT
+using Vec2f = pm::Vec2f;
+#endif
+#ifdef PM_VEC3F_IS_DEFAULT
+using Vec3f = pm::Vec3f;
+constexpr float PI = pm::kPi;  // include/core.h:7
+#endif
 using Ray = pm::BasicRay<PM_VEC3F_TYPE>;
 using IntersectInfo = pm::BasicIntersectInfo<PM_VEC3F_TYPE>;
+using SurfaceInfo = pm::BasicSurfaceInfo<PM_VEC3F_TYPE, PM_VEC2F_TYPE>;
+using AreaLight = pm::BasicAreaLight<PM_VEC3F_TYPE, PM_VEC2F_TYPE>;
 using Camera = pm::BasicCamera<PM_VEC3F_TYPE>;
 using Scene = pm::Scene;
 using Image = pm::BasicImage<PM_VEC3F_TYPE>;
-using UniformSampler = pm::UniformSampler;
-using Integrator = pm::BasicIntegrator<PM_VEC3F_TYPE>;
-using PathTracing = pm::BasicPathTracing<PM_VEC3F_TYPE>;
-using PhotonMapping = pm::BasicPhotonMapping<PM_VEC3F_TYPE>;
+using Sampler = pm::BasicSampler<PM_VEC2F_TYPE>;
+using UniformSampler = pm::BasicUniformSampler<PM_VEC2F_TYPE>;
+using Integrator = pm::BasicIntegrator<PM_VEC3F_TYPE, PM_VEC2F_TYPE>;
+using PathTracing = pm::BasicPathTracing<PM_VEC3F_TYPE, PM_VEC2F_TYPE>;
+using PhotonMapping = pm::BasicPhotonMapping<PM_VEC3F_TYPE, PM_VEC2F_TYPE>;
 #endif
 
 #endif  // PM_RENDER_HPP

@@ -18,6 +18,15 @@
  *     r^2 = 0 (include/photon_map.h:112, 173-174);
  *   - photon indices are positions in the array handed to the map
  *     (include/photon_map.h:195-200).
+ *
+ * Domain: photon and query coordinates must be finite and within +-2^60
+ * (every squared distance is then finite in FP32).  pm_photonmap_build
+ * rejects a map outside it with PM_ERR_INVALID_ARG.  A query outside it (a
+ * NaN or infinite coordinate, or one beyond +-2^60) is left out of the
+ * gather and gets r^2 = NaN, radiance NaN, key = all ones, a kNN row of -1
+ * and no adjoint contribution (the reference's radiance is NaN there too,
+ * and its render loop drops such a sample, examples/main.cpp:49-56); every
+ * other query of the call is unaffected.
  */
 #ifndef PM_GATHER_H
 #define PM_GATHER_H
@@ -30,7 +39,7 @@ extern "C" {
 
 /* ---- status codes ------------------------------------------------------- */
 #define PM_OK 0
-#define PM_ERR_INVALID_ARG 1 /* bad size, null pointer, NaN input, k < 1   */
+#define PM_ERR_INVALID_ARG 1 /* bad size, null pointer, bad photons, k < 1 */
 #define PM_ERR_HIP 2         /* a HIP runtime call failed                  */
 #define PM_ERR_OOM 3         /* device allocation failed                   */
 #define PM_ERR_NOT_BUILT 4   /* query on a map whose build() has not run   */
@@ -145,7 +154,8 @@ int pm_photonmap_get_photon(pm_photonmap* map, int64_t i, float* out9);
  * Batched PhotonMap::queryKNearestPhotons (include/photon_map.h:208-211).
  * q_soa: 3*m floats [x.., y.., z..].  out_idx: m rows of k int32, each row
  * farthest first; slots past min(k, N) hold -1.  out_r2: m floats.
- * k <= 1024. */
+ * Any k (rows of more than 8192 are sorted through a device scratch of
+ * m * 8 * (the power of two >= k) bytes). */
 int pm_knn(pm_photonmap* map, int64_t m, const float* q_soa, int k,
            int32_t* out_idx, float* out_r2, int mem, void* stream);
 

@@ -199,6 +199,25 @@ int pmx_render_path_tracing(const pmx_scene* scene, const pmx_camera* cam,
                             const pmx_render_params* params, float* out_mean,
                             float* out_var_opt, pmx_render_stats* stats_opt);
 
+/* The reference's per-ray Integrator::integrate(ray, scene, sampler)
+ * (include/integrator.h:11-18, 60-105, 566-569) for n rays.  The radiance
+ * arriving along ray i (rays: 6 floats per ray, origin then direction) is one
+ * sample of PathTracing::integrate or PhotonMapping::integrate, drawn from the
+ * random stream (params->seed, i).  params gives max_depth and, for photon
+ * mapping, final_gathering_depth, k and n_emit_global / n_emit_caustics
+ * (width, height and spp are ignored; batch_pixels bounds the rays per
+ * wavefront batch as it bounds pixels).  A sample whose radiance is NaN,
+ * negative or beyond 2^30 gives 0: the reference's render loop skips such a
+ * sample and still divides by the sample count (examples/main.cpp:49-56,
+ * 67).  rays and out_rgb (3n floats) are host or device memory as
+ * params->out_mem says.  The maps must be built (caustics_map may be NULL
+ * when final_gathering_depth is 0).  Synchronous. */
+#define PMX_INTEGRATOR_PATH_TRACING 0
+#define PMX_INTEGRATOR_PHOTON_MAPPING 1
+int pmx_integrate_rays(const pmx_scene* scene, int integrator, pm_photonmap* global_map,
+                       pm_photonmap* caustics_map, const pmx_render_params* params, int64_t n,
+                       const float* rays, float* out_rgb);
+
 /* Surrogate photon array: n photons uniform by area over the triangles,
  * wi cosine-distributed about the face normal, flux = power * albedo of the
  * face's material * (0.5 + 0.5 u).  tris, mat_albedo (n_mat*3 floats) and

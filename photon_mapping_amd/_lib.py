@@ -126,6 +126,7 @@ SIGNATURES = {
                                      VP]),
     "pmx_render_photon_mapping": (None, [VP, VP, VP, VP, VP, VP, VP, VP]),
     "pmx_render_path_tracing": (None, [VP, VP, VP, VP, VP, VP]),
+    "pmx_integrate_rays": (None, [VP, I32, VP, VP, VP, I64, VP, VP]),
 }
 
 # Symbols declared in include/*.h (checked by the CPU tests).

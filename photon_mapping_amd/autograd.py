@@ -12,8 +12,10 @@ are piecewise constant in them).  A final-gathering bounce is an ordinary
 torch product outside: ``albedo[fg_mat] * gather_radiance(...)``, whose
 albedo gradient torch forms itself.
 
-The map holds the flux of the last forward pass; a backward pass must run
-before the map's flux is replaced again.
+The map holds the flux of the last forward pass.  When it has been replaced
+since (another forward pass on the same map, an update_flux call), the
+backward pass uploads the flux its forward pass used, runs the adjoint, and
+puts the newer flux back, so every backward pass sees its own forward flux.
 """
 
 from __future__ import annotations
@@ -35,22 +37,31 @@ class _GatherRadiance(torch.autograd.Function):
     @staticmethod
     def forward(ctx, flux, albedo, pmap: PhotonMap, queries, types, k: int, n_emit: float,
                 recompute: bool):
-        pmap.update_flux(flux.detach().to(torch.float32).contiguous())
+        fl = flux.detach().to(torch.float32).contiguous()
+        pmap.update_flux(fl)
         mats = _materials(albedo, types)
         rgb, _, key = pmap.gather_radiance(queries, mats, k, n_emit, want_key=not recompute)
         ctx.pmap = pmap
         ctx.k = k
         ctx.n_emit = n_emit
-        ctx.save_for_backward(queries, mats, key if key is not None else torch.empty(0))
+        ctx.flux_version = pmap.flux_version
+        ctx.save_for_backward(queries, mats, key if key is not None else torch.empty(0), fl)
         ctx.recompute = recompute
         return rgb
 
     @staticmethod
     def backward(ctx, g_rgb):
-        queries, mats, key = ctx.saved_tensors
-        dflux, dalb = ctx.pmap.gather_adjoint(
+        queries, mats, key, fl = ctx.saved_tensors
+        pmap = ctx.pmap
+        newer = None
+        if pmap.flux_version != ctx.flux_version:
+            newer = pmap._flux_ref
+            pmap.update_flux(fl)
+        dflux, dalb = pmap.gather_adjoint(
             queries, mats, ctx.k, ctx.n_emit, g_rgb.to(torch.float32).contiguous(),
             key=None if ctx.recompute else key)
+        if newer is not None:
+            pmap.update_flux(newer)
         return dflux.t(), dalb, None, None, None, None, None, None
 
 

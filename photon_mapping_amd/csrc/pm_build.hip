@@ -272,7 +272,11 @@ __global__ __launch_bounds__(kThreads) void query_keys_kernel(
   } else {
     x = D.q_soa[li]; y = D.q_soa[D.m + li]; z = D.q_soa[2 * D.m + li];
   }
-  const uint32_t code = hilbert3(x, y, z, D.qlo, D.qscale);
+  // A query outside the supported domain (a non-finite coordinate, or one
+  // beyond +-2^60) sorts after every valid query of its instance and is
+  // left out of the gather (its outputs say so).
+  const bool ok = fabsf(x) <= kCoordLimit && fabsf(y) <= kCoordLimit && fabsf(z) <= kCoordLimit;
+  const uint32_t code = ok ? hilbert3(x, y, z, D.qlo, D.qscale) : kInvalidKeyBit;
   keys[i] = static_cast<K>((static_cast<uint64_t>(inst) << 32) | code);
   vals[i] = static_cast<uint32_t>(i);
 }
@@ -292,6 +296,10 @@ __global__ __launch_bounds__(kThreads) void query_anchor_kernel(
   const int inst = n_inst > 1 ? find_inst(q_prefix, n_inst, i) : 0;
   const GatherDesc& D = descs[inst];
   const uint32_t code = static_cast<uint32_t>(sorted_keys[i] & 0xffffffffu);
+  if (code & kInvalidKeyBit) {  // no anchor for a query the gather leaves out
+    q_anchor[i] = 0;
+    return;
+  }
   // The upper bound of the code lies inside its prefix bucket of the table:
   // anchors before the bucket have smaller codes, those after it larger.
   int lo = 0, hi = 0;
@@ -310,18 +318,22 @@ __global__ __launch_bounds__(kThreads) void query_anchor_kernel(
 
 // ---- kNN row sort: neighbour indices, farthest first ------------------------------
 // One wave per query row: recompute each neighbour's (d^2, index) key from
-// the original-order positions, bitonic-sort the row descending in LDS, and
-// write the indices back (slots past the set hold -1).
+// the original-order positions, bitonic-sort the row descending, and write
+// the indices back (slots past the set hold -1).  The row's keys live in
+// LDS (p2 = the power of two >= k, up to kRowSortLds keys) or, for longer
+// rows, in a global scratch of p2 keys per row (kGlobal).
+template <bool kGlobal>
 __global__ __launch_bounds__(kWave) void knn_rowsort_kernel(const GatherDesc* __restrict__ dp,
-                                                            int32_t* __restrict__ out_idx) {
-  __shared__ uint64_t key[kMaxKnnK];
+                                                            int32_t* __restrict__ out_idx,
+                                                            uint64_t* __restrict__ scratch,
+                                                            int p2) {
+  extern __shared__ uint64_t lds_key[];
   const GatherDesc& D = *dp;
   const int64_t q = blockIdx.x;
+  uint64_t* key = kGlobal ? scratch + q * p2 : lds_key;
   const int lane = threadIdx.x;
   const int k = D.k;
   const int64_t n = D.n;
-  int p2 = 64;
-  while (p2 < k) p2 <<= 1;
   const float qx = D.q_soa[q], qy = D.q_soa[D.m + q], qz = D.q_soa[2 * D.m + q];
   int32_t* row = out_idx + q * k;
   for (int j = lane; j < p2; j += kWave) {
@@ -469,6 +481,9 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
   uint32_t nbad;
   std::memcpy(&nbad, &hb[6], sizeof(nbad));
   if (nbad != 0) return fail(PM_ERR_INVALID_ARG, "photon positions must be finite");
+  for (int c = 0; c < 6; ++c)
+    if (!(std::fabs(hb[c]) <= kCoordLimit))
+      return fail(PM_ERR_INVALID_ARG, "photon coordinates must lie within +-2^60");
   for (int c = 0; c < 3; ++c) {
     const float ext = hb[3 + c] - hb[c];
     map->qlo[c] = hb[c];
@@ -593,18 +608,29 @@ int launch_anchor_bounds(Context* ctx, const GatherDesc* dev_desc, const int64_t
   return launch_gather(ctx, L, s);
 }
 
-int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** out) {
+int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** out,
+                   uint64_t pin_epoch) {
   *out = map->anchors.as<float4>();
   if (map->n == 0 || k_eff >= map->n) return PM_OK;  // every photon is a neighbour
-  if (const float4* hit = cached_anchors(map, k_eff)) {
-    *out = hit;
-    return PM_OK;
-  }
-  PhotonMapImpl::AnchorSet* slot = &map->aset[0];
+  for (auto& a : map->aset)
+    if (a.k == k_eff) {
+      a.stamp = ++map->anchor_clock;
+      if (pin_epoch) a.pin = pin_epoch;
+      *out = a.buf.as<float4>();
+      return PM_OK;
+    }
+  // the least recently used set not pinned by this batch (an empty one first)
+  PhotonMapImpl::AnchorSet* slot = nullptr;
   for (auto& a : map->aset) {
-    if (a.k < 0 || a.stamp < slot->stamp) slot = &a;
+    if (pin_epoch && a.pin == pin_epoch) continue;
+    if (!slot || a.k < 0 || (slot->k >= 0 && a.stamp < slot->stamp)) slot = &a;
     if (slot->k < 0) break;
   }
+  if (!slot) {
+    map->aset.emplace_back();
+    slot = &map->aset.back();
+  }
+  slot->pin = pin_epoch;
   Context* ctx = map->ctx;
   const size_t bytes = static_cast<size_t>(map->n_anchors) * sizeof(float4);
   slot->k = -1;
@@ -677,8 +703,8 @@ int prepare_queries_raw(const QueryScratch& qs, int n_inst, const GatherDesc* de
     hipLaunchKernelGGL(query_keys_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
                        dev_q_prefix, n_inst, total_q, k0, v0);
     PM_HIP(hipGetLastError());
-    if (int rc = radix_sort_u32_raw(qs.counts, qs.partials, k0, v0, k1, v1, total_q,
-                                    kCodeBits, s, &which, kQuerySortLoBit))
+    if (int rc = radix_sort_u32_raw(qs.counts, qs.partials, k0, v0, k1, v1, total_q, 32, s,
+                                    &which, kQuerySortLoBit))
       return rc;
     if (need_anchors)
       hipLaunchKernelGGL(query_anchor_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
@@ -703,10 +729,20 @@ int prepare_queries_raw(const QueryScratch& qs, int n_inst, const GatherDesc* de
   return PM_OK;
 }
 
-int launch_knn_rowsort(const GatherDesc* dev_desc, int64_t m, int32_t* out_idx,
-                       hipStream_t s) {
+int launch_knn_rowsort(const GatherDesc* dev_desc, int64_t m, int k, int32_t* out_idx,
+                       DevBuf* scratch, hipStream_t s) {
   if (m <= 0) return PM_OK;
-  hipLaunchKernelGGL(knn_rowsort_kernel, dim3(m), dim3(kWave), 0, s, dev_desc, out_idx);
+  int p2 = kWave;
+  while (p2 < k) p2 <<= 1;
+  if (p2 <= kRowSortLds) {
+    hipLaunchKernelGGL(knn_rowsort_kernel<false>, dim3(m), dim3(kWave),
+                       static_cast<unsigned>(p2 * sizeof(uint64_t)), s, dev_desc, out_idx,
+                       nullptr, p2);
+  } else {
+    if (int rc = scratch->reserve(static_cast<size_t>(m) * p2 * sizeof(uint64_t))) return rc;
+    hipLaunchKernelGGL(knn_rowsort_kernel<true>, dim3(m), dim3(kWave), 0, s, dev_desc, out_idx,
+                       scratch->as<uint64_t>(), p2);
+  }
   PM_HIP(hipGetLastError());
   return PM_OK;
 }

@@ -258,7 +258,8 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
   tm.mark(2);
   if (mode == kModeKnn) {
     if (n_inst != 1) return fail(PM_ERR_UNSUPPORTED, "kNN rows of one instance only");
-    if (int rc = launch_knn_rowsort(dd, total_q, knn_out_idx, s)) return rc;
+    if (int rc = launch_knn_rowsort(dd, total_q, ds[0].k, knn_out_idx, &ctx->ws_slots, s))
+      return rc;
   }
   tm.mark(3);
   ctx->stats.groups = total_groups;
@@ -324,6 +325,9 @@ struct ZeroFill {
 // anchor set of each instance's k, exactly as a single call builds it.
 int batch_descriptors(Context* ctx, int n_inst, const pm_instance* inst, hipStream_t s,
                       std::vector<GatherDesc>& ds, std::vector<ZeroFill>& fills) {
+  // the anchor sets this batch uses stay cached until its descriptors are
+  // built, however many k values it asks of one map
+  const uint64_t epoch = ++ctx->batch_epoch;
   for (int i = 0; i < n_inst; ++i) {
     const pm_instance& in = inst[i];
     if (int rc = check_map(in.map)) return rc;
@@ -354,7 +358,7 @@ int batch_descriptors(Context* ctx, int n_inst, const pm_instance* inst, hipStre
     }
     const int k_eff = static_cast<int>(std::min<int64_t>(in.k, map->n));
     const float4* anchors = nullptr;
-    if (int rc = ensure_anchors(map, k_eff, s, &anchors)) return rc;
+    if (int rc = ensure_anchors(map, k_eff, s, &anchors, epoch)) return rc;
     GatherDesc d{};
     fill_desc_map_part(map, &d);
     d.anchors = anchors;
@@ -704,7 +708,7 @@ int pm_knn(pm_photonmap* h, int64_t m, const float* q_soa, int k, int32_t* out_i
   auto* map = reinterpret_cast<PhotonMapImpl*>(h);
   Context* ctx = map->ctx;
   if (k < 1) return fail(PM_ERR_INVALID_ARG, "k must be >= 1");
-  if (k > kMaxKnnK) return fail(PM_ERR_UNSUPPORTED, "pm_knn supports k <= 1024");
+  if (k > (1 << 30)) return fail(PM_ERR_UNSUPPORTED, "pm_knn supports k <= 2^30");
   if (m < 0) return fail(PM_ERR_INVALID_ARG, "negative query count");
   if (m == 0) return PM_OK;
   if (!q_soa || !out_idx) return fail(PM_ERR_INVALID_ARG, "null query or output");

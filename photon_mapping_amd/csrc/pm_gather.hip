@@ -1061,7 +1061,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
                               GatherShared& sh, float* drho, int lane, uint32_t* counters,
                               int count_stats, float split, PhaseTimer& pt) {
   const int64_t qloc = lg * kWave + lane;
-  const bool active = qloc < D.m;
+  const bool in_range = qloc < D.m;
   const uint32_t k_eff = static_cast<uint32_t>(D.k_eff);
   pt.mark(7, lane);
 
@@ -1082,14 +1082,14 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
   float4 anq[kAn];
   if (!reuse && !all) {
     const float4* __restrict__ A = D.anchors;
-    const int32_t center = static_cast<int32_t>(D.q_anchor[active ? qloc : lg * kWave]);
+    const int32_t center = static_cast<int32_t>(D.q_anchor[in_range ? qloc : lg * kWave]);
     const int32_t a0 = max(center - kAnchorWindow, 0);
     const int32_t a1 = min(center + kAnchorWindow, D.n_anchors - 1);
 #pragma unroll
     for (int o = 0; o < kAn; ++o) anq[o] = A[min(max(center + o - kAnchorWindow, a0), a1)];
   }
   {
-    const int64_t sp = D.q_base + (active ? qloc : lg * kWave);
+    const int64_t sp = D.q_base + (in_range ? qloc : lg * kWave);
     qi = static_cast<int64_t>(perm[sp]) - D.q_base;
     if (MODE == kModeKnn) {
       qx = D.q_soa[qi];
@@ -1108,6 +1108,11 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
       }
     }
   }
+  // Queries outside the supported domain (a non-finite coordinate or one
+  // beyond +-2^60) were sorted after the valid ones of their instance, so
+  // the active lanes of a group stay a prefix of it.
+  const bool active = in_range && fabsf(qx) <= kCoordLimit && fabsf(qy) <= kCoordLimit &&
+                      fabsf(qz) <= kCoordLimit;
 
   // ---- radius bound: r0sq >= the lane's k-th FP32 d^2 --------------------------
   uint64_t tau_in = 0;
@@ -1588,7 +1593,21 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
     }
   }
 
-  if (MODE == kModeAdjoint || !active) return;
+  if (MODE == kModeAdjoint || !in_range) return;
+  if (!active) {
+    // a query outside the domain: no neighbour set, r^2 and radiance NaN
+    const float nan = __uint_as_float(0x7fc00000u);
+    if (D.out_r2) D.out_r2[qi] = nan;
+    if (D.out_key) D.out_key[qi] = ~0ull;
+    if (kRows) {
+      for (int32_t j = 0; j < D.k; ++j) row[j] = -1;
+    } else if (kAccum) {
+      D.out_rgb[3 * qi + 0] = nan;
+      D.out_rgb[3 * qi + 1] = nan;
+      D.out_rgb[3 * qi + 2] = nan;
+    }
+    return;
+  }
   const float r2 = (k_eff > 0) ? __uint_as_float(static_cast<uint32_t>(tau >> 32)) : 0.0f;
   if (D.out_r2) D.out_r2[qi] = r2;
   if (D.out_key) D.out_key[qi] = (k_eff > 0) ? tau : 0ull;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -26,8 +27,14 @@ constexpr int kStashCap = PM_STASH_CAP;  // per-query candidates kept past the c
 #endif
 constexpr int kGridBits = PM_GRID_BITS;  // space-filling-curve grid bits per axis
 // bits of a curve code: the top 32 bits of the 3 * kGridBits-bit index
-constexpr int kCodeBits = 3 * kGridBits < 32 ? 3 * kGridBits : 32;
-static_assert(kGridBits >= 6 && kGridBits <= 21, "curve grid bits");
+constexpr int kCodeBits = 3 * kGridBits;
+// Bit 31 of a query's curve key marks a query outside the supported domain
+// (kInvalidKeyBit), so curve codes must stay below it.
+static_assert(kGridBits >= 6 && kGridBits <= 10, "curve grid bits");
+constexpr uint32_t kInvalidKeyBit = 1u << 31;
+// Query and photon coordinates must be finite and within +-kCoordLimit
+// (2^60): every squared distance then stays finite in FP32.
+constexpr float kCoordLimit = 1152921504606846976.0f;
 #ifndef PM_QUERY_SORT_LO_BIT
 #define PM_QUERY_SORT_LO_BIT 0
 #endif
@@ -41,7 +48,7 @@ constexpr int kQuerySortLoBit = PM_QUERY_SORT_LO_BIT;  // query keys sort from t
 constexpr int kAnchorWindow = PM_ANCHOR_WINDOW;  // anchors tried on each side of a query's
 constexpr int kAnchorStride = PM_ANCHOR_STRIDE;  // default photons per radius anchor
 constexpr int kAnchorTabBits = 16;  // code prefix bits of the anchor lookup table
-constexpr int kMaxKnnK = 1024;      // row-sort limit of pm_knn
+constexpr int kRowSortLds = 8192;   // kNN row sort: longest row sorted in LDS (64 KiB)
 constexpr int kXcds = 8;            // MI355X accelerator complex dies
 
 // Layout of the per-launch counter block: kXcds work-queue counters, then
@@ -175,14 +182,18 @@ struct PhotonMapImpl {
   int anchor_stride = kAnchorStride;
   int n_anchors = 0;
   // radius bounds of the anchors, one set per k (least recently used
-  // replaced); each holds the centroids with w = the k-NN radius bound
+  // replaced); each holds the centroids with w = the k-NN radius bound.  A
+  // set pinned by the batch being prepared (pin == that batch's epoch) is
+  // never replaced: the cache grows past kAnchorSets instead (a deque, so
+  // the sets never move).
   struct AnchorSet {
     int k = -1;
     uint64_t stamp = 0;
+    uint64_t pin = 0;
     DevBuf buf;
   };
   static constexpr int kAnchorSets = 4;
-  AnchorSet aset[kAnchorSets];
+  std::deque<AnchorSet> aset = std::deque<AnchorSet>(kAnchorSets);
   uint64_t anchor_clock = 0;
   float qlo[3] = {0, 0, 0};
   float qscale[3] = {1, 1, 1};
@@ -201,6 +212,7 @@ struct Context {
   int leaf_size = 8;
   int xcd_aware = 1;
   float run_split = 0.9f;
+  uint64_t batch_epoch = 0;  // pins the anchor sets of the batch being prepared
   int timing = 0;
   int count_stats = 0;
   pm_stats stats{};
@@ -253,7 +265,8 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s);
 int update_flux(PhotonMapImpl* map, const float* flux_dev, hipStream_t s);
 // Anchor radius bounds of the map for k_eff (computed on first use, then
 // cached); *out receives the anchor array the gather must read.
-int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** out);
+int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** out,
+                   uint64_t pin_epoch = 0);
 // The cached anchor set of k_eff (marked most recently used), or nullptr.
 const float4* cached_anchors(PhotonMapImpl* map, int k_eff);
 // Descriptor of the anchor-bound pass of the map for k_eff writing to out
@@ -297,8 +310,10 @@ int prepare_queries_raw(const QueryScratch& qs, int n_inst, const GatherDesc* de
                         const uint32_t** perm_out, bool need_anchors);
 
 int launch_gather(Context* ctx, const GatherLaunch& L, hipStream_t s);
-int launch_knn_rowsort(const GatherDesc* dev_desc, int64_t m, int32_t* out_idx,
-                       hipStream_t s);
+// Sorts m kNN rows of k indices farthest first (rows longer than
+// kRowSortLds keys sort in `scratch`, which it grows).
+int launch_knn_rowsort(const GatherDesc* dev_desc, int64_t m, int k, int32_t* out_idx,
+                       DevBuf* scratch, hipStream_t s);
 int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 int launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
 

@@ -65,7 +65,32 @@ struct RenderDev {
   int width, height, spp, max_depth, fg_depth;
   int caustics;  // 1: caustics gathers are wanted
   uint64_t seed;
+  // given rays (6 floats each: origin, direction) in place of camera samples:
+  // sample i starts with ray i (one sample per ray, width = rays, height 1)
+  const float* rays;
 };
+
+// The first ray of sample sid: the given ray, or the camera's through the
+// jittered sensor point of its pixel (examples/main.cpp:40-41,
+// include/camera.h:41-47).
+__device__ __forceinline__ void first_ray(const RenderDev& R, const pmx_camera& cam,
+                                          uint64_t sid, V3& o, V3& d) {
+  if (R.rays) {
+    const float* r = R.rays + 6 * sid;
+    o = v3(r[0], r[1], r[2]);
+    d = v3(r[3], r[4], r[5]);
+    return;
+  }
+  const int64_t pix = static_cast<int64_t>(sid / static_cast<uint64_t>(R.spp));
+  const int row = static_cast<int>(pix / R.width);
+  const int col = static_cast<int>(pix % R.width);
+  const float su = (2.0f * (col + rnd(R.seed, sid, 0)) - R.width) / R.height;
+  const float sv = (2.0f * (row + rnd(R.seed, sid, 1)) - R.height) / R.height;
+  const V3 pos = ld3(cam.position), fwd = ld3(cam.forward), rgt = ld3(cam.right),
+           up = ld3(cam.up);
+  o = pos + rgt * su + up * sv;
+  d = normalize(pos + fwd * cam.focal - o);
+}
 
 __device__ __forceinline__ V3 ldv(const float* p) { return v3(p[0], p[1], p[2]); }
 
@@ -165,17 +190,9 @@ __global__ __launch_bounds__(kThreads) void camera_kernel(RenderDev R, pmx_camer
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   const uint64_t sid = static_cast<uint64_t>(sample0) + i;
-  const int64_t pix = static_cast<int64_t>(sid / static_cast<uint64_t>(R.spp));
-  const int row = static_cast<int>(pix / R.width);
-  const int col = static_cast<int>(pix % R.width);
   const uint64_t stream = sid;
-  // examples/main.cpp:40-41, include/camera.h:41-47
-  const float su = (2.0f * (col + rnd(R.seed, stream, 0)) - R.width) / R.height;
-  const float sv = (2.0f * (row + rnd(R.seed, stream, 1)) - R.height) / R.height;
-  const V3 pos = ld3(cam.position), fwd = ld3(cam.forward), rgt = ld3(cam.right),
-           up = ld3(cam.up);
-  const V3 sensor = pos + rgt * su + up * sv;
-  const V3 d = normalize(pos + fwd * cam.focal - sensor);
+  V3 sensor, d;
+  first_ray(R, cam, sid, sensor, d);
   Segment& sg = out[i];
   sg.o[0] = sensor.x; sg.o[1] = sensor.y; sg.o[2] = sensor.z;
   sg.d[0] = d.x; sg.d[1] = d.y; sg.d[2] = d.z;
@@ -406,16 +423,9 @@ __global__ __launch_bounds__(kThreads) void path_trace_kernel(RenderDev R, pmx_c
   if (i >= n) return;
   const SceneDev& S = R.S;
   const uint64_t sid = static_cast<uint64_t>(sample0) + i;
-  const int64_t pix = static_cast<int64_t>(sid / static_cast<uint64_t>(R.spp));
-  const int row = static_cast<int>(pix / R.width);
-  const int col = static_cast<int>(pix % R.width);
   const uint64_t seed = R.seed, stream = sid;
-  const float su = (2.0f * (col + rnd(seed, stream, 0)) - R.width) / R.height;
-  const float sv = (2.0f * (row + rnd(seed, stream, 1)) - R.height) / R.height;
-  const V3 pos = ld3(cam.position), fwd = ld3(cam.forward), rgt = ld3(cam.right),
-           up = ld3(cam.up);
-  V3 o = pos + rgt * su + up * sv;
-  V3 d = normalize(pos + fwd * cam.focal - o);
+  V3 o, d;
+  first_ray(R, cam, sid, o, d);
   uint32_t dim = 2;
   V3 tp = v3(1.f, 1.f, 1.f);
   V3 L = v3(0.f, 0.f, 0.f);
@@ -560,14 +570,12 @@ int64_t batch_pixels_of(const pmx_render_params* p) {
 }  // namespace
 }  // namespace pm
 
-using namespace pm;
+namespace pm {
+namespace {
 
-extern "C" {
-
-int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
-                              pm_photonmap* global_map, pm_photonmap* caustics_map,
-                              const pmx_render_params* p, float* out_mean, float* out_var,
-                              pmx_render_stats* stats) {
+int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* global_map,
+              pm_photonmap* caustics_map, const pmx_render_params* p, const float* in_rays,
+              float* out_mean, float* out_var, pmx_render_stats* stats) {
   if (int rc = check_params(scene, cam, p, out_mean)) return rc;
   if (!global_map) return fail(PM_ERR_INVALID_ARG, "the global photon map is required");
   if (p->final_gathering_depth > 0 && !caustics_map)
@@ -579,7 +587,7 @@ int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
   ImageOut img;
   if (int rc = img.begin(p, out_mean, out_var)) return rc;
   RenderDev R{s->dev(0), p->width, p->height, p->spp, p->max_depth, p->final_gathering_depth,
-              p->final_gathering_depth > 0 ? 1 : 0, p->seed};
+              p->final_gathering_depth > 0 ? 1 : 0, p->seed, in_rays};
   const int64_t n_pix = static_cast<int64_t>(p->width) * p->height;
   const int64_t bp = std::min(batch_pixels_of(p), n_pix);
   const int64_t bsamples = bp * p->spp;
@@ -707,16 +715,15 @@ int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
   return PM_OK;
 }
 
-int pmx_render_path_tracing(const pmx_scene* scene, const pmx_camera* cam,
-                            const pmx_render_params* p, float* out_mean, float* out_var,
-                            pmx_render_stats* stats) {
+int render_pt(const pmx_scene* scene, const pmx_camera* cam, const pmx_render_params* p,
+              const float* in_rays, float* out_mean, float* out_var, pmx_render_stats* stats) {
   if (int rc = check_params(scene, cam, p, out_mean)) return rc;
   const auto t_start = std::chrono::steady_clock::now();
   const auto* s = reinterpret_cast<const SceneImpl*>(scene);
   PM_HIP(hipSetDevice(s->ctx->device));
   ImageOut img;
   if (int rc = img.begin(p, out_mean, out_var)) return rc;
-  RenderDev R{s->dev(0), p->width, p->height, p->spp, p->max_depth, 0, 0, p->seed};
+  RenderDev R{s->dev(0), p->width, p->height, p->spp, p->max_depth, 0, 0, p->seed, in_rays};
   const int64_t n_pix = static_cast<int64_t>(p->width) * p->height;
   const int64_t bp = std::min(batch_pixels_of(p), n_pix);
   const int64_t bsamples = bp * p->spp;
@@ -755,6 +762,59 @@ int pmx_render_path_tracing(const pmx_scene* scene, const pmx_camera* cam,
   if (int rc = img.end(p, out_mean, out_var)) return rc;
   if (stats) *stats = st;
   return PM_OK;
+}
+
+}  // namespace
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+int pmx_render_photon_mapping(const pmx_scene* scene, const pmx_camera* cam,
+                              pm_photonmap* global_map, pm_photonmap* caustics_map,
+                              const pmx_render_params* p, float* out_mean, float* out_var,
+                              pmx_render_stats* stats) {
+  return render_pm(scene, cam, global_map, caustics_map, p, nullptr, out_mean, out_var, stats);
+}
+
+int pmx_render_path_tracing(const pmx_scene* scene, const pmx_camera* cam,
+                            const pmx_render_params* p, float* out_mean, float* out_var,
+                            pmx_render_stats* stats) {
+  return render_pt(scene, cam, p, nullptr, out_mean, out_var, stats);
+}
+
+int pmx_integrate_rays(const pmx_scene* scene, int integrator, pm_photonmap* global_map,
+                       pm_photonmap* caustics_map, const pmx_render_params* params, int64_t n,
+                       const float* rays, float* out_rgb) {
+  if (!scene || !params || (n > 0 && (!rays || !out_rgb)))
+    return fail(PM_ERR_INVALID_ARG, "null argument");
+  if (integrator != PMX_INTEGRATOR_PATH_TRACING && integrator != PMX_INTEGRATOR_PHOTON_MAPPING)
+    return fail(PM_ERR_INVALID_ARG, "unknown integrator");
+  if (n < 0 || n > (int64_t{1} << 31) - 1) return fail(PM_ERR_INVALID_ARG, "bad ray count");
+  if (n == 0) return PM_OK;
+  if (params->out_mem != PM_MEM_DEVICE && params->out_mem != PM_MEM_HOST)
+    return fail(PM_ERR_INVALID_ARG, "bad out_mem");
+  const auto* s = reinterpret_cast<const SceneImpl*>(scene);
+  PM_HIP(hipSetDevice(s->ctx->device));
+  // one sample per ray: a width x 1 image of one sample per pixel
+  pmx_render_params p = *params;
+  p.width = static_cast<int32_t>(n);
+  p.height = 1;
+  p.spp = 1;
+  DevBuf staged;
+  const float* d_rays = rays;
+  if (p.out_mem == PM_MEM_HOST) {
+    if (int rc = staged.reserve(static_cast<size_t>(n) * 6 * sizeof(float))) return rc;
+    PM_HIP(hipMemcpy(staged.ptr, rays, static_cast<size_t>(n) * 6 * sizeof(float),
+                     hipMemcpyHostToDevice));
+    d_rays = staged.as<float>();
+  }
+  const pmx_camera unused{};
+  return integrator == PMX_INTEGRATOR_PATH_TRACING
+             ? render_pt(scene, &unused, &p, d_rays, out_rgb, nullptr, nullptr)
+             : render_pm(scene, &unused, global_map, caustics_map, &p, d_rays, out_rgb,
+                         nullptr, nullptr);
 }
 
 }  // extern "C"

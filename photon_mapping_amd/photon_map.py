@@ -129,6 +129,9 @@ class PhotonMap:
         self._handle = None
         self._n = 0
         self._built = False
+        # version of the flux the device map holds, and the array it came from
+        self.flux_version = 0
+        self._flux_ref = None
 
     @classmethod
     def _adopt(cls, ctx: Context, handle: C.c_void_p, n: int) -> "PhotonMap":
@@ -196,12 +199,18 @@ class PhotonMap:
         self._handle = h
         self._n = n
         self._host = []
+        self.flux_version += 1
+        self._flux_ref = flux
 
     def update_flux(self, flux) -> None:
+        """Replaces every photon's flux ((n, 3), original order) without a
+        rebuild; flux_version counts the replacements."""
         mem = _lib.PM_MEM_DEVICE if _is_torch(flux) else _lib.PM_MEM_HOST
         f = _soa(flux)
         stream = _stream_of(f) if mem == _lib.PM_MEM_DEVICE else None
         call("pm_photonmap_update_flux", self._handle, ptr(f), mem, stream)
+        self.flux_version += 1
+        self._flux_ref = flux
 
     def knn(self, queries, k: int):
         """queries (m, 3) -> (indices (m, k) int32 farthest first, -1 padded;

@@ -246,13 +246,45 @@ def caustics_scene() -> Scene:
     return s
 
 
-def water_scene(seed: int = 0, grid: int = 256) -> Scene:
+def _patch_walls():
+    """The box scene's five walls cut into 64 Lambert patches, one material
+    each: floor, ceiling and back wall 4 x 4 patches (materials 0-15, 16-31,
+    32-47), left and right walls 2 (along z) x 4 (along y) patches (48-55,
+    56-63).  Patches keep their wall's albedo (grey 0.75, left (0.63, 0.065,
+    0.05), right (0.14, 0.45, 0.09)) and its inward winding.  Returns the
+    triangles (256) and the (64, 3) albedos."""
+    grey, left, right = (0.75, 0.75, 0.75), (0.63, 0.065, 0.05), (0.14, 0.45, 0.09)
+    tris, albedo = [], []
+
+    def wall(nu, nv, corner, du, dv, normal, rho):
+        o, a, b = (np.asarray(x, np.float64) for x in (corner, du, dv))
+        for i in range(nu):
+            for j in range(nv):
+                p0 = o + a * (i / nu) + b * (j / nv)
+                p1 = o + a * ((i + 1) / nu) + b * (j / nv)
+                p2 = o + a * ((i + 1) / nu) + b * ((j + 1) / nv)
+                p3 = o + a * (i / nu) + b * ((j + 1) / nv)
+                tris.extend(_quad(p0, p1, p2, p3, normal, len(albedo)))
+                albedo.append(rho)
+
+    wall(4, 4, (0, 0, 0), (1, 0, 0), (0, 0, 1), (0, 1, 0), grey)     # floor
+    wall(4, 4, (0, 1, 0), (1, 0, 0), (0, 0, 1), (0, -1, 0), grey)    # ceiling
+    wall(4, 4, (0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), grey)     # back
+    wall(2, 4, (0, 0, 0), (0, 0, 1), (0, 1, 0), (1, 0, 0), left)     # left
+    wall(2, 4, (1, 0, 0), (0, 0, 1), (0, 1, 0), (-1, 0, 0), right)   # right
+    return np.array(tris, dtype=TRI_DTYPE), np.asarray(albedo, np.float32)
+
+
+def water_scene(seed: int = 0, grid: int = 256, patches: bool = True) -> Scene:
     """Differentiation configuration: the box scene's walls and light over a
     water surface, a height field on a grid x grid lattice over the floor
     (2 grid^2 triangles, 131,072 at 256) at y = 0.4 with heights the sum of 8
     sinusoids (wavevector direction uniform, magnitude uniform in [4, 40]
     rad/unit, amplitude uniform in [0, 0.02], phase uniform; numpy seed),
-    glass of rho 1 and IOR 1.33, smooth normals from the analytic gradient."""
+    glass of rho 1 and IOR 1.33, smooth normals from the analytic gradient.
+    With patches (the default) the walls are the 64 one-material patches of
+    _patch_walls, so the scene has 64 Lambert albedo parameters (materials
+    0-63), the light's material (64) and the water (65)."""
     rng = np.random.default_rng(seed)
     ang = rng.uniform(0, 2 * math.pi, 8)
     mag = rng.uniform(4.0, 40.0, 8)
@@ -274,6 +306,13 @@ def water_scene(seed: int = 0, grid: int = 256) -> Scene:
     a, b, c, d = (i, j), (i + 1, j), (i + 1, j + 1), (i, j + 1)
     tri_idx = [(a, d, c), (a, c, b)]
     s = _box_shell(False)
+    if patches:
+        walls, alb = _patch_walls()
+        light = s.tris[10:12].copy()
+        light["material"] = len(alb)
+        s.tris = np.concatenate([walls, light])
+        s.albedo = np.concatenate([alb, s.albedo[3:4]])
+        s.mat_types = np.zeros(len(s.albedo), np.int32)
     water = len(s.albedo)
     recs = []
     for p0, p1, p2 in tri_idx:

@@ -107,3 +107,35 @@ def test_autograd_fits_albedo(gpu, rng):
         first = first if first is not None else float(loss.detach())
     assert float(loss.detach()) < 1e-3 * first
     assert torch.allclose(alb.detach(), true_alb, rtol=2e-2, atol=2e-3)
+
+
+def test_autograd_two_forwards_before_backward(gpu, oracle, rng):
+    """Two forward passes on one map with different flux, then both backward
+    passes: each albedo gradient uses its own forward flux, and the map
+    keeps the newer flux afterwards."""
+    import torch
+
+    from photon_mapping_amd.autograd import gather_radiance
+
+    pos, wi, flux, q, mats, pmap, dev = _setup(gpu, rng, n=8000, m=1000)
+    k, n_emit = 32, 1e4
+    g = rng.normal(size=(len(q), 3)).astype(np.float32)
+    types = torch.from_numpy(mats.view(np.int32)[:, 3].copy())
+    qt = torch.from_numpy(q).to(dev)
+    fluxes = [flux, (flux * np.float32(2.5) + np.float32(0.1)).astype(np.float32)]
+    albs, outs = [], []
+    for f in fluxes:
+        alb = torch.from_numpy(mats[:, :3].copy()).to(dev).requires_grad_(True)
+        out = gather_radiance(pmap, qt, torch.from_numpy(f).to(dev), alb, types, k, n_emit)
+        albs.append(alb)
+        outs.append(out)
+    for out in outs:
+        (out * torch.from_numpy(g).to(dev)).sum().backward()
+    bidx, br2 = oracle.brute_knn(pos, q[:, :3], k)
+    for f, alb in zip(fluxes, albs):
+        _, er = oracle.adjoint_fp64(wi, f, q, mats, bidx, br2, n_emit, g)
+        np.testing.assert_allclose(alb.grad.cpu().numpy(), er, rtol=1e-4,
+                                   atol=1e-6 * np.abs(er).max())
+    # the map still holds the newer flux
+    rgb, _, _ = pmap.gather_radiance(qt, torch.from_numpy(mats).to(dev), k, n_emit)
+    np.testing.assert_array_equal(rgb.cpu().numpy(), outs[1].detach().cpu().numpy())

@@ -147,3 +147,40 @@ def test_plan_graph_replays(gpu, rng):
             assert torch.equal(key, e_key)
     del graph
     plan.close()
+
+
+def test_batched_many_k_on_one_map(gpu, rng):
+    """Five distinct k on one map in one batch (more than the map's four
+    cached radius-bound sets): no set an earlier instance uses is replaced
+    before the launch, so no group falls back to an unbounded search, and
+    every instance equals its single call."""
+    import torch
+
+    from photon_mapping_amd import _lib
+
+    dev = torch.device("cuda:0")
+    mats = torch.from_numpy(materials(3)).to(dev)
+    pos, wi, flux, q = _instance_data(rng, 40000, 3000, "surfaces")
+    pmap = gpu.PhotonMap()
+    pmap.set_photons_soa(torch.from_numpy(pos).to(dev), torch.from_numpy(wi).to(dev),
+                         torch.from_numpy(flux).to(dev))
+    pmap.build()
+    qd = torch.from_numpy(q).to(dev)
+    # descending: a later, smaller k evicting an earlier set would leave that
+    # instance with bounds too tight for its k
+    inst = [(pmap, qd, mats, k, 1e4) for k in (150, 100, 64, 32, 16, 8)]
+    ctx = pmap.ctx
+    print("batched: six k on one map", flush=True)
+    ctx.set_option(_lib.PM_OPT_COUNT_STATS, 1)
+    try:
+        got = gpu.gather_batched(inst)
+        torch.cuda.synchronize()
+        st = ctx.stats()
+    finally:
+        ctx.set_option(_lib.PM_OPT_COUNT_STATS, 0)
+    assert st["fallbacks"] == 0, st
+    for (pm_, q_, mt, k, ne), (rgb, r2, key) in zip(inst, got):
+        e_rgb, e_r2, e_key = pm_.gather_radiance(q_, mt, k, ne)
+        torch.cuda.synchronize()
+        assert torch.equal(rgb.view(torch.int32), e_rgb.view(torch.int32)), k
+        assert torch.equal(key, e_key), k

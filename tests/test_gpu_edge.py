@@ -93,3 +93,55 @@ def test_render_argument_errors():
     unbuilt.set_photons_soa(pos, wi, flux)
     with pytest.raises(pm.PmError):
         sc.render_photon_mapping(s, unbuilt, n_emit, 8, 8, 1)
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf, 3e18])
+def test_out_of_domain_queries_leave_the_others_exact(gpu, oracle, bad):
+    """A query with a NaN or infinite coordinate, or one beyond +-2^60, among
+    valid ones: it gets NaN radiance and r^2, an all-ones key and a kNN row of
+    -1, contributes nothing to the adjoint, and every other query still equals
+    the oracle (its lanes share groups with the bad one)."""
+    from util import materials, photons_surfaces, query_records
+
+    rng = np.random.default_rng(8)
+    pos, wi, flux, nrm = photons_surfaces(rng, 8000)
+    m, k = 700, 32
+    sel = rng.integers(0, len(pos), m)
+    q = query_records(rng, pos[sel] + np.float32(1e-3) * nrm[sel], nrm[sel])
+    badrows = np.array([0, 5, 63, 64, 300, 699])
+    for r in badrows:
+        q[r, int(r) % 3] = bad
+    mats = materials(3)
+    pmap = gpu.PhotonMap()
+    pmap.set_photons_soa(pos, wi, flux)
+    pmap.build()
+    print(f"out-of-domain queries ({bad})", flush=True)
+    rgb, r2, key = pmap.gather_radiance(q, mats, k, 1e4)
+    good = np.setdiff1d(np.arange(m), badrows)
+    assert np.isnan(rgb[badrows]).all() and np.isnan(r2[badrows]).all()
+    assert (key[badrows] == np.uint64(0xFFFFFFFFFFFFFFFF)).all()
+    bidx, br2 = oracle.brute_knn(pos, q[good, :3], k)
+    np.testing.assert_array_equal(r2[good].view(np.uint32), br2.view(np.uint32))
+    expect = oracle.radiance_fp64(wi, flux, q[good], mats, bidx, br2, 1e4)
+    nz = expect != 0
+    assert (rgb[good][~nz] == 0).all()
+    np.testing.assert_allclose(rgb[good][nz], expect[nz], rtol=1e-5)
+    idx, kr2 = pmap.knn(q[:, :3], k)
+    assert (idx[badrows] == -1).all() and np.isnan(kr2[badrows]).all()
+    np.testing.assert_array_equal(idx[good], bidx)
+    g = rng.normal(size=(m, 3)).astype(np.float32)
+    for kk in (key, None):
+        dflux, drho = pmap.gather_adjoint(q, mats, k, 1e4, g, key=kk)
+        ef, er = oracle.adjoint_fp64(wi, flux, q[good], mats, bidx, br2, 1e4, g[good])
+        np.testing.assert_allclose(dflux, ef, rtol=1e-4, atol=1e-5 * np.abs(ef).max())
+        np.testing.assert_allclose(drho, er, rtol=1e-4, atol=1e-6 * np.abs(er).max())
+
+
+def test_photons_beyond_the_domain_are_rejected(gpu):
+    pos = np.zeros((10, 3), np.float32)
+    pos[3, 1] = 2e18
+    pmap = gpu.PhotonMap()
+    pmap.set_photons_soa(pos, pos, pos)
+    with pytest.raises(gpu.PmError) as e:
+        pmap.build()
+    assert e.value.code == 1

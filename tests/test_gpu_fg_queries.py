@@ -105,6 +105,10 @@ def test_fg_queries_match_oracle_distribution(gpu, oracle, name, depth):
     print(f"  image blocks: chi2 {chi2:.1f} on {dof} dof, p = {p:.3g}", flush=True)
     assert dof > 50
     assert p > 1e-4, (chi2, dof, p)
-    # placeholders (no query point in 64 draws) are as rare on both sides
-    assert (qg.view(np.int32)[:, 9] < 0).mean() < 1e-3
-    assert (qc.view(np.int32)[:, 9] < 0).mean() < 1e-3
+    # placeholders (pixels whose 64 draws find no query point: edge pixels
+    # that see past the open box, pixels that see only a specular box) are
+    # as frequent on both sides
+    pg = float((qg.view(np.int32)[:, 9] < 0).mean())
+    pc = float((qc.view(np.int32)[:, 9] < 0).mean())
+    print(f"  placeholders: device {pg:.4f}, restatement {pc:.4f}", flush=True)
+    assert abs(pg - pc) <= 3e-3

@@ -223,3 +223,17 @@ def test_knn_rejects_bad_input(gpu):
     pmap.build()
     with pytest.raises(PmError):
         pmap.knn(ok, 0)
+
+
+@pytest.mark.parametrize("k", [1025, 3000, 9000])
+def test_knn_large_k(gpu, oracle, rng, k):
+    """k beyond 1024 (the reference has no cap): rows up to 8192 sort in LDS,
+    longer rows through a device scratch; both equal brute force."""
+    pos, wi, flux = photons_clustered(rng, 20000)
+    q = queries_for(rng, pos, 60, jitter=0.02)
+    print(f"knn large k={k}", flush=True)
+    pmap = _map(gpu, pos, wi, flux)
+    idx, r2 = pmap.knn(q, k)
+    bidx, br2 = oracle.brute_knn(pos, q, k)
+    np.testing.assert_array_equal(r2, br2)
+    np.testing.assert_array_equal(idx, bidx)

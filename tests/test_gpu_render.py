@@ -148,6 +148,56 @@ def test_photon_mapping_matches_oracle(gpu, name, fg):
     _ci_check(mg.cpu().numpy(), vg.cpu().numpy(), mc, vc, f"pm {name} fg={fg}")
 
 
+@pytest.mark.parametrize("name", ["caustics", "water"])
+def test_photon_mapping_config_scenes_match_oracle(gpu, name):
+    """The caustics configuration (two 5120-triangle icospheres, a mirror and
+    IOR-1.5 glass) and the differentiation configuration (the 131,072-triangle
+    water height field, IOR 1.33, 64 wall patches) rendered with final
+    gathering, a global and a caustics map and k = 100, at a reduced size:
+    the device frame against the CPU restatement within the 99% intervals."""
+    from photon_mapping_amd import scene as sc
+
+    s = sc.caustics_scene() if name == "caustics" else sc.water_scene(0)
+    maps = _maps(s, 60_000, 30_000)
+    gmap, ghost, gemit = maps["global"]
+    cmap, chost, cemit = maps["caustics"]
+    w, h, spp, k = 24, 24, 48, 100
+    print(f"photon mapping {name} fg=1 k={k}", flush=True)
+    mg, vg, st = sc.render_photon_mapping(s, gmap, gemit, w, h, spp, k=k, caustics_map=cmap,
+                                          n_emit_caustics=cemit, final_gathering_depth=1,
+                                          seed=1)
+    mc, vc, dropped = _oracle_render(s, "pm", w, h, spp, fg_depth=1, k=k, global_map=ghost,
+                                     n_emit_global=gemit, caustics_map=chost,
+                                     n_emit_caustics=cemit, seed=2)
+    assert st["samples"] == w * h * spp
+    assert st["gathers_caustics"] > 0 and st["gathers_global"] > 0
+    assert st["dropped_samples"] == dropped == 0
+    _ci_check(mg.cpu().numpy(), vg.cpu().numpy(), mc, vc, f"pm {name} fg=1")
+
+
+def test_plain_configuration_full_size_matches_oracle(gpu):
+    """The CPU-runnable plain configuration at its full size: the box scene,
+    100,000 emitted photon paths (every stored photon kept, the reference's
+    count semantics), k = 100, 256 x 256 x 16 spp, no final gathering."""
+    from photon_mapping_amd import scene as sc
+
+    import torch
+
+    s = sc.box_scene()
+    with sc.NativeScene(s) as ns:
+        gmap, n_stored = sc.trace_photon_map(ns, 100_000, seed=0)
+        gmap.build()
+        pos, wi, flux, _ = sc.trace_photons(ns, n_stored, seed=0)
+        host = tuple(a.cpu().numpy() for a in (pos, wi, flux))
+        print(f"plain configuration: {n_stored} photons from 100000 paths", flush=True)
+        mg, vg, st = sc.render_photon_mapping(ns, gmap, 100_000, 256, 256, 16, k=100, seed=1)
+        torch.cuda.synchronize()
+    mc, vc, dropped = _oracle_render(s, "pm", 256, 256, 16, fg_depth=0, k=100,
+                                     global_map=host, n_emit_global=100_000, seed=2)
+    assert st["samples"] == 256 * 256 * 16 and st["dropped_samples"] == dropped == 0
+    _ci_check(mg.cpu().numpy(), vg.cpu().numpy(), mc, vc, "plain configuration 256x256x16")
+
+
 def test_render_is_deterministic(gpu):
     from photon_mapping_amd import scene as sc
 

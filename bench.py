@@ -71,8 +71,7 @@ PMC_PASSES = [
     ("fetch", ["FETCH_SIZE"]),
     ("write", ["WRITE_SIZE", "TCC_HIT_sum", "TCC_MISS_sum"]),
 ]
-PROFILE_SQ = ROOT / "profiles" / "gather_kernel_radiance_box1M_k100_fg512x64_sq_counters.md"
-PROFILE_TRAFFIC = ROOT / "profiles" / "gather_kernel_radiance_box1M_k100_fg512x64_traffic.md"
+PROFILE_COUNTERS = ROOT / "profiles" / "gather_kernel_radiance_box1M_k100_fg512x64_counters.md"
 
 
 def _read_counter_csv(path):
@@ -154,19 +153,16 @@ def run_pmc_passes(args, save_dir=None):
 
 
 def pmc_from_profiles():
-    """The committed counter summaries of the same workload (when the live
+    """The committed counter summary of the same workload (when the live
     passes are off or fail)."""
     res = {}
+    keys = {"valu_instructions_per_launch": "SQ_INSTS_VALU",
+            "traffic_bytes_per_launch": "traffic_bytes", "l2_hit_rate": "l2_hit_rate"}
     try:
-        for line in PROFILE_SQ.read_text().splitlines():
-            m = re.match(r"\|\s*(SQ_INSTS_VALU)\s*\|\s*([0-9.e+]+)", line)
-            if m:
-                res[m.group(1)] = float(m.group(2))
-        for line in PROFILE_TRAFFIC.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_launch:"):
-                res["traffic_bytes"] = float(line.split(":", 1)[1])
-            if line.startswith("l2_hit_rate:"):
-                res["l2_hit_rate"] = float(line.split(":", 1)[1])
+        for line in PROFILE_COUNTERS.read_text().splitlines():
+            name, _, val = line.partition(":")
+            if name in keys and val.strip():
+                res[keys[name]] = float(val)
     except OSError:
         pass
     return res

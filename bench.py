@@ -810,6 +810,7 @@ def main() -> int:
         "lane_candidates_per_run": st["candidates2"] / max(int(st["runs"]), 1),
         "pass_b_candidates_per_run": st["candidates3"] / max(int(st["runs"]), 1),
         "cut_lists": int(st["cut_lists"]),
+        "split_runs": int(st["split_runs"]),
         # share of wave time per phase of a group (statistics run)
         "phase_share": dict(zip(
             ["query+bound", "centre", "lists", "pass A", "thresholds", "pass B",

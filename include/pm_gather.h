@@ -98,6 +98,8 @@ typedef struct pm_stats {
   int64_t phase_cycles[8];
   int64_t candidates3; /* photons of the selection-pass lists, after they
                         * shrink to the lane windows' sphere              */
+  int64_t split_runs;  /* runs halved because their sphere needed more
+                        * subtrees than a candidate list holds            */
 } pm_stats;
 
 /* Options for pm_context_set_option. */

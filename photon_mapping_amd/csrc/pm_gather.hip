@@ -83,6 +83,9 @@ constexpr int kSlotCap = PM_SLOT_CAP;  // photons of an explicit slot list
 // A candidate list is either nr subtrees (sh.rl.rnode / sh.rl.rpre) or, when nr
 // is this value, an explicit list of photon slots (sh.cslot).
 constexpr uint32_t kSlotList = 0xffffffffu;
+// build_list's answer, when asked to give up rather than coarsen, that the
+// sphere needs more subtrees than a list holds: the caller splits its run.
+constexpr uint32_t kSplitRun = 0xfffffffeu;
 
 template <typename T>
 __device__ __forceinline__ const PM_CONST T* cptr(const T* p) {
@@ -374,15 +377,19 @@ __device__ uint32_t list_prefix(DescRef D, GatherShared& sh, int lane, uint32_t 
 // subtree roots to sh.rl.rnode and the exclusive prefix of their slot counts to
 // sh.rl.rpre; returns the number of subtrees, the total slot count in *total.
 // When the list would overflow, the walk restarts with whole inner nodes
-// allowed, then with a depth cap above the level that overflowed (at most
-// kCapDepth levels: 2^kCapDepth <= kMaxRanges subtrees, so it fits); a
-// frontier that would overflow emits its nodes whole, and a walk that still
-// overflows lists the root (never a truncated list).  *fine (when given) tells whether every entry is a single leaf
-// (asked with leaf_only and no coarsening was needed); coarsenings are
-// counted in *stat when given.
+// allowed.  If that overflows too, a caller that can split its run (split)
+// gets kSplitRun and no list; otherwise the walk restarts with a depth cap
+// above the level that overflowed (at most kCapDepth levels: 2^kCapDepth <=
+// kMaxRanges subtrees, so it fits).  A frontier that would overflow counts as
+// an overflow when split is set, else emits its nodes whole; a walk that
+// still overflows lists the root (never a truncated list).  *fine (when given)
+// tells whether every entry is a single leaf (asked with leaf_only and no
+// coarsening was needed).  A returned list coarser than the sphere asks adds
+// 1 to *stat (when given), 1000 when it is the whole map.
 __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2,
                                GatherShared& sh, int lane, uint32_t* total, uint32_t* stat,
-                               bool leaf_only = false, bool* fine = nullptr) {
+                               bool leaf_only = false, bool* fine = nullptr,
+                               bool split = false) {
   const uint32_t first_leaf = static_cast<uint32_t>(D.first_leaf);
   const int depth_leaf = 31 - __builtin_clz(first_leaf);
   uint32_t nr = 0;
@@ -424,16 +431,17 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
         }
         uint64_t em = __ballot(emit);
         uint64_t pmk = __ballot(push);
-        if (nnext + static_cast<uint32_t>(__popcll(pmk)) > static_cast<uint32_t>(kFront)) {
+        const bool front_full =
+            nnext + static_cast<uint32_t>(__popcll(pmk)) > static_cast<uint32_t>(kFront);
+        if (front_full && !split) {
           emit = emit || push;
           push = false;
           em = __ballot(emit);
           pmk = 0ull;
           coarse = true;
-          if (stat && lane == 0) atomicAdd(stat, 1u);
         }
         const uint32_t ne = static_cast<uint32_t>(__popcll(em));
-        if (nr + ne > static_cast<uint32_t>(kMaxRanges)) {
+        if ((front_full && split) || nr + ne > static_cast<uint32_t>(kMaxRanges)) {
           full = true;
           full_depth = depth + e;
           break;
@@ -449,11 +457,12 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
       depth += e;
     }
     if (!full) break;
-    if (stat && lane == 0) atomicAdd(stat, 1000u);
     if (attempt == 2) {
       whole = true;  // cannot happen with the depth cap; never return a cut list
     } else if (leaf_only) {
       leaf_only = false;
+    } else if (split) {
+      return kSplitRun;
     } else {
       coarse = true;
       max_depth = min(full_depth - 1, kCapDepth);
@@ -465,6 +474,7 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
     if (lane == 0) sh.rl.rnode[0] = 1u;
     __syncthreads();
   }
+  if (stat && lane == 0 && coarse && r2 < INFINITY && depth_leaf > 0) atomicAdd(stat, whole ? 1000u : 1u);
   if (fine) *fine = leaf_only && !coarse && !whole && max_depth == depth_leaf;
   return list_prefix(D, sh, lane, nr, total);
 }
@@ -885,12 +895,14 @@ struct PhaseTimer {
 // {key < klo} plus the `need` smallest keys of [klo, khi), and [klo, khi)
 // holds at most kStashCap keys.  Leaves the run's candidate list (covering
 // every key below the lane bounds) in sh.rl.rnode/sh.rl.rpre: returns its entry
-// count and its photon count in *total.
+// count and its photon count in *total.  With split (a run of several lanes),
+// returns kSplitRun and selects nothing when the centre sweep's sphere needs
+// more subtrees than a list holds.
 __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool part, float r0sq,
                                float cx, float cy, float cz, float dq, float dmax,
                                uint32_t k_eff, GatherShared& sh, int lane, uint32_t* counters,
                                int count_stats, SelState& st, PhaseTimer& pt,
-                               uint32_t* total) {
+                               uint32_t* total, bool split) {
   uint32_t* cut_stat = count_stats ? &counters[kStatOverflow] : nullptr;
   // ---- centre bracket: lane q's k nearest photons lie within X_q of q, hence
   // within X_q + |q - c| of c, so the smallest such sum bounds the centre's
@@ -907,7 +919,9 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
     // plus its |q - c| (each lane's upper end is at most its own bound).
     const float xmax = wave_max(part ? xq : 0.0f);
     const float rl = fminf((r0c * 1.0001f + 2.0f * dmax) * 1.0004f, xmax * 1.0005f) + 2e-30f;
-    nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, true, &fine);
+    nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, true, &fine,
+                    split);
+    if (nr == kSplitRun) return kSplitRun;
     if (count_stats && lane == 0) atomicAdd(&counters[kStatCand1], *total);
     float t_lo = 0.0f, t_hi = INFINITY;
     if (centre_bound(D, sh, lane, nr, *total, fine, cx, cy, cz, r0c * r0c * 1.0001f + 1e-30f,
@@ -1179,7 +1193,6 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         starts |= 1ull << ((c0 + c1) >> 1);
         continue;
       }
-      if (count_stats && lane == 0) atomicAdd(&counters[kStatRuns], 1u);
       const float cx = lx + 0.5f * (hx - lx);
       const float cy = ly + 0.5f * (hy - ly);
       const float cz = lz + 0.5f * (hz - lz);
@@ -1199,7 +1212,13 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         const float rs2 = rs * rs + 1e-30f;
         bool fine = false;
         nr = build_list(D, cx, cy, cz, rs2, sh, lane, &total,
-                        count_stats ? &counters[kStatOverflow] : nullptr, true, &fine);
+                        count_stats ? &counters[kStatOverflow] : nullptr, true, &fine,
+                        c1 - c0 > 1);
+        if (nr == kSplitRun) {
+          if (count_stats && lane == 0) atomicAdd(&counters[kStatSplit], 1u);
+          starts |= 1ull << ((c0 + c1) >> 1);
+          continue;
+        }
         // both scatter passes read the list: keep exactly the photons of
         // the run's sphere when they fit the slot list
         const uint32_t ns = collect_slots(D, sh, lane, nr, total, fine, cx, cy, cz,
@@ -1216,7 +1235,14 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         pt.mark(2, lane);
       } else {
         nr = run_select(D, qx, qy, qz, part, r0sq, cx, cy, cz, dq, dmax, k_eff, sh, lane,
-                        counters, count_stats, st, pt, &total);
+                        counters, count_stats, st, pt, &total, c1 - c0 > 1);
+        // A run whose sphere needs more subtrees than a list holds would
+        // sweep coarse subtrees far beyond it: its halves go separately.
+        if (nr == kSplitRun) {
+          if (count_stats && lane == 0) atomicAdd(&counters[kStatSplit], 1u);
+          starts |= 1ull << ((c0 + c1) >> 1);
+          continue;
+        }
         // With the lane windows known, every key below a lane's khi lies
         // within sqrt(d^2 of khi) + |q - c| of c: the slot list shrinks to
         // the sphere that holds them all before the later passes read it.
@@ -1231,6 +1257,8 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
           if (count_stats && lane == 0) atomicAdd(&counters[kStatCand3], total);
         }
       }
+
+      if (count_stats && lane == 0) atomicAdd(&counters[kStatRuns], 1u);
 
       // ---- pass B: consume neighbours below klo, stash [klo, khi) ---------------
       // (the adjoint runs it only to finish the selection; its scatter pass

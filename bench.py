@@ -811,6 +811,7 @@ def main() -> int:
         "pass_b_candidates_per_run": st["candidates3"] / max(int(st["runs"]), 1),
         "cut_lists": int(st["cut_lists"]),
         "split_runs": int(st["split_runs"]),
+        "narrowed_runs": int(st["narrowed_runs"]),
         # share of wave time per phase of a group (statistics run)
         "phase_share": dict(zip(
             ["query+bound", "centre", "lists", "pass A", "thresholds", "pass B",

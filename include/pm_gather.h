@@ -100,6 +100,8 @@ typedef struct pm_stats {
                         * shrink to the lane windows' sphere              */
   int64_t split_runs;  /* runs halved because their sphere needed more
                         * subtrees than a candidate list holds            */
+  int64_t narrowed_runs; /* one-query runs whose loose radius bound was
+                          * narrowed over leaf lists first                */
 } pm_stats;
 
 /* Options for pm_context_set_option. */

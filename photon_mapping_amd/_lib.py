@@ -66,7 +66,7 @@ class pm_stats(C.Structure):
                 ("refinements", C.c_int64), ("candidates1", C.c_int64),
                 ("candidates2", C.c_int64), ("cut_lists", C.c_int64), ("runs", C.c_int64),
                 ("phase_cycles", C.c_int64 * 8), ("candidates3", C.c_int64),
-                ("split_runs", C.c_int64)]
+                ("split_runs", C.c_int64), ("narrowed_runs", C.c_int64)]
 
 
 class pm_tune_report(C.Structure):

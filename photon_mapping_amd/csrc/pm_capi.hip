@@ -271,7 +271,7 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
       ctx->stats.post_ms = elapsed_ms(ctx->ev[2], ctx->ev[3]);
     }
     if (ctx->count_stats) {
-      uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      uint32_t c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
       PM_HIP(hipMemcpy(c, ctx->ws.counters.as<uint32_t>() + kStatBase, sizeof(c),
                        hipMemcpyDeviceToHost));
       ctx->stats.fallbacks = c[kStatFallback - kStatBase];
@@ -282,6 +282,7 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
       ctx->stats.runs = c[kStatRuns - kStatBase];
       ctx->stats.candidates3 = c[kStatCand3 - kStatBase];
       ctx->stats.split_runs = c[kStatSplit - kStatBase];
+      ctx->stats.narrowed_runs = c[kStatNarrowed - kStatBase];
       uint64_t ph[kPhases];
       PM_HIP(hipMemcpy(ph, ctx->ws.counters.as<uint32_t>() + kPhaseBase, sizeof(ph),
                        hipMemcpyDeviceToHost));

@@ -84,8 +84,13 @@ constexpr int kSlotCap = PM_SLOT_CAP;  // photons of an explicit slot list
 // is this value, an explicit list of photon slots (sh.cslot).
 constexpr uint32_t kSlotList = 0xffffffffu;
 // build_list's answer, when asked to give up rather than coarsen, that the
-// sphere needs more subtrees than a list holds: the caller splits its run.
-constexpr uint32_t kSplitRun = 0xfffffffeu;
+// sphere needs more subtrees than a list holds.
+constexpr uint32_t kNoList = 0xfffffffeu;
+// What build_list does when a list would overflow (after the retry with whole
+// inner nodes, for kListGiveUp; at once, for kListLeavesOnly).
+constexpr int kListCoarsen = 0;
+constexpr int kListGiveUp = 1;
+constexpr int kListLeavesOnly = 2;
 
 template <typename T>
 __device__ __forceinline__ const PM_CONST T* cptr(const T* p) {
@@ -377,19 +382,21 @@ __device__ uint32_t list_prefix(DescRef D, GatherShared& sh, int lane, uint32_t 
 // subtree roots to sh.rl.rnode and the exclusive prefix of their slot counts to
 // sh.rl.rpre; returns the number of subtrees, the total slot count in *total.
 // When the list would overflow, the walk restarts with whole inner nodes
-// allowed.  If that overflows too, a caller that can split its run (split)
-// gets kSplitRun and no list; otherwise the walk restarts with a depth cap
-// above the level that overflowed (at most kCapDepth levels: 2^kCapDepth <=
-// kMaxRanges subtrees, so it fits).  A frontier that would overflow counts as
-// an overflow when split is set, else emits its nodes whole; a walk that
-// still overflows lists the root (never a truncated list).  *fine (when given)
+// allowed (unless give_up is kListLeavesOnly, which returns kNoList and no
+// list at once).  If that overflows too, kListGiveUp returns kNoList;
+// kListCoarsen restarts the walk with a depth cap above the level that
+// overflowed (at most kCapDepth levels: 2^kCapDepth <= kMaxRanges subtrees, so
+// it fits).  A frontier that would overflow counts as an overflow when giving
+// up, else emits its nodes whole; a walk that still overflows lists the root
+// (never a truncated list).  *fine (when given)
 // tells whether every entry is a single leaf (asked with leaf_only and no
 // coarsening was needed).  A returned list coarser than the sphere asks adds
 // 1 to *stat (when given), 1000 when it is the whole map.
 __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2,
                                GatherShared& sh, int lane, uint32_t* total, uint32_t* stat,
                                bool leaf_only = false, bool* fine = nullptr,
-                               bool split = false) {
+                               int give_up = kListCoarsen) {
+  const bool split = give_up != kListCoarsen;
   const uint32_t first_leaf = static_cast<uint32_t>(D.first_leaf);
   const int depth_leaf = 31 - __builtin_clz(first_leaf);
   uint32_t nr = 0;
@@ -459,10 +466,10 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
     if (!full) break;
     if (attempt == 2) {
       whole = true;  // cannot happen with the depth cap; never return a cut list
-    } else if (leaf_only) {
+    } else if (leaf_only && give_up != kListLeavesOnly) {
       leaf_only = false;
     } else if (split) {
-      return kSplitRun;
+      return kNoList;
     } else {
       coarse = true;
       max_depth = min(full_depth - 1, kCapDepth);
@@ -872,6 +879,45 @@ __device__ bool centre_bound(DescRef D, GatherShared& sh, int lane, uint32_t nr,
   return true;
 }
 
+// Radius r with at least k_eff photons within r of c (FP32 d^2, up to the
+// margins), from leaf lists only, so that no coarse subtree is ever swept.
+// cap2 is a squared radius known to hold k_eff photons.  The search starts at
+// the squared radius rb; a sphere whose leaf list overflows shrinks (4-fold,
+// or to the geometric mean with the largest sphere seen holding fewer than
+// k_eff photons), one holding fewer grows; few and over seed those two ends.
+// A sphere that holds k_eff photons gives the bracket's upper end, narrowed by
+// a second histogram over it.  Returns sqrt(cap2) (rounded up) when 7 lists
+// give nothing better.
+__device__ float leaf_bracket(DescRef D, GatherShared& sh, int lane, float cx, float cy,
+                              float cz, float rb, float few, float over, float cap2,
+                              uint32_t k_eff) {
+  float r = sqrt_up(cap2) * 1.00002f + 1e-30f;
+  for (int attempt = 0; attempt < 7 && rb > 0.0f; ++attempt) {
+    uint32_t total = 0;
+    bool fine = false;
+    const uint32_t nr = build_list(D, cx, cy, cz, rb, sh, lane, &total, nullptr, true, &fine,
+                                   kListLeavesOnly);
+    if (nr == kNoList || !fine) {
+      over = rb;
+      rb = few > 0.0f ? sqrtf(few * over) : rb * 0.25f;
+      continue;
+    }
+    float t_lo = 0.0f, t_hi = INFINITY;
+    if (centre_bound(D, sh, lane, nr, total, fine, cx, cy, cz, rb, k_eff, &t_lo, &t_hi)) {
+      const float rb2 = t_hi;
+      r = fminf(r, sqrt_up(rb2) * 1.00002f + 1e-30f);
+      if (centre_bound(D, sh, lane, nr, total, fine, cx, cy, cz, rb2, k_eff, &t_lo, &t_hi)) {
+        r = fminf(r, sqrt_up(t_hi) * 1.00002f + 1e-30f);
+      }
+      break;
+    }
+    if (!(rb < cap2)) break;  // cap2 held its k_eff photons: cannot happen
+    few = rb;
+    rb = over < INFINITY ? sqrtf(few * over) : fminf(rb * 4.0f, cap2);
+  }
+  return r;
+}
+
 // Wave-cycle accounting of the phases of a group, for statistics runs only
 // (acc == nullptr otherwise).  Phases: 0 query and radius bound, 1 centre
 // bound, 2 candidate lists, 3 pass A, 4 thresholds and refinement, 5 pass B,
@@ -895,9 +941,11 @@ struct PhaseTimer {
 // {key < klo} plus the `need` smallest keys of [klo, khi), and [klo, khi)
 // holds at most kStashCap keys.  Leaves the run's candidate list (covering
 // every key below the lane bounds) in sh.rl.rnode/sh.rl.rpre: returns its entry
-// count and its photon count in *total.  With split (a run of several lanes),
-// returns kSplitRun and selects nothing when the centre sweep's sphere needs
-// more subtrees than a list holds.
+// count and its photon count in *total.  When the centre sweep's sphere needs
+// more subtrees than a list holds, a run of several lanes (split) gets
+// kNoList and nothing is selected: the caller halves it.  A run of one lane
+// then has a loose bound (its own): the sphere is narrowed by leaf_bracket
+// first, so no coarse list is swept.
 __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool part, float r0sq,
                                float cx, float cy, float cz, float dq, float dmax,
                                uint32_t k_eff, GatherShared& sh, int lane, uint32_t* counters,
@@ -908,7 +956,7 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
   // within X_q + |q - c| of c, so the smallest such sum bounds the centre's
   // k-NN radius; a histogram around c then brackets it
   const float xq = part ? sqrt_up(cull_bound(r0sq)) * 1.00002f + dq : INFINITY;
-  const float r0c = wave_min(xq) * 1.00002f;
+  float r0c = wave_min(xq) * 1.00002f;
   float rc_hi = INFINITY, rc_lo = 0.0f;
   uint32_t nr = 0;
   bool fine = false;
@@ -918,10 +966,18 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
     // dmax (r_c <= r0c up to the margins), nor than the largest lane bound
     // plus its |q - c| (each lane's upper end is at most its own bound).
     const float xmax = wave_max(part ? xq : 0.0f);
-    const float rl = fminf((r0c * 1.0001f + 2.0f * dmax) * 1.0004f, xmax * 1.0005f) + 2e-30f;
+    float rl = fminf((r0c * 1.0001f + 2.0f * dmax) * 1.0004f, xmax * 1.0005f) + 2e-30f;
     nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, true, &fine,
-                    split);
-    if (nr == kSplitRun) return kSplitRun;
+                    kListGiveUp);
+    if (nr == kNoList) {
+      if (split) return kNoList;
+      // one lane, c = q (dmax = 0): r0c holds k photons of c
+      if (count_stats && lane == 0) atomicAdd(&counters[kStatNarrowed], 1u);
+      const float r02 = r0c * r0c;
+      r0c = leaf_bracket(D, sh, lane, cx, cy, cz, r02 * 0.25f, 0.0f, r02, r02, k_eff);
+      rl = fminf((r0c * 1.0001f + 2.0f * dmax) * 1.0004f, xmax * 1.0005f) + 2e-30f;
+      nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, true, &fine);
+    }
     if (count_stats && lane == 0) atomicAdd(&counters[kStatCand1], *total);
     float t_lo = 0.0f, t_hi = INFINITY;
     if (centre_bound(D, sh, lane, nr, *total, fine, cx, cy, cz, r0c * r0c * 1.0001f + 1e-30f,
@@ -1213,8 +1269,8 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         bool fine = false;
         nr = build_list(D, cx, cy, cz, rs2, sh, lane, &total,
                         count_stats ? &counters[kStatOverflow] : nullptr, true, &fine,
-                        c1 - c0 > 1);
-        if (nr == kSplitRun) {
+                        c1 - c0 > 1 ? kListGiveUp : kListCoarsen);
+        if (nr == kNoList) {
           if (count_stats && lane == 0) atomicAdd(&counters[kStatSplit], 1u);
           starts |= 1ull << ((c0 + c1) >> 1);
           continue;
@@ -1238,7 +1294,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
                         counters, count_stats, st, pt, &total, c1 - c0 > 1);
         // A run whose sphere needs more subtrees than a list holds would
         // sweep coarse subtrees far beyond it: its halves go separately.
-        if (nr == kSplitRun) {
+        if (nr == kNoList) {
           if (count_stats && lane == 0) atomicAdd(&counters[kStatSplit], 1u);
           starts |= 1ull << ((c0 + c1) >> 1);
           continue;
@@ -1769,31 +1825,7 @@ __global__ __launch_bounds__(kWave) void anchor_bound_kernel(GatherLaunch L) {
     }
     sp2 = wave_max(sp2);
     float rb = fminf(sp2 * (static_cast<float>(k_eff) / static_cast<float>(na)) * 2.25f, r0sq);
-    float few = 0.0f;       // largest rb seen holding fewer than k_eff photons
-    float over = INFINITY;  // smallest rb seen whose leaf list overflowed
-    float r = sqrt_up(r0sq) * 1.00002f + 1e-30f;
-    for (int attempt = 0; attempt < 7 && rb > 0.0f; ++attempt) {
-      uint32_t total = 0;
-      bool fine = false;
-      const uint32_t nr = build_list(D, cx, cy, cz, rb, sh, lane, &total, nullptr, true, &fine);
-      if (!fine) {
-        over = rb;
-        rb = few > 0.0f ? sqrtf(few * over) : rb * 0.25f;
-        continue;
-      }
-      float t_lo = 0.0f, t_hi = INFINITY;
-      if (centre_bound(D, sh, lane, nr, total, fine, cx, cy, cz, rb, k_eff, &t_lo, &t_hi)) {
-        const float rb2 = t_hi;
-        r = fminf(r, sqrt_up(rb2) * 1.00002f + 1e-30f);
-        if (centre_bound(D, sh, lane, nr, total, fine, cx, cy, cz, rb2, k_eff, &t_lo, &t_hi)) {
-          r = fminf(r, sqrt_up(t_hi) * 1.00002f + 1e-30f);
-        }
-        break;
-      }
-      if (!(rb < r0sq)) break;  // the slot sphere held its k_eff photons: cannot happen
-      few = rb;
-      rb = over < INFINITY ? sqrtf(few * over) : fminf(rb * 4.0f, r0sq);
-    }
+    const float r = leaf_bracket(D, sh, lane, cx, cy, cz, rb, 0.0f, INFINITY, r0sq, k_eff);
     if (lane == 0) D.anchor_out[4 * a + 3] = r;
     __syncthreads();
   }

@@ -62,7 +62,8 @@ constexpr int kStatOverflow = kStatBase + 4;   // candidate lists cut short
 constexpr int kStatRuns = kStatBase + 5;       // runs of nearby lanes
 constexpr int kStatCand3 = kStatBase + 6;      // selection-pass candidate photons
 constexpr int kStatSplit = kStatBase + 7;      // runs split for an overflowing list
-static_assert(kStatSplit < 32, "statistics end below the phase counters");
+constexpr int kStatNarrowed = kStatBase + 8;   // one-lane runs whose sphere was narrowed
+static_assert(kStatNarrowed < 32, "statistics end below the phase counters");
 // Wave cycles per phase of a group (uint64 each, from word kPhaseBase).
 constexpr int kPhaseBase = 32;
 constexpr int kPhases = 8;

@@ -77,6 +77,7 @@ constexpr int kCapDepth = kMaxRanges >= 256 ? 8 : kMaxRanges >= 128 ? 7 : kMaxRa
 static_assert(kMaxRanges >= 32, "a depth-capped walk must fit the list");
 static_assert((1 << kCapDepth) <= kMaxRanges, "depth cap");
 constexpr int kFront = 256;       // breadth-first frontier capacity
+constexpr int kNarrowTries = 6;   // leaf lists tried when narrowing a loose sphere
 constexpr int kDrhoSlots = 64;    // adjoint: materials accumulated in LDS
 constexpr int kCentreBins = 256;  // centre histogram bins
 constexpr int kSlotCap = PM_SLOT_CAP;  // photons of an explicit slot list
@@ -388,14 +389,15 @@ __device__ uint32_t list_prefix(DescRef D, GatherShared& sh, int lane, uint32_t 
 // overflowed (at most kCapDepth levels: 2^kCapDepth <= kMaxRanges subtrees, so
 // it fits).  A frontier that would overflow counts as an overflow when giving
 // up, else emits its nodes whole; a walk that still overflows lists the root
-// (never a truncated list).  *fine (when given)
+// (never a truncated list).  Giving up also returns kNoList for a list of
+// more than max_total photons (whole inner nodes can make one large).  *fine (when given)
 // tells whether every entry is a single leaf (asked with leaf_only and no
 // coarsening was needed).  A returned list coarser than the sphere asks adds
 // 1 to *stat (when given), 1000 when it is the whole map.
 __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2,
                                GatherShared& sh, int lane, uint32_t* total, uint32_t* stat,
                                bool leaf_only = false, bool* fine = nullptr,
-                               int give_up = kListCoarsen) {
+                               int give_up = kListCoarsen, uint32_t max_total = 0xffffffffu) {
   const bool split = give_up != kListCoarsen;
   const uint32_t first_leaf = static_cast<uint32_t>(D.first_leaf);
   const int depth_leaf = 31 - __builtin_clz(first_leaf);
@@ -483,7 +485,8 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
   }
   if (stat && lane == 0 && coarse && r2 < INFINITY && depth_leaf > 0) atomicAdd(stat, whole ? 1000u : 1u);
   if (fine) *fine = leaf_only && !coarse && !whole && max_depth == depth_leaf;
-  return list_prefix(D, sh, lane, nr, total);
+  nr = list_prefix(D, sh, lane, nr, total);
+  return split && *total > max_total ? kNoList : nr;
 }
 
 // Drops the entries of the current list whose boxes miss the sphere
@@ -935,6 +938,17 @@ struct PhaseTimer {
   }
 };
 
+// Most photons a run's candidate list may hold before the run is halved (or,
+// for one lane, its sphere narrowed): the halves cost about what the run
+// costs once its sphere holds more than a few k photons per lane, and one
+// lane needs more than that only when its bound is loose.
+__device__ __forceinline__ uint32_t run_list_cap(bool part, uint32_t k_eff) {
+  const uint64_t n_part = static_cast<uint64_t>(__popcll(__ballot(part)));
+  const uint64_t per = 2ull * k_eff + 128ull;
+  const uint64_t cap = n_part > 1 ? n_part * per : 4ull * per;
+  return static_cast<uint32_t>(cap < 0xffffffffull ? cap : 0xffffffffull);
+}
+
 // Selection for the lanes of one run (part), with centre c, lane distances
 // dq >= |q - c| and their maximum dmax.  r0sq is the lane's guaranteed bound
 // on its k-th FP32 d^2.  Afterwards the neighbour set of a part lane is
@@ -942,15 +956,16 @@ struct PhaseTimer {
 // holds at most kStashCap keys.  Leaves the run's candidate list (covering
 // every key below the lane bounds) in sh.rl.rnode/sh.rl.rpre: returns its entry
 // count and its photon count in *total.  When the centre sweep's sphere needs
-// more subtrees than a list holds, a run of several lanes (split) gets
-// kNoList and nothing is selected: the caller halves it.  A run of one lane
-// then has a loose bound (its own): the sphere is narrowed by leaf_bracket
-// first, so no coarse list is swept.
+// more subtrees than a list holds, or holds more than 2 k + 128 photons per
+// lane (run_list_cap), a run of several lanes gets kNoList and nothing is selected:
+// the caller halves it.  A run of one lane then has a loose bound (its own):
+// the sphere is narrowed by leaf_bracket first, so no coarse or oversized
+// list is swept.
 __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool part, float r0sq,
                                float cx, float cy, float cz, float dq, float dmax,
                                uint32_t k_eff, GatherShared& sh, int lane, uint32_t* counters,
                                int count_stats, SelState& st, PhaseTimer& pt,
-                               uint32_t* total, bool split) {
+                               uint32_t* total) {
   uint32_t* cut_stat = count_stats ? &counters[kStatOverflow] : nullptr;
   // ---- centre bracket: lane q's k nearest photons lie within X_q of q, hence
   // within X_q + |q - c| of c, so the smallest such sum bounds the centre's
@@ -966,25 +981,52 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
     // dmax (r_c <= r0c up to the margins), nor than the largest lane bound
     // plus its |q - c| (each lane's upper end is at most its own bound).
     const float xmax = wave_max(part ? xq : 0.0f);
-    float rl = fminf((r0c * 1.0001f + 2.0f * dmax) * 1.0004f, xmax * 1.0005f) + 2e-30f;
-    nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, true, &fine,
-                    kListGiveUp);
-    if (nr == kNoList) {
-      if (split) return kNoList;
-      // one lane, c = q (dmax = 0): r0c holds k photons of c
-      if (count_stats && lane == 0) atomicAdd(&counters[kStatNarrowed], 1u);
-      const float r02 = r0c * r0c;
-      r0c = leaf_bracket(D, sh, lane, cx, cy, cz, r02 * 0.25f, 0.0f, r02, r02, k_eff);
-      rl = fminf((r0c * 1.0001f + 2.0f * dmax) * 1.0004f, xmax * 1.0005f) + 2e-30f;
-      nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, true, &fine);
+    // A run of one lane whose list overflows has a loose bound (its own, c =
+    // q, dmax = 0): its sphere is narrowed by a geometric search over leaf
+    // lists (halved while the list overflows, grown towards the last
+    // overflowing radius while it holds fewer than k photons), when a leaf
+    // list can hold k photons at all.  After kNarrowTries lists, or without
+    // that, the lane's own bound gets a coarse list.  Each pass of the loop
+    // either ends it or moves on, and the coarse pass always ends it.
+    const float r0c_in = r0c;
+    const bool narrowable =
+        2ull * k_eff <= static_cast<uint64_t>(kMaxRanges) * static_cast<uint64_t>(D.leaf_size);
+    float few = 0.0f, over = INFINITY;  // radii: fewer than k photons / list overflowed
+    int give = kListGiveUp;
+    for (int t = 0;; ++t) {
+      const float rl = fminf((r0c * 1.0001f + 2.0f * dmax) * 1.0004f, xmax * 1.0005f) + 2e-30f;
+      nr = build_list(D, cx, cy, cz, rl * rl + 1e-30f, sh, lane, total, cut_stat, true, &fine,
+                      give, run_list_cap(part, k_eff));
+      if (nr == kNoList) {
+        if (__popcll(__ballot(part)) > 1) return kNoList;
+        if (t == 0 && count_stats && lane == 0) atomicAdd(&counters[kStatNarrowed], 1u);
+        over = r0c;
+        if (narrowable && t < kNarrowTries) {
+          r0c = few > 0.0f ? sqrtf(few * over) : 0.5f * r0c;
+          give = kListLeavesOnly;
+        } else {
+          r0c = r0c_in;
+          give = kListCoarsen;
+        }
+        continue;
+      }
+      float t_lo = 0.0f, t_hi = INFINITY;
+      if (centre_bound(D, sh, lane, nr, *total, fine, cx, cy, cz, r0c * r0c * 1.0001f + 1e-30f,
+                       k_eff, &t_lo, &t_hi)) {
+        rc_hi = sqrt_up(t_hi) * 1.00001f + 1e-30f;
+        rc_lo = sqrt_down(t_lo) * 0.99999f;
+        break;
+      }
+      if (give != kListLeavesOnly) break;  // the lane's own bound: no bracket
+      few = r0c;  // narrowed too far
+      if (t < kNarrowTries) {
+        r0c = sqrtf(few * over);
+      } else {
+        r0c = r0c_in;
+        give = kListCoarsen;
+      }
     }
     if (count_stats && lane == 0) atomicAdd(&counters[kStatCand1], *total);
-    float t_lo = 0.0f, t_hi = INFINITY;
-    if (centre_bound(D, sh, lane, nr, *total, fine, cx, cy, cz, r0c * r0c * 1.0001f + 1e-30f,
-                     k_eff, &t_lo, &t_hi)) {
-      rc_hi = sqrt_up(t_hi) * 1.00001f + 1e-30f;
-      rc_lo = sqrt_down(t_lo) * 0.99999f;
-    }
   }
   pt.mark(1, lane);
 
@@ -1022,7 +1064,11 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
     } else if (attempt == 0 && rs2 < INFINITY && fine) {
       nr = filter_list(D, sh, lane, nr, cx, cy, cz, rs2, total);
     } else {
-      nr = build_list(D, cx, cy, cz, rs2, sh, lane, total, cut_stat);
+      // (a run of several lanes is halved rather than given a coarse list)
+      const bool halve = attempt == 0 && __popcll(__ballot(part)) > 1;
+      nr = build_list(D, cx, cy, cz, rs2, sh, lane, total, cut_stat, false, nullptr,
+                      halve ? kListGiveUp : kListCoarsen, run_list_cap(part, k_eff));
+      if (nr == kNoList) return kNoList;
     }
     if (count_stats && lane == 0) atomicAdd(&counters[kStatCand2], *total);
     pt.mark(2, lane);
@@ -1291,7 +1337,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         pt.mark(2, lane);
       } else {
         nr = run_select(D, qx, qy, qz, part, r0sq, cx, cy, cz, dq, dmax, k_eff, sh, lane,
-                        counters, count_stats, st, pt, &total, c1 - c0 > 1);
+                        counters, count_stats, st, pt, &total);
         // A run whose sphere needs more subtrees than a list holds would
         // sweep coarse subtrees far beyond it: its halves go separately.
         if (nr == kNoList) {

@@ -77,7 +77,7 @@ constexpr int kCapDepth = kMaxRanges >= 256 ? 8 : kMaxRanges >= 128 ? 7 : kMaxRa
 static_assert(kMaxRanges >= 32, "a depth-capped walk must fit the list");
 static_assert((1 << kCapDepth) <= kMaxRanges, "depth cap");
 constexpr int kFront = 256;       // breadth-first frontier capacity
-constexpr int kNarrowTries = 6;   // leaf lists tried when narrowing a loose sphere
+constexpr int kNarrowTries = 12;  // lists tried when narrowing a loose sphere
 constexpr int kDrhoSlots = 64;    // adjoint: materials accumulated in LDS
 constexpr int kCentreBins = 256;  // centre histogram bins
 constexpr int kSlotCap = PM_SLOT_CAP;  // photons of an explicit slot list
@@ -981,17 +981,19 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
     // dmax (r_c <= r0c up to the margins), nor than the largest lane bound
     // plus its |q - c| (each lane's upper end is at most its own bound).
     const float xmax = wave_max(part ? xq : 0.0f);
-    // A run of one lane whose list overflows has a loose bound (its own, c =
-    // q, dmax = 0): its sphere is narrowed by a geometric search over leaf
-    // lists (halved while the list overflows, grown towards the last
-    // overflowing radius while it holds fewer than k photons), when a leaf
-    // list can hold k photons at all.  After kNarrowTries lists, or without
-    // that, the lane's own bound gets a coarse list.  Each pass of the loop
-    // either ends it or moves on, and the coarse pass always ends it.
+    // A run of one lane whose list overflows (or holds more than
+    // run_list_cap photons) has a loose bound (its own, c = q, dmax = 0): its
+    // sphere is narrowed by a geometric search (halved while the list
+    // overflows, then bisected in log radius between the largest sphere seen
+    // holding fewer than k photons and the smallest that overflowed), when
+    // a leaf list can hold k photons at all.  After kNarrowTries lists, or
+    // without that, the lane's own bound gets a coarse list.  Each pass of
+    // the loop either ends it or moves on, and the coarse pass always ends it.
     const float r0c_in = r0c;
     const bool narrowable =
         2ull * k_eff <= static_cast<uint64_t>(kMaxRanges) * static_cast<uint64_t>(D.leaf_size);
     float few = 0.0f, over = INFINITY;  // radii: fewer than k photons / list overflowed
+    bool narrowing = false;
     int give = kListGiveUp;
     for (int t = 0;; ++t) {
       const float rl = fminf((r0c * 1.0001f + 2.0f * dmax) * 1.0004f, xmax * 1.0005f) + 2e-30f;
@@ -1001,9 +1003,9 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
         if (__popcll(__ballot(part)) > 1) return kNoList;
         if (t == 0 && count_stats && lane == 0) atomicAdd(&counters[kStatNarrowed], 1u);
         over = r0c;
-        if (narrowable && t < kNarrowTries) {
+        narrowing = narrowable && t < kNarrowTries;
+        if (narrowing) {
           r0c = few > 0.0f ? sqrtf(few * over) : 0.5f * r0c;
-          give = kListLeavesOnly;
         } else {
           r0c = r0c_in;
           give = kListCoarsen;
@@ -1017,9 +1019,10 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
         rc_lo = sqrt_down(t_lo) * 0.99999f;
         break;
       }
-      if (give != kListLeavesOnly) break;  // the lane's own bound: no bracket
+      if (!narrowing) break;  // the lane's own bound: no bracket
       few = r0c;  // narrowed too far
-      if (t < kNarrowTries) {
+      narrowing = t < kNarrowTries;
+      if (narrowing) {
         r0c = sqrtf(few * over);
       } else {
         r0c = r0c_in;

@@ -145,3 +145,56 @@ def test_photons_beyond_the_domain_are_rejected(gpu):
     with pytest.raises(gpu.PmError) as e:
         pmap.build()
     assert e.value.code == 1
+
+
+def test_loose_bounds_and_crowded_spheres_stay_exact(gpu, oracle):
+    """A dense cluster (300k photons within 0.02 of the centre) inside a
+    sparse field (3,000 photons in the unit cube), k = 200, with sparse
+    queries all over the cube and a shell of queries just outside the
+    cluster.  Radius bounds there are loose and run spheres take in the
+    cluster: runs are halved and one-query spheres narrowed instead of
+    sweeping coarse lists (no coarsened list, no fallback), and every
+    neighbour set equals the oracle's."""
+    from util import materials, query_records
+
+    rng = np.random.default_rng(21)
+    c = np.float32([0.5, 0.5, 0.5])
+    d = rng.normal(size=(300_000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    dense = c + (0.02 * rng.random((300_000, 1)) ** (1 / 3) * d).astype(np.float32)
+    sparse = rng.random((3000, 3), dtype=np.float32)
+    pos = np.concatenate([dense, sparse]).astype(np.float32)
+    wi = -pos / np.linalg.norm(pos, axis=1, keepdims=True)
+    flux = rng.uniform(0.1, 1.0, pos.shape).astype(np.float32)
+    s = rng.normal(size=(1024, 3))
+    s /= np.linalg.norm(s, axis=1, keepdims=True)
+    shell = c + (rng.uniform(0.022, 0.08, (1024, 1)) * s).astype(np.float32)
+    qpos = np.concatenate([rng.random((4096, 3), dtype=np.float32), shell]).astype(np.float32)
+    q = query_records(rng, qpos)
+    mats = materials(3)
+    k = 200
+    pmap = gpu.PhotonMap()
+    pmap.set_photons_soa(pos, wi, flux)
+    pmap.build()
+    ctx = pmap.ctx
+    from photon_mapping_amd import _lib
+
+    ctx.set_option(_lib.PM_OPT_COUNT_STATS, 1)
+    try:
+        rgb, r2, key = pmap.gather_radiance(q, mats, k, 1e4)
+        st = ctx.stats()
+    finally:
+        ctx.set_option(_lib.PM_OPT_COUNT_STATS, 0)
+    print("crowded spheres:", {x: st[x] for x in ("runs", "split_runs", "narrowed_runs",
+                                                   "cut_lists", "fallbacks")}, flush=True)
+    assert st["cut_lists"] == 0 and st["fallbacks"] == 0
+    assert st["split_runs"] > 0 and st["narrowed_runs"] > 0  # both paths ran
+    idx, er2 = oracle.KdTree(pos).knn(q[:, :3], k)
+    np.testing.assert_array_equal(r2.view(np.uint32), er2.view(np.uint32))
+    assert (key & np.uint64(0xFFFFFFFF)).astype(np.int64).tolist() == idx[:, 0].tolist()
+    expect = oracle.radiance_fp64(wi, flux, q, mats, idx, er2, 1e4)
+    nz = expect != 0
+    assert (rgb[~nz] == 0).all()
+    np.testing.assert_allclose(rgb[nz], expect[nz], rtol=1e-5)
+    kidx, kr2 = pmap.knn(q[:, :3], k)
+    np.testing.assert_array_equal(kidx, idx)

@@ -78,6 +78,8 @@ static_assert(kMaxRanges >= 32, "a depth-capped walk must fit the list");
 static_assert((1 << kCapDepth) <= kMaxRanges, "depth cap");
 constexpr int kFront = 256;       // breadth-first frontier capacity
 constexpr int kNarrowTries = 12;  // lists tried when narrowing a loose sphere
+constexpr int kSparseRuns = 3;        // runs of a group at which runs are cut wider
+constexpr float kSparseSplit = 1.67f;  // ... by this factor on the run split
 constexpr int kDrhoSlots = 64;    // adjoint: materials accumulated in LDS
 constexpr int kCentreBins = 256;  // centre histogram bins
 constexpr int kSlotCap = PM_SLOT_CAP;  // photons of an explicit slot list
@@ -1275,12 +1277,20 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
     // ---- runs of nearby lanes ------------------------------------------------
     const float scale = all ? INFINITY
                             : wave_min(active ? sqrt_up(cull_bound(r0sq)) : INFINITY);
-    const float thr = split * scale;
+    // A group whose queries fall apart into several runs (sparse queries)
+    // pays a run's fixed cost often for few lanes: its runs are cut at a
+    // larger spread, trading candidates per run for fewer runs (measured on
+    // uneven workloads; neutral on dense ones, whose groups stay whole).
+    float thr = split * scale;
     const float px = __shfl_up(qx, 1, kWave);
     const float py = __shfl_up(qy, 1, kWave);
     const float pz = __shfl_up(qz, 1, kWave);
     const float gap = fmaxf(fabsf(qx - px), fmaxf(fabsf(qy - py), fabsf(qz - pz)));
     uint64_t starts = __ballot(active && lane > 0 && gap > thr) | 1ull;
+    if (__popcll(starts) >= kSparseRuns) {
+      thr *= kSparseSplit;
+      starts = __ballot(active && lane > 0 && gap > thr) | 1ull;
+    }
     uint32_t c0 = 0;
     while (c0 < n_act) {
       const uint64_t later = c0 >= 63 ? 0ull : (starts >> (c0 + 1)) << (c0 + 1);

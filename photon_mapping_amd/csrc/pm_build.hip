@@ -243,28 +243,11 @@ __global__ __launch_bounds__(kThreads) void anchor_tab_kernel(
 
 // ---- query preparation ---------------------------------------------------------
 
-__device__ __forceinline__ int find_inst(const int64_t* __restrict__ prefix, int n_inst,
-                                         int64_t i) {
-  int lo = 0, hi = n_inst - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= i) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// Sort key of every query: its instance and its Hilbert code, in the
-// photons' order along the curve; the value is the global query index.
-template <typename K>
-__global__ __launch_bounds__(kThreads) void query_keys_kernel(
-    const GatherDesc* __restrict__ descs, const int64_t* __restrict__ q_prefix,
-    int n_inst, int64_t total, K* __restrict__ keys, uint32_t* __restrict__ vals) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (i >= total) return;
-  const int inst = n_inst > 1 ? find_inst(q_prefix, n_inst, i) : 0;
-  const GatherDesc& D = descs[inst];
-  const int64_t li = i - q_prefix[inst];
+// Hilbert code of local query li of an instance, in the photons' order along
+// the curve.  A query outside the supported domain (a non-finite coordinate,
+// or one beyond +-2^60) gets the invalid bit: it sorts after every valid
+// query of its instance and is left out of the gather (its outputs say so).
+__device__ __forceinline__ uint32_t query_code(const GatherDesc& D, int64_t li) {
   float x, y, z;
   if (D.q) {
     const pm_query& q = D.q[li];
@@ -272,36 +255,17 @@ __global__ __launch_bounds__(kThreads) void query_keys_kernel(
   } else {
     x = D.q_soa[li]; y = D.q_soa[D.m + li]; z = D.q_soa[2 * D.m + li];
   }
-  // A query outside the supported domain (a non-finite coordinate, or one
-  // beyond +-2^60) sorts after every valid query of its instance and is
-  // left out of the gather (its outputs say so).
   const bool ok = fabsf(x) <= kCoordLimit && fabsf(y) <= kCoordLimit && fabsf(z) <= kCoordLimit;
-  const uint32_t code = ok ? hilbert3(x, y, z, D.qlo, D.qscale) : kInvalidKeyBit;
-  keys[i] = static_cast<K>((static_cast<uint64_t>(inst) << 32) | code);
-  vals[i] = static_cast<uint32_t>(i);
+  return ok ? hilbert3(x, y, z, D.qlo, D.qscale) : kInvalidKeyBit;
 }
 
-// Nearest radius anchor of every sorted query: the last anchor whose first
-// code is <= the query's code (upper bound, one back).  It runs over the
-// sorted keys, so reads and writes are coalesced and neighbouring queries
-// search the same table bucket; q_anchor is indexed by sorted position.  An
-// instance's sorted positions are its global query indices' range, since
-// the instance leads the key.
-template <typename K>
-__global__ __launch_bounds__(kThreads) void query_anchor_kernel(
-    const GatherDesc* __restrict__ descs, const int64_t* __restrict__ q_prefix, int n_inst,
-    int64_t total, const K* __restrict__ sorted_keys, uint32_t* __restrict__ q_anchor) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (i >= total) return;
-  const int inst = n_inst > 1 ? find_inst(q_prefix, n_inst, i) : 0;
-  const GatherDesc& D = descs[inst];
-  const uint32_t code = static_cast<uint32_t>(sorted_keys[i] & 0xffffffffu);
-  if (code & kInvalidKeyBit) {  // no anchor for a query the gather leaves out
-    q_anchor[i] = 0;
-    return;
-  }
-  // The upper bound of the code lies inside its prefix bucket of the table:
-  // anchors before the bucket have smaller codes, those after it larger.
+// Nearest radius anchor of a sorted query's code: the last anchor whose
+// first code is <= it (upper bound, one back); 0 for an invalid query, which
+// the gather leaves out.  The upper bound lies inside the code's prefix
+// bucket of the table: anchors before the bucket have smaller codes, those
+// after it larger.
+__device__ __forceinline__ uint32_t anchor_of(const GatherDesc& D, uint32_t code) {
+  if (code & kInvalidKeyBit) return 0;
   int lo = 0, hi = 0;
   if (D.n_anchors > 0) {
     const uint32_t pre = code >> (kCodeBits - kAnchorTabBits);
@@ -313,7 +277,111 @@ __global__ __launch_bounds__(kThreads) void query_anchor_kernel(
     if (D.anchor_code[m] <= code) lo = m + 1;
     else hi = m;
   }
-  q_anchor[i] = static_cast<uint32_t>(lo > 0 ? lo - 1 : 0);
+  return static_cast<uint32_t>(lo > 0 ? lo - 1 : 0);
+}
+
+// Sort key of every query of one instance; the value is the query index.
+__global__ __launch_bounds__(kThreads) void query_keys_kernel(
+    const GatherDesc* __restrict__ descs, int64_t total, uint32_t* __restrict__ keys,
+    uint32_t* __restrict__ vals) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= total) return;
+  keys[i] = query_code(descs[0], i);
+  vals[i] = static_cast<uint32_t>(i);
+}
+
+// Nearest anchor of every sorted query of one instance, over the sorted keys:
+// reads and writes are coalesced and neighbouring queries search the same
+// table bucket; q_anchor is indexed by sorted position.
+__global__ __launch_bounds__(kThreads) void query_anchor_kernel(
+    const GatherDesc* __restrict__ descs, int64_t total, const uint32_t* __restrict__ sorted_keys,
+    uint32_t* __restrict__ q_anchor) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= total) return;
+  q_anchor[i] = anchor_of(descs[0], sorted_keys[i]);
+}
+
+// ---- several instances: one segmented sort -------------------------------------
+// The queries of instance i occupy [q_prefix[i], q_prefix[i + 1]) before and
+// after the sort, so each is a segment of one sort over 32-bit codes, cut
+// into tiles that never straddle two instances; a tile's instance then comes
+// with it, with no search per query.
+
+// Tile table of the segmented sort (one block): instance s gets
+// ceil(m_s / kSortTileElems) tiles, its counts the table range from 256 times
+// its first tile, digit-major within it; the tiles past the last instance's
+// are empty and count into a range of their own.
+__global__ __launch_bounds__(kThreads) void sort_tiles_kernel(const int64_t* __restrict__ q_prefix,
+                                                              int n_seg, SortTile* __restrict__ tiles,
+                                                              int64_t n_tiles) {
+  __shared__ uint32_t wsum[kThreads / kWave];
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const int w = t / kWave;
+  uint32_t run = 0;  // tiles of the instances before this chunk
+  for (int s0 = 0; s0 < n_seg; s0 += kThreads) {
+    const int sg = s0 + t;
+    int64_t b = 0, m = 0;
+    if (sg < n_seg) {
+      b = q_prefix[sg];
+      m = q_prefix[sg + 1] - b;
+    }
+    const uint32_t nt = static_cast<uint32_t>((m + kSortTileElems - 1) / kSortTileElems);
+    uint32_t incl = nt;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o, kWave);
+      if (lane >= o) incl += y;
+    }
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    uint32_t before = 0, chunk = 0;
+    for (int i = 0; i < kThreads / kWave; ++i) {
+      if (i < w) before += wsum[i];
+      chunk += wsum[i];
+    }
+    const uint32_t first = run + before + incl - nt;
+    for (uint32_t j = 0; j < nt; ++j) {
+      SortTile tl;
+      tl.base = b + static_cast<int64_t>(j) * kSortTileElems;
+      tl.n = static_cast<uint32_t>(min<int64_t>(kSortTileElems, m - static_cast<int64_t>(j) * kSortTileElems));
+      tl.cbase = first * 256u + j;
+      tl.stride = nt;
+      tl.seg = static_cast<uint32_t>(sg);
+      tiles[first + j] = tl;
+    }
+    run += chunk;
+    __syncthreads();
+  }
+  for (int64_t i = static_cast<int64_t>(run) + t; i < n_tiles; i += kThreads) {
+    SortTile tl;
+    tl.base = 0;
+    tl.n = 0;
+    tl.cbase = static_cast<uint32_t>(i) * 256u;
+    tl.stride = 1;
+    tl.seg = 0;
+    tiles[i] = tl;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void query_keys_seg_kernel(
+    const GatherDesc* __restrict__ descs, const int64_t* __restrict__ q_prefix,
+    const SortTile* __restrict__ tiles, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const SortTile tl = tiles[blockIdx.x];
+  const GatherDesc& D = descs[tl.seg];
+  const int64_t off = tl.base - q_prefix[tl.seg];
+  for (uint32_t j = threadIdx.x; j < tl.n; j += kThreads) {
+    keys[tl.base + j] = query_code(D, off + j);
+    vals[tl.base + j] = static_cast<uint32_t>(tl.base + j);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void query_anchor_seg_kernel(
+    const GatherDesc* __restrict__ descs, const SortTile* __restrict__ tiles,
+    const uint32_t* __restrict__ sorted_keys, uint32_t* __restrict__ q_anchor) {
+  const SortTile tl = tiles[blockIdx.x];
+  const GatherDesc& D = descs[tl.seg];
+  for (uint32_t j = threadIdx.x; j < tl.n; j += kThreads)
+    q_anchor[tl.base + j] = anchor_of(D, sorted_keys[tl.base + j]);
 }
 
 // ---- kNN row sort: neighbour indices, farthest first ------------------------------
@@ -652,15 +720,23 @@ int ensure_anchors(PhotonMapImpl* map, int k_eff, hipStream_t s, const float4** 
 }
 
 int prepare_reserve(Workspace* ws, int n_inst, int64_t total_q, int64_t total_groups) {
-  const size_t kbytes = n_inst > 1 ? sizeof(uint64_t) : sizeof(uint32_t);
+  (void)total_groups;
   const int64_t nq = std::max<int64_t>(total_q, 1);
-  if (int rc = ws->keys.reserve(nq * kbytes)) return rc;
-  if (int rc = ws->keys2.reserve(nq * kbytes)) return rc;
+  if (int rc = ws->keys.reserve(nq * sizeof(uint32_t))) return rc;
+  if (int rc = ws->keys2.reserve(nq * sizeof(uint32_t))) return rc;
   if (int rc = ws->vals.reserve(nq * sizeof(uint32_t))) return rc;
   if (int rc = ws->vals2.reserve(nq * sizeof(uint32_t))) return rc;
   if (int rc = ws->q_anchor.reserve(nq * sizeof(uint32_t))) return rc;
   if (int rc = ws->counters.reserve(64 * sizeof(uint32_t))) return rc;
-  return sort_reserve(ws, nq);
+  if (n_inst <= 1) return sort_reserve(ws, nq);
+  if (int rc = ws->tiles.reserve(static_cast<size_t>(sort_seg_tiles(nq, n_inst)) *
+                                 sizeof(SortTile)))
+    return rc;
+  if (int rc = ws->counts.reserve(static_cast<size_t>(sort_seg_count_words(nq, n_inst)) *
+                                  sizeof(uint32_t)))
+    return rc;
+  return ws->partials.reserve(static_cast<size_t>(sort_seg_partial_words(nq, n_inst)) *
+                              sizeof(uint32_t));
 }
 
 int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
@@ -681,6 +757,7 @@ int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
   qs.q_anchor = ws->q_anchor.as<uint32_t>();
   qs.counts = ws->counts.as<uint32_t>();
   qs.partials = ws->partials.as<uint32_t>();
+  qs.tiles = n_inst > 1 ? ws->tiles.as<SortTile>() : nullptr;
   return prepare_queries_raw(qs, n_inst, dev_descs, dev_q_prefix, total_q, s, perm_out,
                              need_anchors);
 }
@@ -692,37 +769,35 @@ int prepare_queries_raw(const QueryScratch& qs, int n_inst, const GatherDesc* de
     *perm_out = nullptr;
     return PM_OK;
   }
-  const bool wide = n_inst > 1;
+  uint32_t* k0 = static_cast<uint32_t*>(qs.keys);
+  uint32_t* k1 = static_cast<uint32_t*>(qs.keys2);
   uint32_t* v0 = qs.vals;
   uint32_t* v1 = qs.vals2;
   int which = 0;
-  const dim3 grid(ceil_div(total_q, kThreads));
-  if (!wide) {
-    uint32_t* k0 = static_cast<uint32_t*>(qs.keys);
-    uint32_t* k1 = static_cast<uint32_t*>(qs.keys2);
-    hipLaunchKernelGGL(query_keys_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, n_inst, total_q, k0, v0);
+  if (n_inst <= 1) {
+    const dim3 grid(ceil_div(total_q, kThreads));
+    hipLaunchKernelGGL(query_keys_kernel, grid, dim3(kThreads), 0, s, dev_descs, total_q, k0, v0);
     PM_HIP(hipGetLastError());
     if (int rc = radix_sort_u32_raw(qs.counts, qs.partials, k0, v0, k1, v1, total_q, 32, s,
                                     &which, kQuerySortLoBit))
       return rc;
     if (need_anchors)
-      hipLaunchKernelGGL(query_anchor_kernel<uint32_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                         dev_q_prefix, n_inst, total_q, which ? k1 : k0, qs.q_anchor);
+      hipLaunchKernelGGL(query_anchor_kernel, grid, dim3(kThreads), 0, s, dev_descs, total_q,
+                         which ? k1 : k0, qs.q_anchor);
   } else {
-    uint64_t* k0 = static_cast<uint64_t*>(qs.keys);
-    uint64_t* k1 = static_cast<uint64_t*>(qs.keys2);
-    int ib = 1;
-    while ((1 << ib) < n_inst) ++ib;
-    hipLaunchKernelGGL(query_keys_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                       dev_q_prefix, n_inst, total_q, k0, v0);
+    if (!qs.tiles) return fail(PM_ERR_INVALID_ARG, "a batch needs the sort's tile table");
+    const int64_t n_tiles = sort_seg_tiles(total_q, n_inst);
+    hipLaunchKernelGGL(sort_tiles_kernel, dim3(1), dim3(kThreads), 0, s, dev_q_prefix, n_inst,
+                       qs.tiles, n_tiles);
+    hipLaunchKernelGGL(query_keys_seg_kernel, dim3(n_tiles), dim3(kThreads), 0, s, dev_descs,
+                       dev_q_prefix, qs.tiles, k0, v0);
     PM_HIP(hipGetLastError());
-    if (int rc = radix_sort_u64_raw(qs.counts, qs.partials, k0, v0, k1, v1, total_q, 32 + ib,
-                                    s, &which, kQuerySortLoBit))
+    if (int rc = radix_sort_seg_u32_raw(qs.counts, qs.partials, qs.tiles, n_tiles, k0, v0, k1, v1,
+                                        total_q, 32, s, &which, kQuerySortLoBit))
       return rc;
     if (need_anchors)
-      hipLaunchKernelGGL(query_anchor_kernel<uint64_t>, grid, dim3(kThreads), 0, s, dev_descs,
-                         dev_q_prefix, n_inst, total_q, which ? k1 : k0, qs.q_anchor);
+      hipLaunchKernelGGL(query_anchor_seg_kernel, dim3(n_tiles), dim3(kThreads), 0, s, dev_descs,
+                         qs.tiles, which ? k1 : k0, qs.q_anchor);
   }
   PM_HIP(hipGetLastError());
   *perm_out = which ? v1 : v0;

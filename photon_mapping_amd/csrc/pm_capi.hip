@@ -919,6 +919,7 @@ int pm_gather_radiance_noalloc(pm_photonmap* h, int64_t m, const pm_query* q,
   qs.q_anchor = reinterpret_cast<uint32_t*>(base + lay.q_anchor);
   qs.counts = reinterpret_cast<uint32_t*>(base + lay.counts);
   qs.partials = reinterpret_cast<uint32_t*>(base + lay.partials);
+  qs.tiles = nullptr;  // one instance: uniform tiles
   const uint32_t* perm = nullptr;
   if (int rc = prepare_queries_raw(qs, 1, desc, prefix, m, s, &perm, !all_members)) return rc;
   GatherLaunch L{};

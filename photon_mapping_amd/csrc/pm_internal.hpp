@@ -96,7 +96,18 @@ struct DevBuf {
 // nearest anchor of every query, and the work-queue / statistics
 // counters.
 struct Workspace {
-  DevBuf keys, vals, keys2, vals2, counts, partials, q_anchor, counters;
+  DevBuf keys, vals, keys2, vals2, counts, partials, q_anchor, counters, tiles;
+};
+
+// One tile of a segmented radix sort: elements [base, base + n) of segment
+// seg, whose digit-d count lives at cbase + d * stride of the count table.
+constexpr int kSortTileElems = 4096;
+struct SortTile {
+  int64_t base;
+  uint32_t n;
+  uint32_t cbase;
+  uint32_t stride;
+  uint32_t seg;
 };
 
 // ---- gather instance descriptor (device-resident, read with scalar loads) ----
@@ -255,6 +266,17 @@ int radix_sort_u64(Workspace* ws, uint64_t* keys, uint32_t* vals,
 // uint32 words of the sort histograms and of the scan partials for n keys.
 int64_t sort_count_words(int64_t n);
 int64_t sort_partial_words(int64_t n);
+// Segmented sort of n keys in n_seg segments: the tiles it may use, and the
+// words of its histograms and scan partials.
+int64_t sort_seg_tiles(int64_t n, int n_seg);
+int64_t sort_seg_count_words(int64_t n, int n_seg);
+int64_t sort_seg_partial_words(int64_t n, int n_seg);
+// Stable sort of every segment of a tile table (launch_sort_tiles) on bits
+// [lo_bit, bits): each segment's pairs stay in its own range.
+int radix_sort_seg_u32_raw(uint32_t* counts, uint32_t* partials, const SortTile* tiles,
+                           int64_t n_tiles, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                           uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which,
+                           int lo_bit);
 // The same sorts over caller-provided histogram and partial storage (at
 // least the words above): they allocate nothing.
 int radix_sort_u32_raw(uint32_t* counts, uint32_t* partials, uint32_t* keys, uint32_t* vals,
@@ -295,9 +317,11 @@ int prepare_queries(Workspace* ws, int n_inst, const GatherDesc* dev_descs,
                     const int64_t* dev_q_prefix, const int64_t* dev_g_prefix,
                     int64_t total_q, int64_t total_groups, hipStream_t s,
                     const uint32_t** perm_out, bool need_anchors = true);
-// Device storage of prepare_queries: keys / keys2 hold total_q keys (uint64
-// when n_inst > 1, else uint32), vals / vals2 / q_anchor total_q words,
-// counts and partials the sort_count_words / sort_partial_words of total_q.
+// Device storage of prepare_queries: keys / keys2 hold total_q uint32 keys,
+// vals / vals2 / q_anchor total_q words; counts and partials the
+// sort_count_words / sort_partial_words of total_q for one instance, the
+// sort_seg_count_words / sort_seg_partial_words for several, which also use
+// sort_seg_tiles tiles.
 struct QueryScratch {
   void* keys;
   void* keys2;
@@ -306,6 +330,7 @@ struct QueryScratch {
   uint32_t* q_anchor;
   uint32_t* counts;
   uint32_t* partials;
+  SortTile* tiles;  // sort_seg_tiles entries when n_inst > 1, else unused
 };
 // prepare_queries over caller-provided storage: allocates nothing.
 int prepare_queries_raw(const QueryScratch& qs, int n_inst, const GatherDesc* dev_descs,

@@ -6,6 +6,12 @@
 // digit counts in LDS, so the relative order of equal digits is preserved.
 // Everything is plain kernels (no memset nodes), so a caller can capture the
 // sort in a HIP graph.
+//
+// Segmented form: given a table of tiles that never straddle a segment
+// boundary, with the counts laid out segment-major ((segment, digit, tile of
+// the segment)), the same exclusive scan yields each tile's output position
+// inside its own segment's range, so one pass sorts every segment at once and
+// the segments keep their places.
 
 #include <hip/hip_runtime.h>
 
@@ -22,29 +28,50 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per block
 constexpr int kScanChunk = kSortThreads * kSortItems; // counts per scan block
+static_assert(kSortTile == kSortTileElems, "tile size shared with the tile table");
+
+// The block's tile: its elements [base, base + n) and where its digit counts
+// go (cbase + digit * stride), from the table when there is one, else the
+// uniform tiling (digit-major over all tiles).
+struct TileView {
+  int64_t base;
+  int n;
+  uint32_t cbase;
+  uint32_t stride;
+};
+__device__ __forceinline__ TileView tile_view(const SortTile* tiles, int64_t n,
+                                              int64_t n_tiles) {
+  if (tiles) {
+    const SortTile tl = tiles[blockIdx.x];
+    return {tl.base, static_cast<int>(tl.n), tl.cbase, tl.stride};
+  }
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+  return {base, static_cast<int>(std::min<int64_t>(kSortTile, n - base)), blockIdx.x,
+          static_cast<uint32_t>(n_tiles)};
+}
 
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void digit_hist_kernel(
     const K* __restrict__ keys, int64_t n, int shift, uint32_t* __restrict__ counts,
-    int64_t n_tiles) {
+    int64_t n_tiles, const SortTile* __restrict__ tiles) {
   __shared__ uint32_t h[256];
   const int t = threadIdx.x;
   h[t] = 0;
   __syncthreads();
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+  const TileView tv = tile_view(tiles, n, n_tiles);
   K k[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int64_t i = base + static_cast<int64_t>(r) * kSortThreads + t;
-    k[r] = i < n ? keys[i] : K(0);
+    const int li = r * kSortThreads + t;
+    k[r] = li < tv.n ? keys[tv.base + li] : K(0);
   }
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int64_t i = base + static_cast<int64_t>(r) * kSortThreads + t;
-    if (i < n) atomicAdd(&h[static_cast<uint32_t>(k[r] >> shift) & 255u], 1u);
+    const int li = r * kSortThreads + t;
+    if (li < tv.n) atomicAdd(&h[static_cast<uint32_t>(k[r] >> shift) & 255u], 1u);
   }
   __syncthreads();
-  counts[static_cast<int64_t>(t) * n_tiles + blockIdx.x] = h[t];
+  counts[static_cast<int64_t>(tv.cbase) + static_cast<int64_t>(t) * tv.stride] = h[t];
 }
 
 // Exclusive scan helpers over a uint32 array (length up to 2^31 chunks).
@@ -135,7 +162,7 @@ template <typename K>
 __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift,
-    const uint32_t* __restrict__ offsets, int64_t n_tiles) {
+    const uint32_t* __restrict__ offsets, int64_t n_tiles, const SortTile* __restrict__ tiles) {
   __shared__ uint32_t glob[256];   // first output position of each digit (this tile)
   __shared__ uint32_t loc[256];    // first staged position of each digit
   __shared__ uint32_t seen[256];   // elements of each digit staged so far
@@ -146,11 +173,12 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
-  const int tile_n = static_cast<int>(std::min<int64_t>(kSortTile, n - base));
+  const TileView tv = tile_view(tiles, n, n_tiles);
+  const int64_t base = tv.base;
+  const int tile_n = tv.n;
   // this tile's count of digit t: the difference of consecutive entries of
-  // the digit-major exclusive scan (the last entry closes at n)
-  const int64_t at = static_cast<int64_t>(t) * n_tiles + blockIdx.x;
+  // the exclusive scan of the count table (the last entry closes at n)
+  const int64_t at = static_cast<int64_t>(tv.cbase) + static_cast<int64_t>(t) * tv.stride;
   const uint32_t g = offsets[at];
   const uint32_t next = (at + 1 < 256 * n_tiles) ? offsets[at + 1] : static_cast<uint32_t>(n);
   uint32_t tot;
@@ -235,20 +263,21 @@ int exclusive_scan(uint32_t* partial, uint32_t* data, int64_t n, hipStream_t s) 
 template <typename K>
 int radix_sort_impl(uint32_t* counts, uint32_t* partials, K* keys, uint32_t* vals,
                     K* keys_alt, uint32_t* vals_alt, int64_t n, int bits, hipStream_t s,
-                    int* which, int lo_bit) {
+                    int* which, int lo_bit, const SortTile* tiles = nullptr,
+                    int64_t n_seg_tiles = 0) {
   *which = 0;
   if (n <= 1 || bits <= 0) return PM_OK;
-  const int64_t n_tiles = (n + kSortTile - 1) / kSortTile;
+  const int64_t n_tiles = tiles ? n_seg_tiles : (n + kSortTile - 1) / kSortTile;
   K* kb[2] = {keys, keys_alt};
   uint32_t* vb[2] = {vals, vals_alt};
   int cur = 0;
   for (int shift = lo_bit; shift < bits; shift += 8) {
     hipLaunchKernelGGL(digit_hist_kernel<K>, dim3(n_tiles), dim3(kSortThreads), 0, s,
-                       kb[cur], n, shift, counts, n_tiles);
+                       kb[cur], n, shift, counts, n_tiles, tiles);
     if (int rc = exclusive_scan(partials, counts, n_tiles * 256, s)) return rc;
     hipLaunchKernelGGL(digit_scatter_kernel<K>, dim3(n_tiles), dim3(kSortThreads), 0, s,
                        kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, shift, counts,
-                       n_tiles);
+                       n_tiles, tiles);
     PM_HIP(hipGetLastError());
     cur ^= 1;
   }
@@ -265,6 +294,25 @@ int64_t sort_count_words(int64_t n) {
 
 int64_t sort_partial_words(int64_t n) {
   return (sort_count_words(n) + kScanChunk - 1) / kScanChunk + 1;
+}
+
+int64_t sort_seg_tiles(int64_t n, int n_seg) {
+  // each segment's last tile may be partial: at most n_seg more than uniform
+  return (std::max<int64_t>(n, 1) + kSortTile - 1) / kSortTile + std::max(n_seg, 1);
+}
+
+int64_t sort_seg_count_words(int64_t n, int n_seg) { return sort_seg_tiles(n, n_seg) * 256; }
+
+int64_t sort_seg_partial_words(int64_t n, int n_seg) {
+  return (sort_seg_count_words(n, n_seg) + kScanChunk - 1) / kScanChunk + 1;
+}
+
+int radix_sort_seg_u32_raw(uint32_t* counts, uint32_t* partials, const SortTile* tiles,
+                           int64_t n_tiles, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                           uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which,
+                           int lo_bit) {
+  return radix_sort_impl<uint32_t>(counts, partials, keys, vals, keys_alt, vals_alt, n, bits, s,
+                                   which, lo_bit, tiles, n_tiles);
 }
 
 int sort_reserve(Workspace* ws, int64_t n) {

@@ -184,3 +184,34 @@ def test_batched_many_k_on_one_map(gpu, rng):
         torch.cuda.synchronize()
         assert torch.equal(rgb.view(torch.int32), e_rgb.view(torch.int32)), k
         assert torch.equal(key, e_key), k
+
+
+def test_batched_sort_tiles_at_segment_edges(gpu, rng):
+    """Instances whose query counts sit on and around the sort's 4,096-key
+    tiles (1, 4,095, 4,096, 4,097, 8,193 and 12,288 queries) and an empty one:
+    each instance is sorted inside its own range, so the batched results still
+    equal the single calls bit for bit."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    mats = torch.from_numpy(materials(3)).to(dev)
+    pos, wi, flux, _ = _instance_data(rng, 30000, 1, "surfaces")
+    pmap = gpu.PhotonMap()
+    pmap.set_photons_soa(pos, wi, flux)
+    pmap.build()
+    inst = []
+    for m in (4095, 1, 4096, 0, 4097, 8193, 12288):
+        pick = rng.integers(0, len(pos), m)
+        q = query_records(rng, (pos[pick] + np.float32(1e-3)).astype(np.float32))
+        inst.append((pmap, torch.from_numpy(q).to(dev), mats, 16 + (m % 7), 1000.0))
+    got = gpu.gather_batched(inst)
+    torch.cuda.synchronize()
+    for (pm_, q, mt, k, n_emit), (rgb, r2, key) in zip(inst, got):
+        if q.shape[0] == 0:
+            assert rgb.shape[0] == 0
+            continue
+        e_rgb, e_r2, e_key = pm_.gather_radiance(q, mt, k, n_emit)
+        torch.cuda.synchronize()
+        assert torch.equal(rgb.view(torch.int32), e_rgb.view(torch.int32))
+        assert torch.equal(r2.view(torch.int32), e_r2.view(torch.int32))
+        assert torch.equal(key, e_key)

@@ -586,8 +586,13 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
   PM_HIP(hipSetDevice(s->ctx->device));
   ImageOut img;
   if (int rc = img.begin(p, out_mean, out_var)) return rc;
+  // An empty caustics map estimates 0 everywhere (computeCausticsWithPhotonMap
+  // over no photons, include/integrator.h:155-175), so its queries are not
+  // made at all.
+  const bool caustics_lookups =
+      p->final_gathering_depth > 0 && reinterpret_cast<PhotonMapImpl*>(caustics_map)->n > 0;
   RenderDev R{s->dev(0), p->width, p->height, p->spp, p->max_depth, p->final_gathering_depth,
-              p->final_gathering_depth > 0 ? 1 : 0, p->seed, in_rays};
+              caustics_lookups ? 1 : 0, p->seed, in_rays};
   const int64_t n_pix = static_cast<int64_t>(p->width) * p->height;
   const int64_t bp = std::min(batch_pixels_of(p), n_pix);
   const int64_t bsamples = bp * p->spp;

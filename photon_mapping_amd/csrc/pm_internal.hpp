@@ -263,6 +263,10 @@ int sort_reserve(Workspace* ws, int64_t n);
 int radix_sort_u64(Workspace* ws, uint64_t* keys, uint32_t* vals,
                    uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
                    hipStream_t s, int* which, int lo_bit = 0);
+// In-place exclusive prefix sum of n uint32 words (plain kernels, no
+// allocation); partial holds scan_partial_words(n) words.
+int exclusive_scan_u32(uint32_t* partial, uint32_t* data, int64_t n, hipStream_t s);
+int64_t scan_partial_words(int64_t n);
 // uint32 words of the sort histograms and of the scan partials for n keys.
 int64_t sort_count_words(int64_t n);
 int64_t sort_partial_words(int64_t n);

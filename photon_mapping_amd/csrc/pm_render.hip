@@ -5,13 +5,16 @@
 // camera segment per sample; every round, shade_kernel traces the round's
 // segments and
 //   * adds light hits and next-event estimates to their sample,
-//   * appends gather queries (position, shading normal, wo, material) with
+//   * stages gather queries (position, shading normal, wo, material) with
 //     the segment's weight, one list per photon map,
-//   * appends the next segments: specular children (both Fresnel branches
+//   * stages the next segments: specular children (both Fresnel branches
 //     below depth 3, include/integrator.h:344-374) and final-gathering rays.
-// The round's gather lists are sorted by (sample, lineage), go through
-// pm_gather_radiance, the library's photon-density estimate, and
-// scatter_kernel adds weight * radiance to the samples.  Sample radiances accumulate as unsigned 32.32 fixed point, so
+// Staging uses slots fixed by the segment's index; a prefix sum then compacts
+// each list in segment order, so the lists' order, and with it every
+// estimate, does not depend on scheduling, and no list needs an atomic
+// counter.  The gather lists go through pm_gather_radiance, the library's
+// photon-density estimate, and scatter_kernel adds weight * radiance to the
+// samples.  Sample radiances accumulate as unsigned 32.32 fixed point, so
 // their sums do not depend on the order in which segments finish.
 //
 // Random numbers are counter based: each segment carries a 64-bit stream id
@@ -41,8 +44,7 @@ constexpr int kSplitDepth = 3;                 // include/integrator.h:325, 344
 // A segment's lineage (8 bits) names it among the segments of its sample:
 // 1 for the camera ray, 2 l + b for branch b of a split, | 0x80 for the
 // final-gathering ray of a diffuse hit.  Within one round no two segments of
-// a sample share a lineage, so (sample, lineage) orders a round's gather
-// queries the same way however the segments were scheduled.
+// a sample share a lineage.  A query's weight carries (sample, lineage).
 constexpr int kLineageBits = 8;
 
 struct Segment {
@@ -55,9 +57,51 @@ struct Segment {
   uint64_t stream;
 };
 
+// A round's segments in memory, field-major: word f of segment i at
+// w[f * stride + i] (o, d, w, sample, dim, meta, stream low, stream high), so
+// a wave's loads and stores of one field are contiguous instead of 56 B apart.
+constexpr int kSegWords = 14;
+struct Segs {
+  uint32_t* w;
+  uint32_t stride;
+};
+
+__device__ __forceinline__ Segment seg_load(Segs s, uint32_t i) {
+  const uint32_t* p = s.w + i;
+  const uint32_t st = s.stride;
+  Segment sg;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    sg.o[k] = __uint_as_float(p[k * st]);
+    sg.d[k] = __uint_as_float(p[(3 + k) * st]);
+    sg.w[k] = __uint_as_float(p[(6 + k) * st]);
+  }
+  sg.sample = p[9 * st];
+  sg.dim = p[10 * st];
+  sg.meta = p[11 * st];
+  sg.stream = static_cast<uint64_t>(p[12 * st]) | (static_cast<uint64_t>(p[13 * st]) << 32);
+  return sg;
+}
+
+__device__ __forceinline__ void seg_store(Segs s, uint32_t i, V3 o, V3 d, V3 w, uint32_t sample,
+                                          uint32_t dim, uint32_t meta, uint64_t stream) {
+  uint32_t* p = s.w + i;
+  const uint32_t st = s.stride;
+  p[0] = __float_as_uint(o.x); p[st] = __float_as_uint(o.y); p[2 * st] = __float_as_uint(o.z);
+  p[3 * st] = __float_as_uint(d.x); p[4 * st] = __float_as_uint(d.y);
+  p[5 * st] = __float_as_uint(d.z);
+  p[6 * st] = __float_as_uint(w.x); p[7 * st] = __float_as_uint(w.y);
+  p[8 * st] = __float_as_uint(w.z);
+  p[9 * st] = sample;
+  p[10 * st] = dim;
+  p[11 * st] = meta;
+  p[12 * st] = static_cast<uint32_t>(stream);
+  p[13 * st] = static_cast<uint32_t>(stream >> 32);
+}
+
 struct Weighted {
   float w[3];
-  uint32_t tag;     // sample << 8 | lineage: the query's sort key
+  uint32_t tag;     // sample << 8 | lineage
 };
 
 struct RenderDev {
@@ -108,57 +152,44 @@ __device__ __forceinline__ void add_term(unsigned long long* acc, uint32_t* bad,
   if (b > 0.0f) atomicAdd(&acc[3 * s + 2], __double2ull_rn(static_cast<double>(b) * kFixScale));
 }
 
-// Slot for each lane active here in an append-only list: one atomic per wave
-// (the lowest active lane adds the active count), ranks by lane order.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter) {
-  const uint64_t active = __ballot(1);
-  const int leader = __builtin_ctzll(active);
-  const int lane = static_cast<int>(__lane_id());
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(active)));
-  base = __shfl(base, leader);
-  return base + static_cast<uint32_t>(__popcll(active & ((1ull << lane) - 1ull)));
-}
-
-// Adds v (< 2^kBits) of every lane active here to a 64-bit counter, one
-// atomic per wave: the sum is assembled bit plane by bit plane from ballots.
+// Adds v (< 2^kBits) of every lane active here to one of kRayShards 64-bit
+// counters (chosen by workgroup), one atomic per wave: the sum is assembled
+// bit plane by bit plane from ballots.  Spreading the atomics keeps them
+// from queueing on one address; the host sums the shards.
+constexpr int kRayShards = 256;
 template <int kBits>
-__device__ __forceinline__ void wave_count_add(unsigned long long* counter, uint32_t v) {
+__device__ __forceinline__ void wave_count_add(unsigned long long* shards, uint32_t v) {
   const uint64_t active = __ballot(1);
   unsigned long long sum = 0;
 #pragma unroll
   for (int b = 0; b < kBits; ++b)
     sum += static_cast<unsigned long long>(__popcll(__ballot((v >> b) & 1u))) << b;
-  if (static_cast<int>(__lane_id()) == __builtin_ctzll(active)) atomicAdd(counter, sum);
+  if (static_cast<int>(__lane_id()) == __builtin_ctzll(active))
+    atomicAdd(&shards[blockIdx.x % kRayShards], sum);
 }
 
-__device__ __forceinline__ void push_segment(Segment* out, uint32_t* n_out, uint32_t cap, V3 o,
-                                             V3 d, V3 w, uint32_t sample, uint32_t dim,
+// A segment's outputs go to slots fixed by its index i: next-round segments
+// at 2 i and 2 i + 1 of the round's staging segments, its query of each map at
+// i of that map's staging list.  Its flags (kSegCount bits, kFlagGlobal,
+// kFlagCaustics) say which slots it filled; a prefix sum over the flags then
+// compacts every list in segment order.  No atomics, and the order does not
+// depend on scheduling.
+constexpr uint32_t kSegCount = 3u;       // segments written (0, 1 or 2)
+constexpr uint32_t kFlagGlobal = 4u;     // a global-map query
+constexpr uint32_t kFlagCaustics = 8u;   // a caustics-map query
+
+__device__ __forceinline__ void push_segment(Segs out, uint32_t i, uint32_t& flags, V3 o, V3 d,
+                                             V3 w, uint32_t sample, uint32_t dim,
                                              uint32_t depth, uint32_t kind, uint32_t lineage,
-                                             uint64_t stream, uint32_t* overflow) {
-  const uint32_t at = wave_append(n_out);
-  if (at >= cap) {
-    atomicOr(overflow, 1u);
-    return;
-  }
-  Segment& sg = out[at];
-  sg.o[0] = o.x; sg.o[1] = o.y; sg.o[2] = o.z;
-  sg.d[0] = d.x; sg.d[1] = d.y; sg.d[2] = d.z;
-  sg.w[0] = w.x; sg.w[1] = w.y; sg.w[2] = w.z;
-  sg.sample = sample;
-  sg.dim = dim;
-  sg.meta = depth | (kind << 16) | (lineage << 17);
-  sg.stream = stream;
+                                             uint64_t stream) {
+  const uint32_t at = 2 * i + (flags & kSegCount);
+  ++flags;
+  seg_store(out, at, o, d, w, sample, dim, depth | (kind << 16) | (lineage << 17), stream);
 }
 
-__device__ __forceinline__ void push_query(pm_query* q, Weighted* wq, uint32_t* n_q, uint32_t cap,
-                                           V3 x, V3 ns, V3 wo, int mat, V3 w, uint32_t sample,
-                                           uint32_t lineage, uint32_t* overflow) {
-  const uint32_t at = wave_append(n_q);
-  if (at >= cap) {
-    atomicOr(overflow, 1u);
-    return;
-  }
+__device__ __forceinline__ void push_query(pm_query* q, Weighted* wq, uint32_t at, V3 x, V3 ns,
+                                           V3 wo, int mat, V3 w, uint32_t sample,
+                                           uint32_t lineage) {
   pm_query r;
   r.pos[0] = x.x; r.pos[1] = x.y; r.pos[2] = x.z;
   r.ns[0] = ns.x; r.ns[1] = ns.y; r.ns[2] = ns.z;
@@ -186,52 +217,36 @@ __device__ __forceinline__ uint64_t child_stream(uint64_t stream, uint32_t branc
 
 __global__ __launch_bounds__(kThreads) void camera_kernel(RenderDev R, pmx_camera cam,
                                                           int64_t sample0, uint32_t n,
-                                                          Segment* __restrict__ out) {
+                                                          Segs out) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   const uint64_t sid = static_cast<uint64_t>(sample0) + i;
-  const uint64_t stream = sid;
   V3 sensor, d;
   first_ray(R, cam, sid, sensor, d);
-  Segment& sg = out[i];
-  sg.o[0] = sensor.x; sg.o[1] = sensor.y; sg.o[2] = sensor.z;
-  sg.d[0] = d.x; sg.d[1] = d.y; sg.d[2] = d.z;
-  sg.w[0] = 1.0f; sg.w[1] = 1.0f; sg.w[2] = 1.0f;  // camera pdf 1
-  sg.sample = i;
-  sg.dim = 2;
-  sg.meta = 1u << 17;  // depth 0, camera, lineage 1
-  sg.stream = stream;
+  // camera pdf 1; depth 0, camera, lineage 1; the stream is the sample id
+  seg_store(out, i, sensor, d, v3(1.0f, 1.0f, 1.0f), i, 2, 1u << 17, sid);
 }
 
 struct ShadeOut {
-  Segment* seg;
-  uint32_t* n_seg;
-  uint32_t seg_cap;
-  pm_query* qg;
+  Segs seg;        // staging: 2 slots per input segment
+  pm_query* qg;    // staging: 1 slot per input segment, each map
   Weighted* wg;
-  uint32_t* n_qg;
   pm_query* qc;
   Weighted* wc;
-  uint32_t* n_qc;
-  uint32_t q_cap;
+  uint32_t* flags; // per input segment
   unsigned long long* acc;
   uint32_t* bad;
-  uint32_t* overflow;
-  unsigned long long* rays;
+  unsigned long long* rays;  // kRayShards counters
 };
 
 // One round of PhotonMapping::integrateRecursive and
 // computeIndirectIlluminationRecursive (include/integrator.h:215-252,
 // 290-377) over the round's segments.
-__global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
-                                                         const Segment* __restrict__ in,
-                                                         uint32_t n_in, ShadeOut O) {
-  extern __shared__ int stk_all[];  // SceneImpl::stack_bytes
-  int* stk = stk_all + threadIdx.x;
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n_in) return;
+__device__ __forceinline__ uint32_t shade_one(const RenderDev& R, const Segs& in, uint32_t i,
+                                              const ShadeOut& O, int* stk, uint32_t& n_rays) {
   const SceneDev& S = R.S;
-  const Segment sg = in[i];
+  uint32_t flags = 0;
+  const Segment sg = seg_load(in, i);
   const V3 o = ldv(sg.o), d = ldv(sg.d), w = ldv(sg.w);
   const uint32_t depth = sg.meta & 0xffffu, kind = (sg.meta >> 16) & 1u;
   const uint32_t lineage = sg.meta >> 17;
@@ -239,8 +254,8 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
   const uint64_t stream = sg.stream;
   const uint64_t seed = R.seed;
   // integrateRecursive returns 0 at maxDepth (include/integrator.h:291)
-  if (kind == kKindCamera && static_cast<int>(depth) >= R.max_depth) return;
-  uint32_t n_rays = 1;
+  if (kind == kKindCamera && static_cast<int>(depth) >= R.max_depth) return flags;
+  n_rays = 1;
   const Hit h = closest_hit(S, o, d, INFINITY, stk);
   if (h.tri >= 0) {
     const pmx_triangle& tri = S.tris[h.tri];
@@ -259,8 +274,8 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
         add_term(O.acc, O.bad, sg.sample, w.x * le.x, w.y * le.y, w.z * le.z);
       } else if (type == PM_MATERIAL_LAMBERT) {
         if (static_cast<int>(depth) >= R.fg_depth) {
-          push_query(O.qg, O.wg, O.n_qg, O.q_cap, x, ns_raw, wo, tri.material, w, sg.sample,
-                     lineage, O.overflow);
+          push_query(O.qg, O.wg, i, x, ns_raw, wo, tri.material, w, sg.sample, lineage);
+          flags |= kFlagGlobal;
         } else {
           // direct illumination by light sampling (include/integrator.h:178-213)
           if (S.n_lights > 0) {
@@ -291,15 +306,17 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
           }
           // caustics by the caustics map (include/integrator.h:155-175)
           if (R.caustics)
-            push_query(O.qc, O.wc, O.n_qc, O.q_cap, x, ns_raw, wo, tri.material, w, sg.sample,
-                       lineage, O.overflow);
+            {
+            push_query(O.qc, O.wc, i, x, ns_raw, wo, tri.material, w, sg.sample, lineage);
+            flags |= kFlagCaustics;
+          }
           // final gathering (include/integrator.h:215-252), recursion depth 0
           if (R.max_depth > 0) {
             float scale;
             const V3 dir = sample_bxdf(type, eta, wo, ns, false, seed, stream, dim, scale);
             const float c = fabsf(dot(ns, dir)) * scale;
-            push_segment(O.seg, O.n_seg, O.seg_cap, x, dir, mul(w, rho) * c, sg.sample, dim,
-                         0, kKindGather, lineage | 0x80u, stream, O.overflow);
+            push_segment(O.seg, i, flags, x, dir, mul(w, rho) * c, sg.sample, dim, 0,
+                         kKindGather, lineage | 0x80u, stream);
           }
         }
       } else if (static_cast<int>(depth) + 1 < R.max_depth) {
@@ -309,8 +326,8 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
           float scale;
           const V3 dir = sample_bxdf(type, eta, wo, ns, false, seed, stream, dim, scale);
           const float c = cos_term(wo, dir, ns, ng, false) * scale;
-          push_segment(O.seg, O.n_seg, O.seg_cap, x, dir, mul(w, rho) * c, sg.sample, dim,
-                       depth + 1, kKindCamera, lineage, stream, O.overflow);
+          push_segment(O.seg, i, flags, x, dir, mul(w, rho) * c, sg.sample, dim, depth + 1,
+                       kKindCamera, lineage, stream);
         } else {
           // sampleAllDirection (include/material.h:123-129, 203-243) in the
           // shading frame; each branch's weight is f |cos| with f = (its
@@ -351,9 +368,8 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
             const V3 dl = dirs[b];
             const V3 dir = ft * dl.x + ns * dl.y + fb * dl.z;
             const float c = share[b] / fabsf(dl.y) * fabsf(dot(dir, ns));
-            push_segment(O.seg, O.n_seg, O.seg_cap, x, dir, mul(w, rho) * c, sg.sample, 0,
-                         depth + 1, kKindCamera, 2 * lineage + b, child_stream(stream, b),
-                         O.overflow);
+            push_segment(O.seg, i, flags, x, dir, mul(w, rho) * c, sg.sample, 0, depth + 1,
+                         kKindCamera, 2 * lineage + b, child_stream(stream, b));
           }
         }
       }
@@ -361,18 +377,82 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R,
       // a final-gathering ray: the photon map at a diffuse hit, or one more
       // sampled bounce at a specular hit
       if (type == PM_MATERIAL_LAMBERT) {
-        push_query(O.qg, O.wg, O.n_qg, O.q_cap, x, ns_raw, wo, tri.material, w, sg.sample,
-                   lineage, O.overflow);
+        push_query(O.qg, O.wg, i, x, ns_raw, wo, tri.material, w, sg.sample, lineage);
+        flags |= kFlagGlobal;
       } else if (static_cast<int>(depth) + 1 < R.max_depth) {
         float scale;
         const V3 dir = sample_bxdf(type, eta, wo, ns, false, seed, stream, dim, scale);
         const float c = fabsf(dot(ns, dir)) * scale;
-        push_segment(O.seg, O.n_seg, O.seg_cap, x, dir, mul(w, rho) * c, sg.sample, dim,
-                     depth + 1, kKindGather, lineage, stream, O.overflow);
+        push_segment(O.seg, i, flags, x, dir, mul(w, rho) * c, sg.sample, dim, depth + 1,
+                     kKindGather, lineage, stream);
       }
     }
   }
+  return flags;
+}
+
+__global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R, Segs in, uint32_t n_in,
+                                                         ShadeOut O) {
+  extern __shared__ int stk_all[];  // SceneImpl::stack_bytes
+  int* stk = stk_all + threadIdx.x;
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n_in) return;
+  uint32_t n_rays = 0;
+  O.flags[i] = shade_one(R, in, i, O, stk, n_rays);
   wave_count_add<2>(O.rays, n_rays);
+}
+
+// The three per-segment counts of a shade round (segments, global queries,
+// caustics queries), to be prefix-summed.
+__global__ __launch_bounds__(kThreads) void flag_counts_kernel(const uint32_t* __restrict__ flags,
+                                                               uint32_t n,
+                                                               uint32_t* __restrict__ c_seg,
+                                                               uint32_t* __restrict__ c_qg,
+                                                               uint32_t* __restrict__ c_qc) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t f = flags[i];
+  c_seg[i] = f & kSegCount;
+  c_qg[i] = (f & kFlagGlobal) ? 1u : 0u;
+  c_qc[i] = (f & kFlagCaustics) ? 1u : 0u;
+}
+
+// The lists' lengths: the last exclusive prefix plus the last count.
+__global__ void list_totals_kernel(const uint32_t* __restrict__ flags, uint32_t n,
+                                   const uint32_t* __restrict__ p_seg,
+                                   const uint32_t* __restrict__ p_qg,
+                                   const uint32_t* __restrict__ p_qc, uint32_t* __restrict__ out) {
+  if (threadIdx.x != 0 || n == 0) return;
+  const uint32_t f = flags[n - 1];
+  out[0] = p_seg[n - 1] + (f & kSegCount);
+  out[1] = p_qg[n - 1] + ((f & kFlagGlobal) ? 1u : 0u);
+  out[2] = p_qc[n - 1] + ((f & kFlagCaustics) ? 1u : 0u);
+}
+
+// Moves the staged outputs of a shade round to their compacted positions.
+__global__ __launch_bounds__(kThreads) void compact_kernel(
+    const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ p_seg,
+    const uint32_t* __restrict__ p_qg, const uint32_t* __restrict__ p_qc, Segs stage, Segs next,
+    const pm_query* __restrict__ qg_s, const Weighted* __restrict__ wg_s,
+    pm_query* __restrict__ qg, Weighted* __restrict__ wg, const pm_query* __restrict__ qc_s,
+    const Weighted* __restrict__ wc_s, pm_query* __restrict__ qc, Weighted* __restrict__ wc) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t f = flags[i];
+  const uint32_t ns = f & kSegCount;
+  for (uint32_t j = 0; j < ns; ++j) {
+    const Segment sg = seg_load(stage, 2 * i + j);
+    seg_store(next, p_seg[i] + j, ldv(sg.o), ldv(sg.d), ldv(sg.w), sg.sample, sg.dim, sg.meta,
+              sg.stream);
+  }
+  if (f & kFlagGlobal) {
+    qg[p_qg[i]] = qg_s[i];
+    wg[p_qg[i]] = wg_s[i];
+  }
+  if (f & kFlagCaustics) {
+    qc[p_qc[i]] = qc_s[i];
+    wc[p_qc[i]] = wc_s[i];
+  }
 }
 
 // weight * estimated radiance into the samples
@@ -386,29 +466,6 @@ __global__ __launch_bounds__(kThreads) void scatter_kernel(const Weighted* __res
   const Weighted ww = wq[i];
   add_term(acc, bad, ww.tag >> kLineageBits, ww.w[0] * rgb[3 * i + 0], ww.w[1] * rgb[3 * i + 1],
            ww.w[2] * rgb[3 * i + 2]);
-}
-
-__global__ __launch_bounds__(kThreads) void tag_kernel(const Weighted* __restrict__ wq, uint32_t n,
-                                                       uint32_t* __restrict__ keys,
-                                                       uint32_t* __restrict__ vals) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  keys[i] = wq[i].tag;
-  vals[i] = i;
-}
-
-// The round's queries in (sample, lineage) order.
-__global__ __launch_bounds__(kThreads) void permute_kernel(const uint32_t* __restrict__ perm,
-                                                           uint32_t n,
-                                                           const pm_query* __restrict__ q,
-                                                           const Weighted* __restrict__ wq,
-                                                           pm_query* __restrict__ q_out,
-                                                           Weighted* __restrict__ w_out) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t j = perm[i];
-  q_out[i] = q[j];
-  w_out[i] = wq[j];
 }
 
 // PathTracing::integrate (include/integrator.h:60-101), one sample per lane.
@@ -598,43 +655,44 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
   const int64_t bsamples = bp * p->spp;
   if (bsamples > (int64_t{1} << (32 - kLineageBits)))
     return fail(PM_ERR_INVALID_ARG, "batch too large (at most 2^24 samples)");
-  int sample_bits = 1;
-  while ((int64_t{1} << sample_bits) < bsamples) ++sample_bits;
-  const int key_bits = sample_bits + kLineageBits;
   // Buffers grow to each round's needs: a segment spawns at most two
   // segments and one query per photon map.
-  DevBuf seg[2], qg, wg, qc, wc, rgb, r2, acc, bad, counters;
-  DevBuf keys_a, keys_b, vals_a, vals_b, q_sorted, w_sorted;
-  Workspace ws;
+  DevBuf seg[2], stage, qg_s, wg_s, qc_s, wc_s, qg, wg, qc, wc, rgb, r2, acc, bad, counters;
+  DevBuf flags, c_seg, c_qg, c_qc, partial;
   int rc = acc.reserve(static_cast<size_t>(bsamples) * 3 * sizeof(unsigned long long));
   if (!rc) rc = bad.reserve(static_cast<size_t>(bsamples) * sizeof(uint32_t));
-  if (!rc) rc = counters.reserve(16 * sizeof(unsigned long long));
-  if (!rc) rc = seg[0].reserve(static_cast<size_t>(bsamples) * sizeof(Segment));
+  if (!rc) rc = counters.reserve((8 + kRayShards + 8) * sizeof(unsigned long long));
+  if (!rc) rc = seg[0].reserve(static_cast<size_t>(bsamples) * kSegWords * sizeof(uint32_t));
   if (rc) return rc;
-  const auto reserve_round = [&](size_t n_in, DevBuf& out) -> int {
-    const size_t q = n_in;
-    int e = out.reserve(2 * n_in * sizeof(Segment));
-    if (!e) e = qg.reserve(q * sizeof(pm_query));
-    if (!e) e = wg.reserve(q * sizeof(Weighted));
-    if (!e && R.caustics) e = qc.reserve(q * sizeof(pm_query));
-    if (!e && R.caustics) e = wc.reserve(q * sizeof(Weighted));
-    if (!e) e = rgb.reserve(q * 3 * sizeof(float));
-    if (!e) e = r2.reserve(q * sizeof(float));
-    if (!e) e = keys_a.reserve(q * sizeof(uint32_t));
-    if (!e) e = keys_b.reserve(q * sizeof(uint32_t));
-    if (!e) e = vals_a.reserve(q * sizeof(uint32_t));
-    if (!e) e = vals_b.reserve(q * sizeof(uint32_t));
-    if (!e) e = q_sorted.reserve(q * sizeof(pm_query));
-    if (!e) e = w_sorted.reserve(q * sizeof(Weighted));
-    if (!e) e = sort_reserve(&ws, static_cast<int64_t>(q));
+  // A round of n_in segments stages at most 2 n_in segments and n_in queries
+  // per map (fixed slots), then compacts them.
+  const auto reserve_round = [&](size_t n_in, DevBuf& next) -> int {
+    int e = stage.reserve(2 * n_in * kSegWords * sizeof(uint32_t));
+    if (!e) e = next.reserve(2 * n_in * kSegWords * sizeof(uint32_t));
+    if (!e) e = qg_s.reserve(n_in * sizeof(pm_query));
+    if (!e) e = wg_s.reserve(n_in * sizeof(Weighted));
+    if (!e) e = qg.reserve(n_in * sizeof(pm_query));
+    if (!e) e = wg.reserve(n_in * sizeof(Weighted));
+    if (!e && R.caustics) e = qc_s.reserve(n_in * sizeof(pm_query));
+    if (!e && R.caustics) e = wc_s.reserve(n_in * sizeof(Weighted));
+    if (!e && R.caustics) e = qc.reserve(n_in * sizeof(pm_query));
+    if (!e && R.caustics) e = wc.reserve(n_in * sizeof(Weighted));
+    if (!e) e = rgb.reserve(n_in * 3 * sizeof(float));
+    if (!e) e = r2.reserve(n_in * sizeof(float));
+    if (!e) e = flags.reserve(n_in * sizeof(uint32_t));
+    if (!e) e = c_seg.reserve(n_in * sizeof(uint32_t));
+    if (!e) e = c_qg.reserve(n_in * sizeof(uint32_t));
+    if (!e) e = c_qc.reserve(n_in * sizeof(uint32_t));
+    if (!e) e = partial.reserve(static_cast<size_t>(scan_partial_words(n_in)) * sizeof(uint32_t));
     return e;
   };
-  // counter words (uint32): 0 segments out, 1 global queries, 2 caustics
-  // queries, 3 overflow flag; uint64 at words 8 (rays) and 10 (dropped samples)
+  // counter words (uint32): 0-2 the lists' lengths after a round; uint64 at
+  // words 8 (dropped samples) and from 16 the kRayShards ray counters
   auto* c32 = counters.as<uint32_t>();
-  auto* rays = reinterpret_cast<unsigned long long*>(c32 + 8);
-  auto* dropped = reinterpret_cast<unsigned long long*>(c32 + 10);
-  if (int e = zero_words(counters.ptr, 16 * sizeof(unsigned long long))) return e;
+  auto* dropped = reinterpret_cast<unsigned long long*>(c32 + 8);
+  auto* rays = reinterpret_cast<unsigned long long*>(c32 + 16);
+  if (int e = zero_words(counters.ptr, (8 + kRayShards + 8) * sizeof(unsigned long long)))
+    return e;
   pmx_render_stats st{};
   for (int64_t pix0 = 0; pix0 < n_pix; pix0 += bp) {
     const int64_t np = std::min(bp, n_pix - pix0);
@@ -643,9 +701,12 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
       return e;
     if (int e = zero_words(bad.ptr, static_cast<size_t>(ns) * sizeof(uint32_t))) return e;
     int cur_i = 0;
-    if (int e = seg[cur_i].reserve(static_cast<size_t>(ns) * sizeof(Segment))) return e;
+    // each segment buffer's field stride: the records it was reserved for
+    uint32_t stride[2] = {ns, 0};
+    if (int e = seg[cur_i].reserve(static_cast<size_t>(ns) * kSegWords * sizeof(uint32_t)))
+      return e;
     hipLaunchKernelGGL(camera_kernel, dim3(grid_of(ns)), dim3(kThreads), 0, nullptr, R, *cam,
-                       pix0 * p->spp, ns, seg[cur_i].as<Segment>());
+                       pix0 * p->spp, ns, Segs{seg[cur_i].as<uint32_t>(), stride[cur_i]});
     PM_HIP(hipGetLastError());
     uint32_t n_cur = ns;
     ++st.batches;
@@ -653,21 +714,37 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
       if (2 * static_cast<uint64_t>(n_cur) > 0xffffffffull)
         return fail(PM_ERR_UNSUPPORTED, "render round too large");
       if (int e = reserve_round(n_cur, seg[cur_i ^ 1])) return e;
-      Segment* cur = seg[cur_i].as<Segment>();
-      Segment* nxt = seg[cur_i ^ 1].as<Segment>();
-      if (int e = zero_words(c32, 4 * sizeof(uint32_t))) return e;
-      ShadeOut O{nxt, c32 + 0, 2 * n_cur, qg.as<pm_query>(), wg.as<Weighted>(), c32 + 1,
-                 qc.as<pm_query>(), wc.as<Weighted>(), c32 + 2, n_cur,
-                 acc.as<unsigned long long>(), bad.as<uint32_t>(), c32 + 3, rays};
-      hipLaunchKernelGGL(shade_kernel, dim3(grid_of(n_cur)), dim3(kThreads), s->stack_bytes(0),
-                         nullptr, R, cur,
+      stride[cur_i ^ 1] = 2 * n_cur;
+      const Segs cur{seg[cur_i].as<uint32_t>(), stride[cur_i]};
+      const Segs nxt{seg[cur_i ^ 1].as<uint32_t>(), stride[cur_i ^ 1]};
+      const Segs stg{stage.as<uint32_t>(), 2 * n_cur};
+      ShadeOut O{stg, qg_s.as<pm_query>(), wg_s.as<Weighted>(), qc_s.as<pm_query>(),
+                 wc_s.as<Weighted>(), flags.as<uint32_t>(), acc.as<unsigned long long>(),
+                 bad.as<uint32_t>(), rays};
+      const dim3 grid(grid_of(n_cur));
+      hipLaunchKernelGGL(shade_kernel, grid, dim3(kThreads), s->stack_bytes(0), nullptr, R, cur,
                          n_cur, O);
+      hipLaunchKernelGGL(flag_counts_kernel, grid, dim3(kThreads), 0, nullptr,
+                         flags.as<uint32_t>(), n_cur, c_seg.as<uint32_t>(), c_qg.as<uint32_t>(),
+                         c_qc.as<uint32_t>());
       PM_HIP(hipGetLastError());
-      uint32_t cnt[4];
+      for (DevBuf* c : {&c_seg, &c_qg, &c_qc})
+        if (int e = exclusive_scan_u32(partial.as<uint32_t>(), c->as<uint32_t>(), n_cur, nullptr))
+          return e;
+      hipLaunchKernelGGL(list_totals_kernel, dim3(1), dim3(64), 0, nullptr, flags.as<uint32_t>(),
+                         n_cur, c_seg.as<uint32_t>(), c_qg.as<uint32_t>(), c_qc.as<uint32_t>(),
+                         c32);
+      hipLaunchKernelGGL(compact_kernel, grid, dim3(kThreads), 0, nullptr, flags.as<uint32_t>(),
+                         n_cur, c_seg.as<uint32_t>(), c_qg.as<uint32_t>(), c_qc.as<uint32_t>(),
+                         stg, nxt, qg_s.as<pm_query>(), wg_s.as<Weighted>(), qg.as<pm_query>(),
+                         wg.as<Weighted>(), qc_s.as<pm_query>(), wc_s.as<Weighted>(),
+                         qc.as<pm_query>(), wc.as<Weighted>());
+      PM_HIP(hipGetLastError());
+      uint32_t cnt[3];
       PM_HIP(hipMemcpy(cnt, c32, sizeof(cnt), hipMemcpyDeviceToHost));
-      // the capacities bound what a round can produce, so this is a guard
-      if (cnt[3]) return fail(PM_ERR_UNSUPPORTED, "render segment or query list overflow");
       ++st.iterations;
+      // The lists are in segment order, which follows from the camera order
+      // through every compaction, so the estimates do not depend on scheduling.
       for (int map_i = 0; map_i < 2; ++map_i) {
         const uint32_t nq = cnt[1 + map_i];
         if (nq == 0) continue;
@@ -675,27 +752,13 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
         const double n_emit = map_i == 0 ? p->n_emit_global : p->n_emit_caustics;
         const pm_query* q = map_i == 0 ? qg.as<pm_query>() : qc.as<pm_query>();
         const Weighted* wq = map_i == 0 ? wg.as<Weighted>() : wc.as<Weighted>();
-        // (sample, lineage) order, so the estimates do not depend on scheduling
-        hipLaunchKernelGGL(tag_kernel, dim3(grid_of(nq)), dim3(kThreads), 0, nullptr, wq, nq,
-                           keys_a.as<uint32_t>(), vals_a.as<uint32_t>());
-        PM_HIP(hipGetLastError());
-        int which = 0;
-        if (int e = radix_sort_u32(&ws, keys_a.as<uint32_t>(), vals_a.as<uint32_t>(),
-                                   keys_b.as<uint32_t>(), vals_b.as<uint32_t>(), nq, key_bits,
-                                   nullptr, &which))
+        if (int e = pm_gather_radiance(map, nq, q, s->mats.as<pm_material>(), s->n_mat, p->k,
+                                       n_emit, rgb.as<float>(), r2.as<float>(), nullptr,
+                                       PM_MEM_DEVICE, nullptr))
           return e;
-        hipLaunchKernelGGL(permute_kernel, dim3(grid_of(nq)), dim3(kThreads), 0, nullptr,
-                           which ? vals_b.as<uint32_t>() : vals_a.as<uint32_t>(), nq, q, wq,
-                           q_sorted.as<pm_query>(), w_sorted.as<Weighted>());
-        PM_HIP(hipGetLastError());
-        if (int e = pm_gather_radiance(map, nq, q_sorted.as<pm_query>(),
-                                       s->mats.as<pm_material>(), s->n_mat, p->k, n_emit,
-                                       rgb.as<float>(), r2.as<float>(), nullptr, PM_MEM_DEVICE,
-                                       nullptr))
-          return e;
-        hipLaunchKernelGGL(scatter_kernel, dim3(grid_of(nq)), dim3(kThreads), 0, nullptr,
-                           w_sorted.as<Weighted>(), rgb.as<float>(), nq,
-                           acc.as<unsigned long long>(), bad.as<uint32_t>());
+        hipLaunchKernelGGL(scatter_kernel, dim3(grid_of(nq)), dim3(kThreads), 0, nullptr, wq,
+                           rgb.as<float>(), nq, acc.as<unsigned long long>(),
+                           bad.as<uint32_t>());
         PM_HIP(hipGetLastError());
         (map_i == 0 ? st.gathers_global : st.gathers_caustics) += nq;
       }
@@ -707,12 +770,16 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
                        static_cast<uint32_t>(np), pix0, img.d_mean, img.d_var, dropped);
     PM_HIP(hipGetLastError());
   }
-  unsigned long long tail[2];
-  PM_HIP(hipMemcpy(tail, rays, sizeof(tail), hipMemcpyDeviceToHost));
+  unsigned long long tail[1 + kRayShards];
+  PM_HIP(hipMemcpy(tail, dropped, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  PM_HIP(hipMemcpy(tail + 1, rays, kRayShards * sizeof(unsigned long long),
+                   hipMemcpyDeviceToHost));
   PM_HIP(hipDeviceSynchronize());
+  unsigned long long n_rays_all = 0;
+  for (int k = 0; k < kRayShards; ++k) n_rays_all += tail[1 + k];
   st.samples = n_pix * p->spp;
-  st.rays = static_cast<int64_t>(tail[0]);
-  st.dropped_samples = static_cast<int64_t>(tail[1]);
+  st.rays = static_cast<int64_t>(n_rays_all);
+  st.dropped_samples = static_cast<int64_t>(tail[0]);
   st.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
               .count();
   if (int rc = img.end(p, out_mean, out_var)) return rc;
@@ -736,10 +803,12 @@ int render_pt(const pmx_scene* scene, const pmx_camera* cam, const pmx_render_pa
   DevBuf acc, bad, counters;
   int rc = acc.reserve(static_cast<size_t>(bsamples) * 3 * sizeof(unsigned long long));
   if (!rc) rc = bad.reserve(static_cast<size_t>(bsamples) * sizeof(uint32_t));
-  if (!rc) rc = counters.reserve(4 * sizeof(unsigned long long));
+  if (!rc) rc = counters.reserve((1 + kRayShards) * sizeof(unsigned long long));
   if (rc) return rc;
-  auto* rays = counters.as<unsigned long long>();
-  if (int e = zero_words(counters.ptr, 4 * sizeof(unsigned long long))) return e;
+  // the dropped-sample count, then the kRayShards ray counters
+  auto* dropped = counters.as<unsigned long long>();
+  auto* rays = dropped + 1;
+  if (int e = zero_words(counters.ptr, (1 + kRayShards) * sizeof(unsigned long long))) return e;
   pmx_render_stats st{};
   for (int64_t pix0 = 0; pix0 < n_pix; pix0 += bp) {
     const int64_t np = std::min(bp, n_pix - pix0);
@@ -751,17 +820,19 @@ int render_pt(const pmx_scene* scene, const pmx_camera* cam, const pmx_render_pa
     PM_HIP(hipGetLastError());
     hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(np)), dim3(kThreads), 0, nullptr,
                        acc.as<unsigned long long>(), bad.as<uint32_t>(), p->spp,
-                       static_cast<uint32_t>(np), pix0, img.d_mean, img.d_var, rays + 1);
+                       static_cast<uint32_t>(np), pix0, img.d_mean, img.d_var, dropped);
     PM_HIP(hipGetLastError());
     ++st.batches;
     ++st.iterations;
   }
-  unsigned long long tail[2];
-  PM_HIP(hipMemcpy(tail, rays, sizeof(tail), hipMemcpyDeviceToHost));
+  unsigned long long tail[1 + kRayShards];
+  PM_HIP(hipMemcpy(tail, dropped, sizeof(tail), hipMemcpyDeviceToHost));
   PM_HIP(hipDeviceSynchronize());
+  unsigned long long n_rays_all = 0;
+  for (int k = 0; k < kRayShards; ++k) n_rays_all += tail[1 + k];
   st.samples = n_pix * p->spp;
-  st.rays = static_cast<int64_t>(tail[0]);
-  st.dropped_samples = static_cast<int64_t>(tail[1]);
+  st.rays = static_cast<int64_t>(n_rays_all);
+  st.dropped_samples = static_cast<int64_t>(tail[0]);
   st.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
               .count();
   if (int rc = img.end(p, out_mean, out_var)) return rc;

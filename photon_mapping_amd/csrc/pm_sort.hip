@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   }
 }
 
-int exclusive_scan(uint32_t* partial, uint32_t* data, int64_t n, hipStream_t s) {
+int exclusive_scan_impl(uint32_t* partial, uint32_t* data, int64_t n, hipStream_t s) {
   const int64_t n_chunks = (n + kScanChunk - 1) / kScanChunk;
   hipLaunchKernelGGL(scan_reduce_kernel, dim3(n_chunks), dim3(kSortThreads), 0, s, data,
                      n, partial);
@@ -274,7 +274,7 @@ int radix_sort_impl(uint32_t* counts, uint32_t* partials, K* keys, uint32_t* val
   for (int shift = lo_bit; shift < bits; shift += 8) {
     hipLaunchKernelGGL(digit_hist_kernel<K>, dim3(n_tiles), dim3(kSortThreads), 0, s,
                        kb[cur], n, shift, counts, n_tiles, tiles);
-    if (int rc = exclusive_scan(partials, counts, n_tiles * 256, s)) return rc;
+    if (int rc = exclusive_scan_impl(partials, counts, n_tiles * 256, s)) return rc;
     hipLaunchKernelGGL(digit_scatter_kernel<K>, dim3(n_tiles), dim3(kSortThreads), 0, s,
                        kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, shift, counts,
                        n_tiles, tiles);
@@ -286,6 +286,15 @@ int radix_sort_impl(uint32_t* counts, uint32_t* partials, K* keys, uint32_t* val
 }
 
 }  // namespace
+
+int exclusive_scan_u32(uint32_t* partial, uint32_t* data, int64_t n, hipStream_t s) {
+  if (n <= 0) return PM_OK;
+  return exclusive_scan_impl(partial, data, n, s);
+}
+
+int64_t scan_partial_words(int64_t n) {
+  return (std::max<int64_t>(n, 1) + kScanChunk - 1) / kScanChunk + 1;
+}
 
 int64_t sort_count_words(int64_t n) {
   const int64_t n_tiles = (std::max<int64_t>(n, 1) + kSortTile - 1) / kSortTile;

@@ -247,148 +247,164 @@ __device__ __forceinline__ uint32_t shade_one(const RenderDev& R, const Segs& in
   const SceneDev& S = R.S;
   uint32_t flags = 0;
   const Segment sg = seg_load(in, i);
-  const V3 o = ldv(sg.o), d = ldv(sg.d), w = ldv(sg.w);
-  const uint32_t depth = sg.meta & 0xffffu, kind = (sg.meta >> 16) & 1u;
+  V3 o = ldv(sg.o), d = ldv(sg.d), w = ldv(sg.w);
+  uint32_t depth = sg.meta & 0xffffu;
+  const uint32_t kind = (sg.meta >> 16) & 1u;
   const uint32_t lineage = sg.meta >> 17;
   uint32_t dim = sg.dim;
   const uint64_t stream = sg.stream;
   const uint64_t seed = R.seed;
-  // integrateRecursive returns 0 at maxDepth (include/integrator.h:291)
-  if (kind == kKindCamera && static_cast<int>(depth) >= R.max_depth) return flags;
-  n_rays = 1;
-  const Hit h = closest_hit(S, o, d, INFINITY, stk);
-  if (h.tri >= 0) {
-    const pmx_triangle& tri = S.tris[h.tri];
-    const int m = material_of(S, tri);
-    const int type = S.mats[m].type;
-    const V3 rho = ldv(S.mats[m].rho);
-    const V3 x = o + d * h.t;
-    const V3 ns_raw = shading_normal(tri, h.u, h.v);
-    const V3 ns = normalize(ns_raw);
-    const V3 wo = d * -1.0f;
-    const float eta = S.ior ? S.ior[m] : 1.5f;
-    if (kind == kKindCamera) {
-      if (tri.is_light) {
-        // a light hit returns its emission (include/integrator.h:296-299)
-        const V3 le = emission(S, tri);
-        add_term(O.acc, O.bad, sg.sample, w.x * le.x, w.y * le.y, w.z * le.z);
-      } else if (type == PM_MATERIAL_LAMBERT) {
-        if (static_cast<int>(depth) >= R.fg_depth) {
-          push_query(O.qg, O.wg, i, x, ns_raw, wo, tri.material, w, sg.sample, lineage);
-          flags |= kFlagGlobal;
-        } else {
-          // direct illumination by light sampling (include/integrator.h:178-213)
-          if (S.n_lights > 0) {
-            int li = static_cast<int>(rnd(seed, stream, dim++) * S.n_lights);
-            li = li < S.n_lights - 1 ? li : S.n_lights - 1;
-            const pmx_triangle& lt = S.tris[S.lights[li]];
-            const V3 l0 = ld3(lt.v[0]);
-            const V3 lc = cross(ld3(lt.v[1]) - l0, ld3(lt.v[2]) - l0);
-            const float area = 0.5f * sqrtf(dot(lc, lc));
-            const float su0 = sqrtf(rnd(seed, stream, dim++));
-            const float b0 = 1.0f - su0, b1 = rnd(seed, stream, dim++) * su0;
-            const V3 lp = l0 * (1.0f - b0 - b1) + ld3(lt.v[1]) * b0 + ld3(lt.v[2]) * b1;
-            const V3 lns = shading_normal(lt, b0, b1);
-            const V3 to = lp - x;
-            const float r = sqrtf(dot(to, to));
-            const V3 wi = to * (1.0f / r);
-            const float pdf_dir = (1.0f / area) * r * r / fabsf(dot(wi * -1.0f, lns));
-            const bool blocked = occluded(S, x, wi, r - kRayEps, stk);
-            ++n_rays;
-            if (!blocked) {
-              const float f = lambert_on(wo, wi, ns);
-              const float c = f * fabsf(dot(wi, ns)) /
-                              ((1.0f / static_cast<float>(S.n_lights)) * pdf_dir);
-              const V3 le = emission(S, lt);
-              add_term(O.acc, O.bad, sg.sample, w.x * rho.x * c * le.x, w.y * rho.y * c * le.y,
-                       w.z * rho.z * c * le.z);
+  // A bounce with a single child (a sampled specular direction) continues
+  // here rather than as a segment of the next round, with the same state the
+  // segment would carry (so the same random numbers): chains of specular
+  // bounces, such as total internal reflection in glass, end within one
+  // round instead of keeping a few segments alive for many rounds.  depth
+  // grows with every continuation and each needs depth + 1 < max_depth, so
+  // the loop ends.
+  for (;;) {
+    // integrateRecursive returns 0 at maxDepth (include/integrator.h:291)
+    if (kind == kKindCamera && static_cast<int>(depth) >= R.max_depth) return flags;
+    ++n_rays;
+    const Hit h = closest_hit(S, o, d, INFINITY, stk);
+    if (h.tri >= 0) {
+      const pmx_triangle& tri = S.tris[h.tri];
+      const int m = material_of(S, tri);
+      const int type = S.mats[m].type;
+      const V3 rho = ldv(S.mats[m].rho);
+      const V3 x = o + d * h.t;
+      const V3 ns_raw = shading_normal(tri, h.u, h.v);
+      const V3 ns = normalize(ns_raw);
+      const V3 wo = d * -1.0f;
+      const float eta = S.ior ? S.ior[m] : 1.5f;
+      if (kind == kKindCamera) {
+        if (tri.is_light) {
+          // a light hit returns its emission (include/integrator.h:296-299)
+          const V3 le = emission(S, tri);
+          add_term(O.acc, O.bad, sg.sample, w.x * le.x, w.y * le.y, w.z * le.z);
+        } else if (type == PM_MATERIAL_LAMBERT) {
+          if (static_cast<int>(depth) >= R.fg_depth) {
+            push_query(O.qg, O.wg, i, x, ns_raw, wo, tri.material, w, sg.sample, lineage);
+            flags |= kFlagGlobal;
+          } else {
+            // direct illumination by light sampling (include/integrator.h:178-213)
+            if (S.n_lights > 0) {
+              int li = static_cast<int>(rnd(seed, stream, dim++) * S.n_lights);
+              li = li < S.n_lights - 1 ? li : S.n_lights - 1;
+              const pmx_triangle& lt = S.tris[S.lights[li]];
+              const V3 l0 = ld3(lt.v[0]);
+              const V3 lc = cross(ld3(lt.v[1]) - l0, ld3(lt.v[2]) - l0);
+              const float area = 0.5f * sqrtf(dot(lc, lc));
+              const float su0 = sqrtf(rnd(seed, stream, dim++));
+              const float b0 = 1.0f - su0, b1 = rnd(seed, stream, dim++) * su0;
+              const V3 lp = l0 * (1.0f - b0 - b1) + ld3(lt.v[1]) * b0 + ld3(lt.v[2]) * b1;
+              const V3 lns = shading_normal(lt, b0, b1);
+              const V3 to = lp - x;
+              const float r = sqrtf(dot(to, to));
+              const V3 wi = to * (1.0f / r);
+              const float pdf_dir = (1.0f / area) * r * r / fabsf(dot(wi * -1.0f, lns));
+              const bool blocked = occluded(S, x, wi, r - kRayEps, stk);
+              ++n_rays;
+              if (!blocked) {
+                const float f = lambert_on(wo, wi, ns);
+                const float c = f * fabsf(dot(wi, ns)) /
+                                ((1.0f / static_cast<float>(S.n_lights)) * pdf_dir);
+                const V3 le = emission(S, lt);
+                add_term(O.acc, O.bad, sg.sample, w.x * rho.x * c * le.x, w.y * rho.y * c * le.y,
+                         w.z * rho.z * c * le.z);
+              }
+            }
+            // caustics by the caustics map (include/integrator.h:155-175)
+            if (R.caustics)
+              {
+              push_query(O.qc, O.wc, i, x, ns_raw, wo, tri.material, w, sg.sample, lineage);
+              flags |= kFlagCaustics;
+            }
+            // final gathering (include/integrator.h:215-252), recursion depth 0
+            if (R.max_depth > 0) {
+              float scale;
+              const V3 dir = sample_bxdf(type, eta, wo, ns, false, seed, stream, dim, scale);
+              const float c = fabsf(dot(ns, dir)) * scale;
+              push_segment(O.seg, i, flags, x, dir, mul(w, rho) * c, sg.sample, dim, 0,
+                           kKindGather, lineage | 0x80u, stream);
             }
           }
-          // caustics by the caustics map (include/integrator.h:155-175)
-          if (R.caustics)
-            {
-            push_query(O.qc, O.wc, i, x, ns_raw, wo, tri.material, w, sg.sample, lineage);
-            flags |= kFlagCaustics;
-          }
-          // final gathering (include/integrator.h:215-252), recursion depth 0
-          if (R.max_depth > 0) {
+        } else if (static_cast<int>(depth) + 1 < R.max_depth) {
+          // specular: one sampled direction from depth 3 on, every direction below
+          const V3 ng = geometric_normal(tri);
+          if (static_cast<int>(depth) >= kSplitDepth) {
             float scale;
             const V3 dir = sample_bxdf(type, eta, wo, ns, false, seed, stream, dim, scale);
-            const float c = fabsf(dot(ns, dir)) * scale;
-            push_segment(O.seg, i, flags, x, dir, mul(w, rho) * c, sg.sample, dim, 0,
-                         kKindGather, lineage | 0x80u, stream);
-          }
-        }
-      } else if (static_cast<int>(depth) + 1 < R.max_depth) {
-        // specular: one sampled direction from depth 3 on, every direction below
-        const V3 ng = geometric_normal(tri);
-        if (static_cast<int>(depth) >= kSplitDepth) {
-          float scale;
-          const V3 dir = sample_bxdf(type, eta, wo, ns, false, seed, stream, dim, scale);
-          const float c = cos_term(wo, dir, ns, ng, false) * scale;
-          push_segment(O.seg, i, flags, x, dir, mul(w, rho) * c, sg.sample, dim, depth + 1,
-                       kKindCamera, lineage, stream);
-        } else {
-          // sampleAllDirection (include/material.h:123-129, 203-243) in the
-          // shading frame; each branch's weight is f |cos| with f = (its
-          // share) * rho / |cos|
-          V3 ft, fb;
-          basis(ns, ft, fb);
-          const V3 wol = v3(dot(wo, ft), dot(wo, ns), dot(wo, fb));
-          V3 dirs[2];
-          float share[2];
-          int nb = 0;
-          if (type == PM_MATERIAL_MIRROR) {
-            dirs[0] = v3(-wol.x, wol.y, -wol.z);
-            share[0] = 1.0f;
-            nb = 1;
+            const float c = cos_term(wo, dir, ns, ng, false) * scale;
+            o = x;
+            d = dir;
+            w = mul(w, rho) * c;
+            ++depth;
+            continue;
           } else {
-            const bool outside = wol.y > 0.0f;
-            const float eta_o = outside ? 1.0f : eta, eta_i = outside ? eta : 1.0f;
-            const V3 n = outside ? v3(0.f, 1.f, 0.f) : v3(0.f, -1.f, 0.f);
-            const float cos_o = dot(wol, n);
-            const float fr = schlick(cos_o, eta_o, eta_i);
-            dirs[0] = wol * -1.0f + n * (2.0f * cos_o);
-            share[0] = fr;
-            nb = 1;
-            const V3 th = (wol - n * cos_o) * (-eta_o / eta_i);
-            const float th2 = dot(th, th);
-            if (th2 <= 1.0f) {
-              dirs[1] = th + n * (-sqrtf(fmaxf(1.0f - th2, 0.0f)));
-              share[1] = (1.0f - fr) * (eta_o * eta_o) / (eta_i * eta_i);
-              nb = 2;
+            // sampleAllDirection (include/material.h:123-129, 203-243) in the
+            // shading frame; each branch's weight is f |cos| with f = (its
+            // share) * rho / |cos|
+            V3 ft, fb;
+            basis(ns, ft, fb);
+            const V3 wol = v3(dot(wo, ft), dot(wo, ns), dot(wo, fb));
+            V3 dirs[2];
+            float share[2];
+            int nb = 0;
+            if (type == PM_MATERIAL_MIRROR) {
+              dirs[0] = v3(-wol.x, wol.y, -wol.z);
+              share[0] = 1.0f;
+              nb = 1;
             } else {
-              share[0] = 1.0f;  // total internal reflection
+              const bool outside = wol.y > 0.0f;
+              const float eta_o = outside ? 1.0f : eta, eta_i = outside ? eta : 1.0f;
+              const V3 n = outside ? v3(0.f, 1.f, 0.f) : v3(0.f, -1.f, 0.f);
+              const float cos_o = dot(wol, n);
+              const float fr = schlick(cos_o, eta_o, eta_i);
+              dirs[0] = wol * -1.0f + n * (2.0f * cos_o);
+              share[0] = fr;
+              nb = 1;
+              const V3 th = (wol - n * cos_o) * (-eta_o / eta_i);
+              const float th2 = dot(th, th);
+              if (th2 <= 1.0f) {
+                dirs[1] = th + n * (-sqrtf(fmaxf(1.0f - th2, 0.0f)));
+                share[1] = (1.0f - fr) * (eta_o * eta_o) / (eta_i * eta_i);
+                nb = 2;
+              } else {
+                share[0] = 1.0f;  // total internal reflection
+              }
+            }
+            // (both slots unrolled, so dirs and share stay in registers)
+  #pragma unroll
+            for (int b = 0; b < 2; ++b) {
+              if (b >= nb) break;
+              const V3 dl = dirs[b];
+              const V3 dir = ft * dl.x + ns * dl.y + fb * dl.z;
+              const float c = share[b] / fabsf(dl.y) * fabsf(dot(dir, ns));
+              push_segment(O.seg, i, flags, x, dir, mul(w, rho) * c, sg.sample, 0, depth + 1,
+                           kKindCamera, 2 * lineage + b, child_stream(stream, b));
             }
           }
-          // (both slots unrolled, so dirs and share stay in registers)
-#pragma unroll
-          for (int b = 0; b < 2; ++b) {
-            if (b >= nb) break;
-            const V3 dl = dirs[b];
-            const V3 dir = ft * dl.x + ns * dl.y + fb * dl.z;
-            const float c = share[b] / fabsf(dl.y) * fabsf(dot(dir, ns));
-            push_segment(O.seg, i, flags, x, dir, mul(w, rho) * c, sg.sample, 0, depth + 1,
-                         kKindCamera, 2 * lineage + b, child_stream(stream, b));
-          }
+        }
+      } else {
+        // a final-gathering ray: the photon map at a diffuse hit, or one more
+        // sampled bounce at a specular hit
+        if (type == PM_MATERIAL_LAMBERT) {
+          push_query(O.qg, O.wg, i, x, ns_raw, wo, tri.material, w, sg.sample, lineage);
+          flags |= kFlagGlobal;
+        } else if (static_cast<int>(depth) + 1 < R.max_depth) {
+          float scale;
+          const V3 dir = sample_bxdf(type, eta, wo, ns, false, seed, stream, dim, scale);
+          const float c = fabsf(dot(ns, dir)) * scale;
+          o = x;
+          d = dir;
+          w = mul(w, rho) * c;
+          ++depth;
+          continue;
         }
       }
-    } else {
-      // a final-gathering ray: the photon map at a diffuse hit, or one more
-      // sampled bounce at a specular hit
-      if (type == PM_MATERIAL_LAMBERT) {
-        push_query(O.qg, O.wg, i, x, ns_raw, wo, tri.material, w, sg.sample, lineage);
-        flags |= kFlagGlobal;
-      } else if (static_cast<int>(depth) + 1 < R.max_depth) {
-        float scale;
-        const V3 dir = sample_bxdf(type, eta, wo, ns, false, seed, stream, dim, scale);
-        const float c = fabsf(dot(ns, dir)) * scale;
-        push_segment(O.seg, i, flags, x, dir, mul(w, rho) * c, sg.sample, dim, depth + 1,
-                     kKindGather, lineage, stream);
-      }
     }
+    return flags;
   }
-  return flags;
 }
 
 __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R, Segs in, uint32_t n_in,
@@ -399,7 +415,7 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R, Segs in, u
   if (i >= n_in) return;
   uint32_t n_rays = 0;
   O.flags[i] = shade_one(R, in, i, O, stk, n_rays);
-  wave_count_add<2>(O.rays, n_rays);
+  wave_count_add<16>(O.rays, n_rays);
 }
 
 // The three per-segment counts of a shade round (segments, global queries,

@@ -260,9 +260,6 @@ int radix_sort_u32(Workspace* ws, uint32_t* keys, uint32_t* vals,
                    hipStream_t s, int* which, int lo_bit = 0);
 // Reserves the sort histograms and scan partials for n keys.
 int sort_reserve(Workspace* ws, int64_t n);
-int radix_sort_u64(Workspace* ws, uint64_t* keys, uint32_t* vals,
-                   uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
-                   hipStream_t s, int* which, int lo_bit = 0);
 // In-place exclusive prefix sum of n uint32 words (plain kernels, no
 // allocation); partial holds scan_partial_words(n) words.
 int exclusive_scan_u32(uint32_t* partial, uint32_t* data, int64_t n, hipStream_t s);
@@ -275,7 +272,7 @@ int64_t sort_partial_words(int64_t n);
 int64_t sort_seg_tiles(int64_t n, int n_seg);
 int64_t sort_seg_count_words(int64_t n, int n_seg);
 int64_t sort_seg_partial_words(int64_t n, int n_seg);
-// Stable sort of every segment of a tile table (launch_sort_tiles) on bits
+// Stable sort of every segment of a tile table (sort_tiles_kernel) on bits
 // [lo_bit, bits): each segment's pairs stay in its own range.
 int radix_sort_seg_u32_raw(uint32_t* counts, uint32_t* partials, const SortTile* tiles,
                            int64_t n_tiles, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
@@ -285,9 +282,6 @@ int radix_sort_seg_u32_raw(uint32_t* counts, uint32_t* partials, const SortTile*
 // least the words above): they allocate nothing.
 int radix_sort_u32_raw(uint32_t* counts, uint32_t* partials, uint32_t* keys, uint32_t* vals,
                        uint32_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
-                       hipStream_t s, int* which, int lo_bit = 0);
-int radix_sort_u64_raw(uint32_t* counts, uint32_t* partials, uint64_t* keys, uint32_t* vals,
-                       uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
                        hipStream_t s, int* which, int lo_bit = 0);
 
 int build_photon_map(PhotonMapImpl* map, hipStream_t s);

@@ -337,12 +337,6 @@ int radix_sort_u32_raw(uint32_t* counts, uint32_t* partials, uint32_t* keys, uin
                                    s, which, lo_bit);
 }
 
-int radix_sort_u64_raw(uint32_t* counts, uint32_t* partials, uint64_t* keys, uint32_t* vals,
-                       uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
-                       hipStream_t s, int* which, int lo_bit) {
-  return radix_sort_impl<uint64_t>(counts, partials, keys, vals, keys_alt, vals_alt, n, bits,
-                                   s, which, lo_bit);
-}
 
 int radix_sort_u32(Workspace* ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                    uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which,
@@ -352,12 +346,5 @@ int radix_sort_u32(Workspace* ws, uint32_t* keys, uint32_t* vals, uint32_t* keys
                             keys_alt, vals_alt, n, bits, s, which, lo_bit);
 }
 
-int radix_sort_u64(Workspace* ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
-                   uint32_t* vals_alt, int64_t n, int bits, hipStream_t s, int* which,
-                   int lo_bit) {
-  if (int rc = sort_reserve(ws, n)) return rc;
-  return radix_sort_u64_raw(ws->counts.as<uint32_t>(), ws->partials.as<uint32_t>(), keys, vals,
-                            keys_alt, vals_alt, n, bits, s, which, lo_bit);
-}
 
 }  // namespace pm

@@ -281,6 +281,18 @@ __device__ __forceinline__ float funkey(uint32_t k) {
 __device__ __forceinline__ float wave_min(float v) { return funkey(wave_min_u32(fkey(v))); }
 __device__ __forceinline__ float wave_max(float v) { return funkey(wave_max_u32(fkey(v))); }
 
+// Wave-uniform sum of a float, in a fixed order (the same scan as above).
+__device__ __forceinline__ float wave_sum(float v) {
+  auto add = [](float x, uint32_t y) { return x + __uint_as_float(y); };
+  v = add(v, dpp_u32<0x111, 0xf>(0u, __float_as_uint(v)));
+  v = add(v, dpp_u32<0x112, 0xf>(0u, __float_as_uint(v)));
+  v = add(v, dpp_u32<0x114, 0xf>(0u, __float_as_uint(v)));
+  v = add(v, dpp_u32<0x118, 0xf>(0u, __float_as_uint(v)));
+  v = add(v, dpp_u32<0x142, 0xa>(0u, __float_as_uint(v)));
+  v = add(v, dpp_u32<0x143, 0xc>(0u, __float_as_uint(v)));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), kWave - 1));
+}
+
 // Square roots for radius bounds, from the hardware instruction (1 ulp);
 // every bound carries a relative margin far above that.  The upper form
 // never underestimates a denormal argument, the lower never overestimates.
@@ -558,13 +570,14 @@ struct SharedNormal {
 template <int ST, typename Body>
 __device__ __forceinline__ void for_candidates_st(DescRef D, GatherShared& sh, int lane,
                                                   uint32_t nr, uint32_t total,
-                                                  const SharedNormal& sn, Body&& body) {
-  if (total == 0) return;
+                                                  const SharedNormal& sn, Body&& body,
+                                                  uint32_t begin = 0) {
+  if (total <= begin) return;
   constexpr bool kWF = ST != kStagePos;
   const float4* __restrict__ P = D.pos4;
   const float4* __restrict__ W = D.wif4;
   const float2* __restrict__ F = D.flux2;
-  uint32_t t = lane;
+  uint32_t t = begin + static_cast<uint32_t>(lane);
   uint32_t r = 0;
   float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p;
   float2 f = make_float2(0.f, 0.f);
@@ -577,7 +590,7 @@ __device__ __forceinline__ void for_candidates_st(DescRef D, GatherShared& sh, i
       f = F[slot];
     }
   }
-  for (uint32_t o = 0; o < total; o += kChunk) {
+  for (uint32_t o = begin; o < total; o += kChunk) {
     sh.u.c.pos[lane] = p;
     if (ST == kStageFull) {
       sh.u.c.wif[lane] = w;
@@ -612,10 +625,11 @@ __device__ __forceinline__ void for_candidates_st(DescRef D, GatherShared& sh, i
 
 template <bool WF, typename Body>
 __device__ __forceinline__ void for_candidates(DescRef D, GatherShared& sh, int lane,
-                                               uint32_t nr, uint32_t total, Body&& body) {
+                                               uint32_t nr, uint32_t total, Body&& body,
+                                               uint32_t begin = 0) {
   const SharedNormal none{0.0f, 0.0f, 0.0f, 0.0f};
-  for_candidates_st<WF ? kStageFull : kStagePos>(D, sh, lane, nr, total, none,
-                                                 [&](uint32_t cnt, uint64_t) { body(cnt); });
+  for_candidates_st<WF ? kStageFull : kStagePos>(
+      D, sh, lane, nr, total, none, [&](uint32_t cnt, uint64_t) { body(cnt); }, begin);
 }
 
 // The normal of the lanes with on set, when they agree: true (and *sn set,
@@ -760,6 +774,66 @@ __device__ uint32_t filter_slots(DescRef D, GatherShared& sh, int lane, uint32_t
   }
   __syncthreads();
   return static_cast<uint32_t>(uni(static_cast<int>(out)));
+}
+
+// The run's core: the photons of the explicit slot list that lie below every
+// lane's window by the triangle inequality, d(c, p) < min over the run's
+// lanes of (sqrt(bound_q) - |q - c|) with bound_q a lane's lower key bound
+// (its squared distance part).  Such a photon's FP32 d^2 from every lane of
+// the run is below that lane's bound (the 1e-4 relative margin covers the
+// FP32 roundings), so a histogram pass counts it in every lane's lowest bin
+// and a selection pass finds it below klo for every lane, without its
+// pairwise d^2.  Returns the squared core radius (0 when there is none).
+__device__ __forceinline__ float core_radius2(bool part, float bound_d2, float dq) {
+  float cr = INFINITY;
+  if (part) cr = (bound_d2 > 0.0f && bound_d2 <= FLT_MAX) ? sqrtf(bound_d2) * 0.9999f - dq : 0.0f;
+  cr = wave_min(cr);
+  return cr > 0.0f ? cr * cr : 0.0f;
+}
+
+// Keeps the entries of the explicit slot list whose photon has FP32 d^2 from
+// c <= lim2, the core ones (d^2 from c < core2) first, each part in list
+// order; returns the new length and the core's in *n_core.  Core entries are
+// compacted in place (an entry never moves up past one not yet read, as in
+// filter_slots), the others go through cscratch, which is free whenever a
+// slot list is complete.
+__device__ uint32_t partition_slots(DescRef D, GatherShared& sh, int lane, uint32_t total,
+                                    float cx, float cy, float cz, float lim2, float core2,
+                                    uint32_t* n_core) {
+  const float4* __restrict__ P = D.pos4;
+  uint32_t nc = 0, nk = 0;
+  for (uint32_t base = 0; base < total; base += 4 * kWave) {
+    uint32_t sl[4];
+    float4 p[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t t = base + static_cast<uint32_t>(u * kWave + lane);
+      sl[u] = 0;
+      p[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (t < total) {
+        sl[u] = sh.cslot[t];
+        p[u] = P[sl[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t t = base + static_cast<uint32_t>(u * kWave + lane);
+      const float d2 = d2_ref(cx, cy, cz, p[u].x, p[u].y, p[u].z);
+      const bool core = t < total && d2 < core2;
+      const bool keep = t < total && !core && d2 <= lim2;
+      const uint64_t cm = __ballot(core);
+      if (core) sh.cslot[nc + lanes_below(cm, lane)] = sl[u];
+      nc += static_cast<uint32_t>(__popcll(cm));
+      const uint64_t km = __ballot(keep);
+      if (keep) sh.cscratch[nk + lanes_below(km, lane)] = sl[u];
+      nk += static_cast<uint32_t>(__popcll(km));
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = static_cast<uint32_t>(lane); i < nk; i += kWave) sh.cslot[nc + i] = sh.cscratch[i];
+  __syncthreads();
+  *n_core = static_cast<uint32_t>(uni(static_cast<int>(nc)));
+  return static_cast<uint32_t>(uni(static_cast<int>(nc + nk)));
 }
 
 // A staged candidate as the consume loops see it.
@@ -1057,7 +1131,9 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
   float rs2 = rs * rs + 1e-30f;
 
   st.klo = 0; st.khi = 0; st.need = 0; st.cnt = 0;
+  uint32_t n_core = 0;  // leading core photons of the slot list (pass A skips them)
   for (int attempt = 0; attempt < 2; ++attempt) {
+    n_core = 0;
     uint32_t ns = kSlotList;
     if (attempt == 0 && rs2 < INFINITY) {
       // exactly the photons of the run's sphere, when they fit
@@ -1087,6 +1163,13 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
         c1 = 1.0f;
       }
     }
+    if (nr == kSlotList) {
+      // the core goes first and counts in every lane's lowest bin unbinned
+      // (a lane with a degenerate bracket has none)
+      const float core2 = core_radius2(part, inv_w > 0.0f ? lo2 : 0.0f, dq);
+      if (core2 > 0.0f)
+        *total = partition_slots(D, sh, lane, *total, cx, cy, cz, INFINITY, core2, &n_core);
+    }
     clear_hist(sh, lane);
     __syncthreads();
     for_candidates<false>(D, sh, lane, nr, *total, [&](uint32_t cnt) {
@@ -1094,7 +1177,8 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
         hist_add(sh, bin_of(d2_ref(qx, qy, qz, c.p.x, c.p.y, c.p.z), inv_w, c1), lane);
         return false;
       });
-    });
+    }, n_core);
+    if (part) sh.hist[lane] += n_core;  // bin 0
     __syncthreads();
     pt.mark(3, lane);
     uint32_t cum = 0, below = 0, cb = 0, bsel = kBins + 1;
@@ -1150,7 +1234,7 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
           hist_add(sh, static_cast<uint32_t>((key - st.klo) / w), lane);
         }
       }
-    });
+    }, n_core);  // the core lies below every lane's window
     __syncthreads();
     if (mine) {
       uint32_t cum = 0;
@@ -1172,6 +1256,52 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
   }
   pt.mark(4, lane);
   return nr;
+}
+
+// The flux of the run's core (the first n entries of the slot list, each
+// below every lane's klo) that passes the Lambert sign test of a lane with on
+// set, added to (ar, ag, ab): the sign decided once with the shared normal
+// as kStageEff decides it, photons it leaves open by each lane's own test.
+__device__ void add_core_flux(DescRef D, const GatherShared& sh, int lane, uint32_t n,
+                              const SharedNormal& sn, bool on, float nx, float ny, float nz,
+                              float& ar, float& ag, float& ab) {
+  float sr = 0.0f, sg = 0.0f, sb = 0.0f;
+  for (uint32_t b = 0; b < n; b += kWave) {
+    const uint32_t t = b + static_cast<uint32_t>(lane);
+    float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 f = make_float2(0.f, 0.f);
+    if (t < n) {
+      const uint32_t s = sh.cslot[t];
+      w = D.wif4[s];
+      f = D.flux2[s];
+    }
+    const float d0 = dot_ref(w.x, w.y, w.z, sn.x, sn.y, sn.z);
+    const float mw = (fabsf(w.x) + fabsf(w.y) + fabsf(w.z)) * sn.m + 1e-30f;
+    const bool amb = t < n && !(fabsf(d0) > mw);
+    if (t < n && !amb && d0 > 0.0f) {
+      sr += w.w;
+      sg += f.x;
+      sb += f.y;
+    }
+    for (uint64_t am = __ballot(amb); am != 0ull; am &= am - 1ull) {
+      const uint32_t s = sh.cslot[b + static_cast<uint32_t>(__builtin_ctzll(am))];
+      const float4 wj = D.wif4[s];
+      const float2 fj = D.flux2[s];
+      if (on && !(dot_ref(wj.x, wj.y, wj.z, nx, ny, nz) < 0.0f)) {
+        ar = ar + wj.w;
+        ag = ag + fj.x;
+        ab = ab + fj.y;
+      }
+    }
+  }
+  sr = wave_sum(sr);
+  sg = wave_sum(sg);
+  sb = wave_sum(sb);
+  if (on) {
+    ar += sr;
+    ag += sg;
+    ab += sb;
+  }
 }
 
 template <int K>
@@ -1320,6 +1450,13 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
       SelState st;
       uint32_t total = 0;
       uint32_t nr = 0;
+      // When the accumulating lanes' normals agree within 5%, each photon's
+      // Lambert sign is decided once per chunk (kStageEff) instead of per
+      // lane; photons it leaves open take the lane's own test in the drain.
+      const bool acc_lane = part && lane_lambert;
+      SharedNormal sn{0.0f, 0.0f, 0.0f, 0.0f};
+      bool shared = false;
+      uint32_t n_core = 0;  // leading core photons of the slot list (pass B skips them)
       if (reuse) {
         st.klo = tau_in + 1; st.khi = tau_in + 1; st.need = 0; st.cnt = 0;
         const float xs = part ? sqrt_up(cull_bound(r0sq)) * 1.00002f + dq : 0.0f;
@@ -1361,13 +1498,24 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         // With the lane windows known, every key below a lane's khi lies
         // within sqrt(d^2 of khi) + |q - c| of c: the slot list shrinks to
         // the sphere that holds them all before the later passes read it.
+        shared = kAccum && shared_normal(acc_lane, nx, ny, nz, &sn);
         if (nr == kSlotList) {
           const float kd2 = __uint_as_float(static_cast<uint32_t>(st.khi >> 32));
           float xr = 0.0f;
           if (part) xr = kd2 <= FLT_MAX ? sqrt_up(cull_bound(kd2)) * 1.00002f + dq : INFINITY;
           const float rb = wave_max(xr) * 1.0002f;
-          if (rb < INFINITY) {
-            total = filter_slots(D, sh, lane, total, cx, cy, cz, rb * rb + 1e-30f);
+          const float lim2 = rb < INFINITY ? rb * rb + 1e-30f : INFINITY;
+          // The core, below every lane's klo, is put first: pass B adds its
+          // flux once for the run when the sign test is shared (the adjoint's
+          // selection has nothing to do with it at all).
+          float core2 = 0.0f;
+          if (shared || MODE == kModeAdjoint) {
+            core2 = core_radius2(part, __uint_as_float(static_cast<uint32_t>(st.klo >> 32)), dq);
+          }
+          if (core2 > 0.0f) {
+            total = partition_slots(D, sh, lane, total, cx, cy, cz, lim2, core2, &n_core);
+          } else if (rb < INFINITY) {
+            total = filter_slots(D, sh, lane, total, cx, cy, cz, lim2);
           }
           if (count_stats && lane == 0) atomicAdd(&counters[kStatCand3], total);
         }
@@ -1379,12 +1527,6 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
       // (the adjoint runs it only to finish the selection; its scatter pass
       // follows once tau is known)
       if (!reuse) {
-        const bool acc_lane = part && lane_lambert;
-        // When the accumulating lanes' normals agree within 5%, each photon's
-        // Lambert sign is decided once per chunk (kStageEff) instead of per
-        // lane; photons it leaves open take the lane's own test in the drain.
-        SharedNormal sn{0.0f, 0.0f, 0.0f, 0.0f};
-        const bool shared = kAccum && !all && shared_normal(acc_lane, nx, ny, nz, &sn);
         auto pass_b = [&](auto kind_tag, auto shared_tag) {
           constexpr int kind = decltype(kind_tag)::value;  // 0 all, 1 keys, 2 d^2 bits
           constexpr bool kShared = decltype(shared_tag)::value;
@@ -1392,6 +1534,9 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
           const uint64_t span = st.khi - st.klo;
           const uint32_t lo32 = static_cast<uint32_t>(st.klo >> 32);
           const uint32_t span32 = static_cast<uint32_t>(st.khi >> 32) - lo32;
+          if (kShared && n_core > 0u) {
+            add_core_flux(D, sh, lane, n_core, sn, acc_lane, nx, ny, nz, acc_r, acc_g, acc_b);
+          }
           for_candidates_st<kSt>(D, sh, lane, nr, total, sn, [&](uint32_t cnt, uint64_t amb) {
             const uint64_t m = consume_st<kSt>(sh, cnt, [&](const Cand& c) {
               const float d2 = d2_ref(qx, qy, qz, c.p.x, c.p.y, c.p.z);
@@ -1454,7 +1599,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
                 }
               }
             }
-          });
+          }, n_core);
         };
         using No = std::false_type;
         using Yes = std::true_type;

@@ -106,7 +106,7 @@ typedef struct pm_stats {
 
 /* Options for pm_context_set_option. */
 #define PM_OPT_TIMING 1         /* 0/1: record HIP events (syncs stream)    */
-#define PM_OPT_WAVES_PER_CU 2   /* persistent waves per CU (default 12)     */
+#define PM_OPT_WAVES_PER_CU 2   /* persistent waves per CU (default 16)     */
 #define PM_OPT_ANCHOR_STRIDE 3  /* photons per radius anchor, 1..4096 (4)   */
 #define PM_OPT_LEAF_SIZE 4      /* default kd-tree leaf size (default 8)    */
 #define PM_OPT_XCD_AWARE 5      /* 0/1: per-XCD work queues (default 1)     */

@@ -1132,7 +1132,7 @@ int pm_autotune(pm_photonmap* h, int64_t m, const pm_query* q, const pm_material
   }
   ctx->run_split = best_split;
   int best_waves = ctx->waves_per_cu;
-  for (int w : {6, 8, 10, 12, 16}) {
+  for (int w : {8, 12, 16, 20}) {
     if (w == best_waves) continue;
     ctx->waves_per_cu = w;
     double t = 0.0;

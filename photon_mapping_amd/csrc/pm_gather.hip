@@ -115,45 +115,42 @@ struct RangeList {
   uint32_t rpre[kMaxRanges];
 };
 
-// Pass B's use of the lane-bin storage: the per-lane stash (key and slot,
-// entry e of lane l at [e * kWave + l]) and the slots of the staged chunk.
+// Pass B's use of the lane-bin storage: the per-lane stash of keys (entry e
+// of lane l at [e * kWave + l]; a key's photon slot is found again through
+// the map's inverse permutation) and the slots of the staged chunk.
 struct StashArea {
   uint64_t key[kStashCap * kWave];
-  uint32_t slot[kStashCap * kWave];
   uint32_t chunk_slot[kChunk];
 };
 
 // One wave's LDS (one-wave workgroups).  Buffers whose lifetimes do not
-// overlap share storage, which keeps the block at 13 KB so that 12 waves
-// fit a CU's 160 KB.
+// overlap share storage, which keeps the block at 10 KB so that 16 waves
+// fit a CU's 160 KB: the candidate list and the staged positions live
+// throughout; the lane bins (pass A, refinement, centre bins, the adjoint's
+// query staging), pass B's staged flux and stash, collect_slots' output and
+// the breadth-first frontier of a list walk take turns in the rest.
 struct GatherShared {
-  union {
-    uint32_t hist[(kBins + 2) * kWave];  // per-lane bins; centre bins alias the start
-    StashArea sb;                        // pass B / finish
-    uint32_t cscratch[kSlotCap];         // collect_slots output before it is accepted
-  };
-  // The chunk staging buffers are dead while a candidate list is built, so
-  // the breadth-first frontier shares their storage.
-  union {
-    struct {
-      float4 pos[kChunk];
-      float4 wif[kChunk];  // with a shared normal: effective flux (rgb, ambiguous flag)
-      float2 flux[kChunk];
-    } c;
-    uint32_t front[2][kFront];
-  } u;
   // A candidate list is a subtree list or an explicit slot list, never both.
   union {
     RangeList rl;
     uint32_t cslot[kSlotCap];
+  };
+  float4 pos[kChunk];  // staged candidate positions
+  union {
+    uint32_t hist[(kBins + 2) * kWave];  // per-lane bins
+    struct {
+      float4 wif[kChunk];  // with a shared normal: effective flux (rgb)
+      float2 flux[kChunk];
+      StashArea sb;
+    } b;                                 // pass B / finish
+    uint32_t cscratch[kSlotCap];         // collect_slots output before it is accepted
+    uint32_t front[2][kFront];           // list walks
   };
 };
 // The adjoint kernel also accumulates dL/drho per material in LDS.
 struct AdjointShared : GatherShared {
   float drho[3 * kDrhoSlots];
 };
-static_assert(sizeof(StashArea) <= (kBins + 2) * kWave * sizeof(uint32_t),
-              "the stash fits the lane bins");
 static_assert(kSlotCap * sizeof(uint32_t) <= sizeof(RangeList), "slot list fits the range list");
 static_assert(kCentreBins <= (kBins + 2) * kWave, "centre bins alias the lane bins");
 
@@ -323,11 +320,9 @@ struct SelState {
 struct Stash {
   uint32_t n;
   __device__ __forceinline__ void reset() { n = 0; }
-  __device__ __forceinline__ void put(GatherShared& sh, int lane, uint64_t k, uint32_t slot) {
+  __device__ __forceinline__ void put(GatherShared& sh, int lane, uint64_t k) {
     if (n < static_cast<uint32_t>(kStashCap)) {
-      const uint32_t at = n * kWave + static_cast<uint32_t>(lane);
-      sh.sb.key[at] = k;
-      sh.sb.slot[at] = slot;
+      sh.b.sb.key[n * kWave + static_cast<uint32_t>(lane)] = k;
       ++n;
     }
   }
@@ -425,7 +420,7 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
     nr = 0;
     bool full = false;
     int full_depth = 0;
-    if (lane == 0) sh.u.front[0][0] = 1u;
+    if (lane == 0) sh.front[0][0] = 1u;
     __syncthreads();
     uint32_t ncur = 1;
     int cur = 0;
@@ -442,7 +437,7 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
         uint32_t f = 0;
         bool emit = false, push = false;
         if (i < width) {
-          f = (sh.u.front[cur][i >> e] << e) | (i & sub);
+          f = (sh.front[cur][i >> e] << e) | (i & sub);
           const float4 mn = D.boxes[2 * f];
           const float4 mx = D.boxes[2 * f + 1];
           float n2, f2;
@@ -470,7 +465,7 @@ __device__ uint32_t build_list(DescRef D, float cx, float cy, float cz, float r2
           break;
         }
         if (emit) sh.rl.rnode[nr + lanes_below(em, lane)] = f;
-        if (push) sh.u.front[cur ^ 1][nnext + lanes_below(pmk, lane)] = f;
+        if (push) sh.front[cur ^ 1][nnext + lanes_below(pmk, lane)] = f;
         nr += ne;
         nnext += static_cast<uint32_t>(__popcll(pmk));
       }
@@ -554,8 +549,8 @@ struct SharedNormal {
 };
 
 // Iterates the candidate photons of the current list: body(cnt, amb) is
-// called once per staged chunk, after sh.u.c.pos (and with kStageFull also
-// sh.u.c.wif/flux, with kStageEff the effective flux in sh.u.c.wif) hold cnt
+// called once per staged chunk, after sh.pos (and with kStageFull also
+// sh.b.wif/flux, with kStageEff the effective flux in sh.b.wif) hold cnt
 // candidates; amb has bit j set for each staged photon j whose sign
 // kStageEff left open.  The next chunk's loads are in flight while body
 // runs.
@@ -591,10 +586,10 @@ __device__ __forceinline__ void for_candidates_st(DescRef D, GatherShared& sh, i
     }
   }
   for (uint32_t o = begin; o < total; o += kChunk) {
-    sh.u.c.pos[lane] = p;
+    sh.pos[lane] = p;
     if (ST == kStageFull) {
-      sh.u.c.wif[lane] = w;
-      sh.u.c.flux[lane] = f;
+      sh.b.wif[lane] = w;
+      sh.b.flux[lane] = f;
     }
     uint64_t amb_mask = 0ull;
     if (ST == kStageEff) {
@@ -602,10 +597,10 @@ __device__ __forceinline__ void for_candidates_st(DescRef D, GatherShared& sh, i
       const float mw = (fabsf(w.x) + fabsf(w.y) + fabsf(w.z)) * sn.m + 1e-30f;
       const bool amb = !(fabsf(d0) > mw);
       const bool inc = !amb && d0 > 0.0f;
-      sh.u.c.wif[lane] = make_float4(inc ? w.w : 0.0f, inc ? f.x : 0.0f, inc ? f.y : 0.0f, 0.0f);
+      sh.b.wif[lane] = make_float4(inc ? w.w : 0.0f, inc ? f.x : 0.0f, inc ? f.y : 0.0f, 0.0f);
       amb_mask = __ballot(amb && o + static_cast<uint32_t>(lane) < total);
     }
-    if (kWF) sh.sb.chunk_slot[lane] = static_cast<uint32_t>(slot);
+    if (kWF) sh.b.sb.chunk_slot[lane] = static_cast<uint32_t>(slot);  // (aliases the bins)
     __syncthreads();
     t += kChunk;
     if (t < total) {
@@ -846,11 +841,11 @@ struct Cand {
 template <int ST>
 __device__ __forceinline__ Cand load_cand(const GatherShared& sh, uint32_t j) {
   Cand c;
-  c.p = sh.u.c.pos[j];
+  c.p = sh.pos[j];
   c.w = make_float4(0.f, 0.f, 0.f, 0.f);
   c.f = make_float2(0.f, 0.f);
-  if (ST != kStagePos) c.w = sh.u.c.wif[j];
-  if (ST == kStageFull) c.f = sh.u.c.flux[j];
+  if (ST != kStagePos) c.w = sh.b.wif[j];
+  if (ST == kStageFull) c.f = sh.b.flux[j];
   return c;
 }
 
@@ -1227,7 +1222,7 @@ __device__ uint32_t run_select(DescRef D, float qx, float qy, float qz, bool par
     __syncthreads();
     for_candidates<false>(D, sh, lane, nr, *total, [&](uint32_t cnt) {
       for (uint32_t j = 0; j < cnt; ++j) {
-        const float4 p = sh.u.c.pos[j];
+        const float4 p = sh.pos[j];
         const float d2 = d2_ref(qx, qy, qz, p.x, p.y, p.z);
         const uint64_t key = make_key(d2, __float_as_uint(p.w));
         if (mine && key >= st.klo && key < st.khi) {
@@ -1570,7 +1565,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
               return flag;
             });
             drain(m, [&](uint32_t j) {
-              const float4 p = sh.u.c.pos[j];
+              const float4 p = sh.pos[j];
               const float d2 = d2_ref(qx, qy, qz, p.x, p.y, p.z);
               const uint32_t idx = __float_as_uint(p.w);
               const uint64_t key = make_key(d2, idx);
@@ -1578,7 +1573,7 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
                 if (n_written < static_cast<uint32_t>(D.k))
                   row[n_written++] = static_cast<int32_t>(idx);
               } else {
-                stash.put(sh, lane, key, kAccum ? sh.sb.chunk_slot[j] : 0u);
+                stash.put(sh, lane, key);
               }
             });
             // photons whose sign the shared normal left open (rare): each
@@ -1586,10 +1581,10 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
             while (kShared && amb != 0ull) {
               const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(amb));
               amb &= amb - 1ull;
-              const float4 p = sh.u.c.pos[j];
+              const float4 p = sh.pos[j];
               const float d2 = d2_ref(qx, qy, qz, p.x, p.y, p.z);
               if (acc_lane && make_key(d2, __float_as_uint(p.w)) < st.klo) {
-                const uint32_t s = sh.sb.chunk_slot[j];
+                const uint32_t s = sh.b.sb.chunk_slot[j];
                 const float4 w = D.wif4[s];
                 const float2 f = D.flux2[s];
                 if (!(dot_ref(w.x, w.y, w.z, nx, ny, nz) < 0.0f)) {
@@ -1628,12 +1623,12 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         uint64_t sk[kStashCap];  // the lane's stash; unused entries hold kKeyMax
 #pragma unroll
         for (int e = 0; e < kStashCap; ++e) {
-          sk[e] = static_cast<uint32_t>(e) < stash.n ? sh.sb.key[e * kWave + lane] : kKeyMax;
+          sk[e] = static_cast<uint32_t>(e) < stash.n ? sh.b.sb.key[e * kWave + lane] : kKeyMax;
         }
 #pragma unroll 1
         for (uint32_t i = 0; i < n_max; ++i) {
           // (past the lane's own entries this is a stale key; it is not used)
-          const uint64_t ki = sh.sb.key[i * kWave + lane];
+          const uint64_t ki = sh.b.sb.key[i * kWave + lane];
           uint32_t rank = 0;
 #pragma unroll
           for (int j = 0; j < kStashCap; ++j) rank += sk[j] < ki ? 1u : 0u;
@@ -1656,7 +1651,8 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
               wm[u] = make_float4(0.f, 0.f, 0.f, 0.f);
               fm[u] = make_float2(0.f, 0.f);
               if (im[u] < 32u) {
-                const int32_t sl = static_cast<int32_t>(sh.sb.slot[im[u] * kWave + lane]);
+                const uint32_t idx = static_cast<uint32_t>(sh.b.sb.key[im[u] * kWave + lane]);
+                const int32_t sl = D.inv_perm[idx];
                 wm[u] = D.wif4[sl];
                 fm[u] = D.flux2[sl];
               }
@@ -1742,11 +1738,11 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
             while (kSh && amb != 0ull) {
               const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(amb));
               amb &= amb - 1ull;
-              const float4 p = sh.u.c.pos[j];
+              const float4 p = sh.pos[j];
               const uint32_t db = __float_as_uint(d2_ref(qx, qy, qz, p.x, p.y, p.z));
               const bool nb = db < t_hi || (db == t_hi && __float_as_uint(p.w) <= t_lo);
               if (contributes && nb) {
-                const uint32_t sl = sh.sb.chunk_slot[j];
+                const uint32_t sl = sh.b.sb.chunk_slot[j];
                 const float4 w = D.wif4[sl];
                 const float2 f = D.flux2[sl];
                 if (!(dot_ref(w.x, w.y, w.z, nx, ny, nz) < 0.0f)) {
@@ -1928,14 +1924,16 @@ __device__ __forceinline__ int find_instance(const PM_CONST int64_t* prefix, int
   return uni(lo);
 }
 
-// Minimum resident waves per SIMD the register allocation must allow
-// (a tuning macro; 1 lets the compiler use up to 512 registers).
+// Resident waves per SIMD the register allocation of the selection kernels
+// must allow (a tuning macro): four, as many as their LDS allows.  The
+// adjoint kernel's larger LDS holds it at three, so it is left free.
 #ifndef PM_GATHER_MIN_WAVES
-#define PM_GATHER_MIN_WAVES 1
+#define PM_GATHER_MIN_WAVES 4
 #endif
 
 template <int MODE>
-__global__ __launch_bounds__(kWave, PM_GATHER_MIN_WAVES) void gather_kernel(GatherLaunch L) {
+__global__ __launch_bounds__(kWave, MODE == kModeAdjoint ? 1 : PM_GATHER_MIN_WAVES) void
+gather_kernel(GatherLaunch L) {
   __shared__ std::conditional_t<MODE == kModeAdjoint, AdjointShared, GatherShared> sh;
   const int lane = threadIdx.x;
   float* drho = nullptr;

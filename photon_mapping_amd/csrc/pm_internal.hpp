@@ -15,7 +15,7 @@ namespace pm {
 // ---- tuning constants ------------------------------------------------------
 constexpr int kWave = 64;           // CDNA wavefront width
 #ifndef PM_BINS
-#define PM_BINS 32
+#define PM_BINS 28
 #endif
 constexpr int kBins = PM_BINS;      // selection histogram bins per query
 #ifndef PM_STASH_CAP
@@ -221,7 +221,7 @@ struct PhotonMapImpl {
 struct Context {
   int device = 0;
   int n_cu = 256;
-  int waves_per_cu = 12;
+  int waves_per_cu = 16;
   int anchor_stride = kAnchorStride;
   int leaf_size = 8;
   int xcd_aware = 1;

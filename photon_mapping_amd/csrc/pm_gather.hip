@@ -300,9 +300,12 @@ __device__ __forceinline__ float sqrt_down(float x) {
   return x < FLT_MIN ? 0.0f : __builtin_amdgcn_sqrtf(x);
 }
 
+// (the hardware lane counts: lane is the wave lane, threadIdx.x of a one-wave
+// workgroup)
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask, int lane) {
-  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  return static_cast<uint32_t>(__popcll(mask & lt));
+  (void)lane;
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
 // Per-lane selection state.

@@ -454,9 +454,33 @@ __global__ void fill_f32_kernel(float* p, float v, int64_t n) {
   if (i < n) p[i] = v;
 }
 
+// out[c n + i] += acc[c n + inv_perm[i]] for the three channels: gradients
+// accumulated in slot order (the adjoint's atomics land on neighbouring
+// slots, a few cache lines per wave) are added to the caller's array in
+// original photon order.  Writes are coalesced; reads gather by slot.
+__global__ __launch_bounds__(kThreads) void slot_grad_add_kernel(
+    const float* __restrict__ acc, const int32_t* __restrict__ inv_perm, int64_t n,
+    float* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t sl = inv_perm[i];
+  out[i] += acc[sl];
+  out[n + i] += acc[n + sl];
+  out[2 * n + i] += acc[2 * n + sl];
+}
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace
+
+int launch_slot_grad_add(const float* acc, const int32_t* inv_perm, int64_t n, float* out,
+                         hipStream_t s) {
+  if (n <= 0) return PM_OK;
+  hipLaunchKernelGGL(slot_grad_add_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s,
+                     acc, inv_perm, n, out);
+  PM_HIP(hipGetLastError());
+  return PM_OK;
+}
 
 int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
   if (n <= 0) return PM_OK;

@@ -1006,7 +1006,12 @@ int pm_gather_adjoint(pm_photonmap* h, int64_t m, const pm_query* q, const pm_ma
   d.dl_drgb = dg;
   d.key_in = dkey;
   d.fg_mat = dfg;
-  d.dl_dflux = dflux;
+  // the kernel accumulates dL/dflux by slot (neighbouring photons share
+  // cache lines), then one pass adds it into the caller's photon order
+  if (int rc = ctx->ws_grad.reserve(3 * static_cast<size_t>(n) * sizeof(float))) return rc;
+  float* acc = ctx->ws_grad.as<float>();
+  if (int rc = launch_fill_f32(acc, 0.0f, 3 * n, s)) return rc;
+  d.dl_dflux = acc;
   d.dl_dalbedo = drho;
   d.m = m;
   d.k = k;
@@ -1014,6 +1019,7 @@ int pm_gather_adjoint(pm_photonmap* h, int64_t m, const pm_query* q, const pm_ma
   d.n_emit = static_cast<float>(n_emit);
   d.all_members = k_eff >= n ? 1 : 0;
   if (int rc = run_single(map, d, kModeAdjoint, s, nullptr)) return rc;
+  if (int rc = launch_slot_grad_add(acc, map->inv_perm.as<int32_t>(), n, dflux, s)) return rc;
   if (mem == PM_MEM_HOST) {
     PM_HIP(hipMemcpyAsync(dl_dflux, dflux, 3 * n * sizeof(float), hipMemcpyDeviceToHost, s));
     if (n_mats)

@@ -80,7 +80,7 @@ constexpr int kFront = 256;       // breadth-first frontier capacity
 constexpr int kNarrowTries = 12;  // lists tried when narrowing a loose sphere
 constexpr int kSparseRuns = 3;        // runs of a group at which runs are cut wider
 constexpr float kSparseSplit = 1.67f;  // ... by this factor on the run split
-constexpr int kDrhoSlots = 64;    // adjoint: materials accumulated in LDS
+constexpr int kDrhoSlots = 128;   // adjoint: materials accumulated in LDS
 constexpr int kCentreBins = 256;  // centre histogram bins
 constexpr int kSlotCap = PM_SLOT_CAP;  // photons of an explicit slot list
 // A candidate list is either nr subtrees (sh.rl.rnode / sh.rl.rpre) or, when nr
@@ -288,6 +288,38 @@ __device__ __forceinline__ float wave_sum(float v) {
   v = add(v, dpp_u32<0x142, 0xa>(0u, __float_as_uint(v)));
   v = add(v, dpp_u32<0x143, 0xc>(0u, __float_as_uint(v)));
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), kWave - 1));
+}
+
+// Adds a lane's (r, g, b) to the albedo gradient of material mat (nothing
+// for mat < 0).  Materials below kDrhoSlots accumulate in the block's LDS
+// copy.  The others go to global memory, summed first over the lanes that
+// share the material: many lanes of a wave usually share one (a large face,
+// or a light quad typed Lambert), and one address taking an atomic per lane
+// from every wave serialises them.  Call with every lane of the wave active.
+__device__ __forceinline__ void add_albedo_grad(float* lds, float* glob, int mat, float r,
+                                                float g, float b, int lane) {
+  if (mat >= 0 && mat < kDrhoSlots) {
+    atomicAdd(&lds[3 * mat + 0], r);
+    atomicAdd(&lds[3 * mat + 1], g);
+    atomicAdd(&lds[3 * mat + 2], b);
+  }
+  bool pending = mat >= kDrhoSlots;
+  uint64_t pm = __ballot(pending);
+  while (pm != 0ull) {
+    const int leader = static_cast<int>(__builtin_ctzll(pm));
+    const int m0 = __builtin_amdgcn_readlane(mat, leader);
+    const bool mine = pending && mat == m0;
+    const float sr = wave_sum(mine ? r : 0.0f);
+    const float sg = wave_sum(mine ? g : 0.0f);
+    const float sb = wave_sum(mine ? b : 0.0f);
+    if (lane == leader) {
+      atomicAdd(&glob[3 * m0 + 0], sr);
+      atomicAdd(&glob[3 * m0 + 1], sg);
+      atomicAdd(&glob[3 * m0 + 2], sb);
+    }
+    pending = pending && !mine;
+    pm = __ballot(pending);
+  }
 }
 
 // Square roots for radius bounds, from the hardware instruction (1 ulp);
@@ -1023,6 +1055,22 @@ __device__ __forceinline__ uint32_t run_list_cap(bool part, uint32_t k_eff) {
   return static_cast<uint32_t>(cap < 0xffffffffull ? cap : 0xffffffffull);
 }
 
+// Most photons the list of a run of several lanes may hold in the adjoint
+// with reused keys before the run is halved.  Both scatter passes cost about
+// (list photons) x (run lanes), and the run's sphere holds every lane's tau
+// ball: a lane with a wide ball next to a dense region (a caustic's edge)
+// makes the list many times what the other lanes need, while its halves
+// keep the wide ball to few lanes.  A list of a compact run holds about
+// 2 k_eff photons plus the slack of its leaves.
+#ifndef PM_REUSE_CAP_K
+#define PM_REUSE_CAP_K 4
+#endif
+__device__ __forceinline__ uint32_t reuse_list_cap(uint32_t k_eff) {
+  if (PM_REUSE_CAP_K <= 0) return 0xffffffffu;
+  const uint64_t cap = static_cast<uint64_t>(PM_REUSE_CAP_K) * k_eff + 256ull;
+  return static_cast<uint32_t>(cap < 0xffffffffull ? cap : 0xffffffffull);
+}
+
 // Selection for the lanes of one run (part), with centre c, lane distances
 // dq >= |q - c| and their maximum dmax.  r0sq is the lane's guaranteed bound
 // on its k-th FP32 d^2.  Afterwards the neighbour set of a part lane is
@@ -1461,14 +1509,17 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         const float rs = wave_max(xs) * 1.0002f;
         const float rs2 = rs * rs + 1e-30f;
         bool fine = false;
+        const bool several = c1 - c0 > 1;
         nr = build_list(D, cx, cy, cz, rs2, sh, lane, &total,
                         count_stats ? &counters[kStatOverflow] : nullptr, true, &fine,
-                        c1 - c0 > 1 ? kListGiveUp : kListCoarsen);
+                        several ? kListGiveUp : kListCoarsen,
+                        several ? reuse_list_cap(k_eff) : 0xffffffffu);
         if (nr == kNoList) {
           if (count_stats && lane == 0) atomicAdd(&counters[kStatSplit], 1u);
           starts |= 1ull << ((c0 + c1) >> 1);
           continue;
         }
+        if (count_stats && lane == 0) atomicAdd(&counters[kStatCand1], total);
         // both scatter passes read the list: keep exactly the photons of
         // the run's sphere when they fit the slot list
         const uint32_t ns = collect_slots(D, sh, lane, nr, total, fine, cx, cy, cz,
@@ -1759,34 +1810,20 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         };
         if (shared) pass1(std::true_type{});
         else pass1(std::false_type{});
+        float dr = 0.f, dg = 0.f, db = 0.f, fr = 0.f, fgv = 0.f, fb = 0.f;
         if (contributes) {
           const float inv = 1.0f / (kPiF * denom);
-          const float dr = ge_r * s_r * inv, dg = ge_g * s_g * inv, db = ge_b * s_b * inv;
-          if (mat < kDrhoSlots) {
-            atomicAdd(&drho[3 * mat + 0], dr);
-            atomicAdd(&drho[3 * mat + 1], dg);
-            atomicAdd(&drho[3 * mat + 2], db);
-          } else {
-            atomicAdd(&D.dl_dalbedo[3 * mat + 0], dr);
-            atomicAdd(&D.dl_dalbedo[3 * mat + 1], dg);
-            atomicAdd(&D.dl_dalbedo[3 * mat + 2], db);
-          }
+          dr = ge_r * s_r * inv; dg = ge_g * s_g * inv; db = ge_b * s_b * inv;
           if (fgm >= 0) {
             // forward value of the estimate L = (rho/pi) * S / denom
             const float lr = ((rho_r / kPiF) * s_r) / denom;
             const float lgv = ((rho_g / kPiF) * s_g) / denom;
             const float lb = ((rho_b / kPiF) * s_b) / denom;
-            if (fgm < kDrhoSlots) {
-              atomicAdd(&drho[3 * fgm + 0], g_r * lr);
-              atomicAdd(&drho[3 * fgm + 1], g_g * lgv);
-              atomicAdd(&drho[3 * fgm + 2], g_b * lb);
-            } else {
-              atomicAdd(&D.dl_dalbedo[3 * fgm + 0], g_r * lr);
-              atomicAdd(&D.dl_dalbedo[3 * fgm + 1], g_g * lgv);
-              atomicAdd(&D.dl_dalbedo[3 * fgm + 2], g_b * lb);
-            }
+            fr = g_r * lr; fgv = g_g * lgv; fb = g_b * lb;
           }
         }
+        add_albedo_grad(drho, D.dl_dalbedo, contributes ? mat : -1, dr, dg, db, lane);
+        add_albedo_grad(drho, D.dl_dalbedo, contributes ? fgm : -1, fr, fgv, fb, lane);
 
         // (2) per photon, one lane each: sum of the coefficients of the
         // queries whose set holds it (the same FP32 key test as above), then
@@ -1827,8 +1864,9 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
             const uint32_t t = base + static_cast<uint32_t>(lane);
             const bool ok = t < total;
             float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p;
+            int32_t slot = 0;
             if (ok) {
-              const int32_t slot = list_slot(D, sh, nr, t, r);
+              slot = list_slot(D, sh, nr, t, r);
               p = P[slot];
               w = W[slot];
             }
@@ -1867,9 +1905,9 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
             if (__ballot(ok && per_query) != 0ull) queries(std::true_type{});
             else if (__ballot(ok && inc) != 0ull) queries(std::false_type{});
             if (ok && (a_r != 0.0f || a_g != 0.0f || a_b != 0.0f)) {
-              atomicAdd(&dflux[idx], a_r);
-              atomicAdd(&dflux[n + idx], a_g);
-              atomicAdd(&dflux[2 * n + idx], a_b);
+              atomicAdd(&dflux[slot], a_r);
+              atomicAdd(&dflux[n + slot], a_g);
+              atomicAdd(&dflux[2 * n + slot], a_b);
             }
           }
         }

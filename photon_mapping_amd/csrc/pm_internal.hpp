@@ -138,7 +138,7 @@ struct alignas(16) GatherDesc {
   const float* dl_drgb;
   const uint64_t* key_in;
   const int32_t* fg_mat;
-  float* dl_dflux;
+  float* dl_dflux;         // ADJOINT: 3*n gradients in slot order (accumulated)
   float* dl_dalbedo;
   // sizes / parameters
   int64_t m;               // queries of this instance
@@ -235,6 +235,7 @@ struct Context {
   DevBuf ws_slots;
   DevBuf ws_stage_in, ws_stage_out, ws_stage_aux;
   DevBuf ws_tmp;
+  DevBuf ws_grad;  // adjoint: per-photon gradients in slot order (3 n floats)
   // descriptor staging: pinned host arena mirrored into a device arena;
   // `stage_done` is recorded after each upload so the host side is only
   // rewritten once the previous copy has completed.
@@ -342,5 +343,9 @@ int launch_knn_rowsort(const GatherDesc* dev_desc, int64_t m, int k, int32_t* ou
                        DevBuf* scratch, hipStream_t s);
 int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 int launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
+// out[c n + i] += acc[c n + inv_perm[i]], c = 0, 1, 2 (slot-order gradients
+// into original photon order).
+int launch_slot_grad_add(const float* acc, const int32_t* inv_perm, int64_t n, float* out,
+                         hipStream_t s);
 
 }  // namespace pm

@@ -138,3 +138,41 @@ def test_adjoint_all_photons(gpu, oracle, rng):
         dflux, drho = pmap.gather_adjoint(q, mats, 100, 50.0, g, key=kk)
         np.testing.assert_allclose(dflux, ef, rtol=1e-4, atol=1e-6 * np.abs(ef).max())
         np.testing.assert_allclose(drho, er, rtol=1e-4, atol=1e-6 * np.abs(er).max())
+
+
+def test_adjoint_many_materials(gpu, oracle, rng):
+    """More materials than the kernel accumulates in LDS (128): the gradients
+    of the others are summed over the lanes of a wave that share a material,
+    then added with one global atomic per material.  Half the queries and
+    half the final-gathering materials sit on two such materials (as queries
+    on a light quad typed Lambert crowd one index); the rest spread over all
+    200, in both modes."""
+    from photon_mapping_amd.photon_map import make_materials
+
+    pos, wi, flux, nrm = photons_surfaces(rng, 20000)
+    m, n_mats = 3000, 200
+    sel = rng.integers(0, len(pos), m)
+    qpos = pos[sel] + rng.normal(scale=0.002, size=(m, 3)).astype(np.float32)
+    q = query_records(rng, qpos, nrm[sel], n_mats=n_mats)
+    hot = rng.random(m) < 0.5
+    q.view(np.int32)[hot, 9] = rng.choice([130, 199], int(hot.sum()))
+    fg = rng.integers(0, n_mats, m).astype(np.int32)
+    fg[rng.random(m) < 0.5] = 170
+    mats = make_materials(rng.uniform(0.1, 0.9, size=(n_mats, 3)))
+    pmap = gpu.PhotonMap()
+    pmap.set_photons_soa(pos, wi, flux)
+    pmap.build()
+    k, n_emit = 32, 3e4
+    g = rng.normal(size=(m, 3)).astype(np.float32)
+    print("adjoint, 200 materials", flush=True)
+    _, _, key = pmap.gather_radiance(q, mats, k, n_emit)
+    bidx, br2 = oracle.brute_knn(pos, q[:, :3], k)
+    rho_fg = mats[fg, :3]
+    ef, er = oracle.adjoint_fp64(wi, flux, q, mats, bidx, br2, n_emit, g * rho_fg)
+    L = oracle.radiance_fp64(wi, flux, q, mats, bidx, br2, n_emit)
+    np.add.at(er, fg, g * L)
+    assert np.abs(er[128:]).max() > 0  # the global path carries gradient
+    for kk in (key, None):
+        dflux, drho = pmap.gather_adjoint(q, mats, k, n_emit, g, key=kk, fg_mat=fg)
+        np.testing.assert_allclose(dflux, ef, rtol=1e-4, atol=1e-5 * np.abs(ef).max())
+        np.testing.assert_allclose(drho, er, rtol=1e-4, atol=1e-6 * np.abs(er).max())

@@ -16,7 +16,7 @@
 //           the lane's own anchor bound), and all of them within the largest
 //           R_q + |q - c| of c: that sphere's photons are the run's candidate
 //           list (a few hundred at leaf size 8);
-//   pass A  per lane, a histogram of d^2 over [0, R_q^2] in 32 linear bins
+//   pass A  per lane, a histogram of d^2 over [0, R_q^2] in kBins linear bins
 //           (LDS) finds the bin holding the k-th key, i.e. a key window;
 //           the list then shrinks to the sphere that holds every window;
 //   pass B  photons below the window are neighbours and are consumed on the

@@ -152,22 +152,27 @@ __global__ __launch_bounds__(kSortThreads) void scan_down_kernel(
   }
 }
 
-// Scatter of one digit pass.  The tile's elements are first ranked (stable:
-// wave ballots find equal-digit peers, per-wave digit counts in LDS order
-// the waves) and staged in LDS in digit order; the staged tile is then
-// written out with consecutive lanes on consecutive elements, so each digit's
-// run of the tile lands in consecutive addresses instead of one scattered
-// element per lane.
+// Scatter of one digit pass.  Wave w ranks the consecutive elements
+// [w * 1024, (w + 1) * 1024) of the tile, 64 per round: wave ballots find
+// each lane's equal-digit peers, and the wave's own running count of every
+// digit (LDS, touched by no other wave) places them after the wave's earlier
+// rounds.  One barrier later, each digit's counts over the waves give the
+// waves' offsets within the digit and the digits' starts within the tile,
+// and the tile is staged in LDS in digit order (stable: element order is
+// wave, round, lane).  The staged tile is then written out with consecutive
+// lanes on consecutive elements, so each digit's run of the tile lands in
+// consecutive addresses instead of one scattered element per lane.
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift,
     const uint32_t* __restrict__ offsets, int64_t n_tiles, const SortTile* __restrict__ tiles) {
-  __shared__ uint32_t glob[256];   // first output position of each digit (this tile)
-  __shared__ uint32_t loc[256];    // first staged position of each digit
-  __shared__ uint32_t seen[256];   // elements of each digit staged so far
-  __shared__ uint32_t wcnt[kSortThreads / 64][256];
-  __shared__ uint32_t wsum[kSortThreads / 64];
+  constexpr int kWaves = kSortThreads / 64;
+  constexpr int kWaveElems = kSortItems * 64;
+  __shared__ uint32_t glob[256];            // first output position of each digit (this tile)
+  __shared__ uint32_t loc[256];             // first staged position of each digit
+  __shared__ uint32_t wcnt[kWaves][256];    // a wave's running digit counts, then its offsets
+  __shared__ uint32_t wsum[kWaves];
   __shared__ K skey[kSortTile];
   __shared__ uint32_t sval[kSortTile];
   const int t = threadIdx.x;
@@ -176,41 +181,35 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   const TileView tv = tile_view(tiles, n, n_tiles);
   const int64_t base = tv.base;
   const int tile_n = tv.n;
-  // this tile's count of digit t: the difference of consecutive entries of
-  // the exclusive scan of the count table (the last entry closes at n)
-  const int64_t at = static_cast<int64_t>(tv.cbase) + static_cast<int64_t>(t) * tv.stride;
-  const uint32_t g = offsets[at];
-  const uint32_t next = (at + 1 < 256 * n_tiles) ? offsets[at + 1] : static_cast<uint32_t>(n);
-  uint32_t tot;
-  const uint32_t l0 = block_exclusive_scan(next - g, wsum, &tot);
-  glob[t] = g;
-  loc[t] = l0;
-  seen[t] = 0;
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // the tile's output start of digit t, from the exclusive scan of the
+  // count table
+  glob[t] = offsets[static_cast<int64_t>(tv.cbase) + static_cast<int64_t>(t) * tv.stride];
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) wcnt[i][t] = 0;
   // every element of the thread is loaded up front, so the tile's loads are
   // in flight together rather than one round at a time
   K keys_r[kSortItems];
   uint32_t vals_r[kSortItems];
+  uint32_t pos_r[kSortItems];  // place among the wave's elements of its digit
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int li = r * kSortThreads + t;
+    const int li = w * kWaveElems + r * 64 + lane;
     keys_r[r] = 0;
     vals_r[r] = 0;
+    pos_r[r] = 0;
     if (li < tile_n) {
       keys_r[r] = kin[base + li];
       vals_r[r] = vin[base + li];
     }
   }
+  __syncthreads();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t* mine = wcnt[w];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-#pragma unroll
-    for (int i = 0; i < kSortThreads / 64; ++i) wcnt[i][t] = 0;
-    __syncthreads();
-    const int li = r * kSortThreads + t;
+    const int li = w * kWaveElems + r * 64 + lane;
     const bool valid = li < tile_n;
-    const K key = keys_r[r];
-    const uint32_t val = vals_r[r];
-    const uint32_t d = valid ? static_cast<uint32_t>(key >> shift) & 255u : 0u;
+    const uint32_t d = valid ? static_cast<uint32_t>(keys_r[r] >> shift) & 255u : 0u;
     const uint64_t vmask = __ballot(valid);
     uint64_t peers = valid ? vmask : ~vmask;
 #pragma unroll
@@ -220,20 +219,33 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
       peers &= bit ? m : ~m;
     }
     const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt_mask));
-    if (valid && rank == 0) wcnt[w][d] = static_cast<uint32_t>(__popcll(peers));
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = loc[d] + seen[d] + rank;
-      for (int i = 0; i < w; ++i) pos += wcnt[i][d];
-      skey[pos] = key;
-      sval[pos] = val;
-    }
-    __syncthreads();
-    {
-      uint32_t add = 0;
+    // every lane reads its digit's count before the group's first lane
+    // (one instruction later) advances it
+    const uint32_t before = valid ? mine[d] : 0u;
+    pos_r[r] = before + rank;
+    if (valid && rank == 0) mine[d] = before + static_cast<uint32_t>(__popcll(peers));
+  }
+  __syncthreads();
+  // digit t: the waves' exclusive offsets (in place) and the tile's count
+  uint32_t cnt = 0;
 #pragma unroll
-      for (int i = 0; i < kSortThreads / 64; ++i) add += wcnt[i][t];
-      seen[t] += add;
+  for (int i = 0; i < kWaves; ++i) {
+    const uint32_t c = wcnt[i][t];
+    wcnt[i][t] = cnt;
+    cnt += c;
+  }
+  uint32_t tot;
+  const uint32_t l0 = block_exclusive_scan(cnt, wsum, &tot);
+  loc[t] = l0;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int li = w * kWaveElems + r * 64 + lane;
+    if (li < tile_n) {
+      const uint32_t d = static_cast<uint32_t>(keys_r[r] >> shift) & 255u;
+      const uint32_t p = loc[d] + wcnt[w][d] + pos_r[r];
+      skey[p] = keys_r[r];
+      sval[p] = vals_r[r];
     }
   }
   __syncthreads();

@@ -152,6 +152,38 @@ __global__ __launch_bounds__(kSortThreads) void scan_down_kernel(
   }
 }
 
+// Exclusive scan of one digit's row of a uniform (digit-major) count table
+// in place, one block per digit: the row's entries become each tile's count
+// of the digit in the tiles before it, and totals[digit] the digit's count
+// over every tile.  The scatter adds the digits' starts, the exclusive scan of
+// the totals, itself.
+__global__ __launch_bounds__(kSortThreads) void digit_scan_kernel(uint32_t* __restrict__ counts,
+                                                                  int64_t n_tiles,
+                                                                  uint32_t* __restrict__ totals) {
+  __shared__ uint32_t wsum[kSortThreads / 64];
+  uint32_t* row = counts + static_cast<int64_t>(blockIdx.x) * n_tiles;
+  uint32_t carry = 0;
+  for (int64_t b = 0; b < n_tiles; b += kScanChunk) {
+    const int64_t i0 = b + static_cast<int64_t>(threadIdx.x) * kSortItems;
+    uint32_t v[kSortItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+      v[r] = i0 + r < n_tiles ? row[i0 + r] : 0u;
+      sum += v[r];
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(sum, wsum, &tot) + carry;
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+      if (i0 + r < n_tiles) row[i0 + r] = run;
+      run += v[r];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
 // Scatter of one digit pass.  Wave w ranks the consecutive elements
 // [w * 1024, (w + 1) * 1024) of the tile, 64 per round: wave ballots find
 // each lane's equal-digit peers, and the wave's own running count of every
@@ -166,7 +198,8 @@ template <typename K>
 __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift,
-    const uint32_t* __restrict__ offsets, int64_t n_tiles, const SortTile* __restrict__ tiles) {
+    const uint32_t* __restrict__ offsets, int64_t n_tiles, const SortTile* __restrict__ tiles,
+    const uint32_t* __restrict__ totals) {
   constexpr int kWaves = kSortThreads / 64;
   constexpr int kWaveElems = kSortItems * 64;
   __shared__ uint32_t glob[256];            // first output position of each digit (this tile)
@@ -182,8 +215,13 @@ __global__ __launch_bounds__(kSortThreads) void digit_scatter_kernel(
   const int64_t base = tv.base;
   const int tile_n = tv.n;
   // the tile's output start of digit t, from the exclusive scan of the
-  // count table
-  glob[t] = offsets[static_cast<int64_t>(tv.cbase) + static_cast<int64_t>(t) * tv.stride];
+  // count table (per digit row, plus the digit's start, when totals are given)
+  uint32_t g = offsets[static_cast<int64_t>(tv.cbase) + static_cast<int64_t>(t) * tv.stride];
+  if (totals) {
+    uint32_t all;
+    g += block_exclusive_scan(totals[t], wsum, &all);
+  }
+  glob[t] = g;
 #pragma unroll
   for (int i = 0; i < kWaves; ++i) wcnt[i][t] = 0;
   // every element of the thread is loaded up front, so the tile's loads are
@@ -286,10 +324,19 @@ int radix_sort_impl(uint32_t* counts, uint32_t* partials, K* keys, uint32_t* val
   for (int shift = lo_bit; shift < bits; shift += 8) {
     hipLaunchKernelGGL(digit_hist_kernel<K>, dim3(n_tiles), dim3(kSortThreads), 0, s,
                        kb[cur], n, shift, counts, n_tiles, tiles);
-    if (int rc = exclusive_scan_impl(partials, counts, n_tiles * 256, s)) return rc;
+    // uniform tiles: one row scan per digit (the digits' starts are added by
+    // the scatter); a segmented table: the scan over the whole table
+    const uint32_t* totals = nullptr;
+    if (!tiles) {
+      hipLaunchKernelGGL(digit_scan_kernel, dim3(256), dim3(kSortThreads), 0, s, counts, n_tiles,
+                         partials);
+      totals = partials;
+    } else if (int rc = exclusive_scan_impl(partials, counts, n_tiles * 256, s)) {
+      return rc;
+    }
     hipLaunchKernelGGL(digit_scatter_kernel<K>, dim3(n_tiles), dim3(kSortThreads), 0, s,
                        kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, shift, counts,
-                       n_tiles, tiles);
+                       n_tiles, tiles, totals);
     PM_HIP(hipGetLastError());
     cur ^= 1;
   }
@@ -314,7 +361,8 @@ int64_t sort_count_words(int64_t n) {
 }
 
 int64_t sort_partial_words(int64_t n) {
-  return (sort_count_words(n) + kScanChunk - 1) / kScanChunk + 1;
+  // the scan partials of the table, and at least the 256 digit totals
+  return std::max<int64_t>((sort_count_words(n) + kScanChunk - 1) / kScanChunk + 1, 256);
 }
 
 int64_t sort_seg_tiles(int64_t n, int n_seg) {

@@ -263,8 +263,10 @@ int pm_plan_destroy(pm_plan* plan);
  * Times pm_gather_radiance on the caller's workload (device pointers) under a
  * small grid of settings, coordinate by coordinate: leaf size {8, 16, 32}
  * (the map is rebuilt), query-run split factor {0.5 .. 1.25} and persistent
- * waves per CU {8, 12, 16, 20}.  The fastest setting stays in force (map leaf
- * size, context options); results do not depend on it.  Synchronises. */
+ * waves per CU {8, 12, 16, 20}, each timed as the median of five calls, the
+ * value in force timed again in each sweep beside the candidates.  The fastest
+ * setting stays in force (map leaf size, context options); results do not
+ * depend on it.  Synchronises. */
 typedef struct pm_tune_report {
   int32_t leaf_size;    /* chosen leaf size                                  */
   int32_t waves_per_cu; /* chosen persistent waves per CU                    */

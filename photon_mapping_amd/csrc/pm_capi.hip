@@ -1097,16 +1097,17 @@ int pm_autotune(pm_photonmap* h, int64_t m, const pm_query* q, const pm_material
   ctx->timing = 1;
   ctx->count_stats = 0;
   int tried = 0;
-  // best of two timed calls after one warm-up (anchors, workspaces)
+  // median of five timed calls after one warm-up (anchors, workspaces)
   auto time_it = [&](double* ms) -> int {
-    double best = 1e30;
-    for (int rep = 0; rep < 3; ++rep) {
+    double t[5];
+    for (int rep = 0; rep < 6; ++rep) {
       if (int rc = pm_gather_radiance(h, m, q, mats, n_mats, k, n_emit, rgb.as<float>(), nullptr,
                                       nullptr, PM_MEM_DEVICE, stream))
         return rc;
-      if (rep > 0) best = std::min(best, ctx->stats.sort_ms + ctx->stats.gather_ms);
+      if (rep > 0) t[rep - 1] = ctx->stats.sort_ms + ctx->stats.gather_ms;
     }
-    *ms = best;
+    std::sort(t, t + 5);
+    *ms = t[2];
     ++tried;
     return PM_OK;
   };
@@ -1117,35 +1118,54 @@ int pm_autotune(pm_photonmap* h, int64_t m, const pm_query* q, const pm_material
   };
   double initial = 0.0;
   if (int rc = time_it(&initial)) return restore(rc);
-  double best = initial;
+  // Each coordinate's sweep times the value in force again beside the
+  // candidates, so a candidate must beat it under the same conditions.
+  double last = initial;
   int best_leaf = map->leaf_size;
-  for (int leaf : {8, 16, 32}) {
-    if (leaf == best_leaf) continue;
-    if (int rc = pm_photonmap_build(h, leaf, stream)) return restore(rc);
-    double t = 0.0;
-    if (int rc = time_it(&t)) return restore(rc);
-    if (t < best) { best = t; best_leaf = leaf; }
+  {
+    double lb = 1e30;
+    const int cur = map->leaf_size;
+    for (int leaf : {cur, 8, 16, 32}) {
+      if (leaf == cur && lb < 1e30) continue;
+      if (map->leaf_size != leaf)
+        if (int rc = pm_photonmap_build(h, leaf, stream)) return restore(rc);
+      double t = 0.0;
+      if (int rc = time_it(&t)) return restore(rc);
+      if (t < lb) { lb = t; best_leaf = leaf; }
+    }
+    if (map->leaf_size != best_leaf)
+      if (int rc = pm_photonmap_build(h, best_leaf, stream)) return restore(rc);
+    last = lb;
   }
-  if (map->leaf_size != best_leaf)
-    if (int rc = pm_photonmap_build(h, best_leaf, stream)) return restore(rc);
   float best_split = ctx->run_split;
-  for (float sp : {0.5f, 0.6f, 0.75f, 0.9f, 1.0f, 1.25f}) {
-    if (sp == best_split) continue;
-    ctx->run_split = sp;
-    double t = 0.0;
-    if (int rc = time_it(&t)) return restore(rc);
-    if (t < best) { best = t; best_split = sp; }
+  {
+    double lb = 1e30;
+    const float cur = ctx->run_split;
+    for (float sp : {cur, 0.5f, 0.6f, 0.75f, 0.9f, 1.0f, 1.25f}) {
+      if (sp == cur && lb < 1e30) continue;
+      ctx->run_split = sp;
+      double t = 0.0;
+      if (int rc = time_it(&t)) return restore(rc);
+      if (t < lb) { lb = t; best_split = sp; }
+    }
+    ctx->run_split = best_split;
+    last = lb;
   }
-  ctx->run_split = best_split;
   int best_waves = ctx->waves_per_cu;
-  for (int w : {8, 12, 16, 20}) {
-    if (w == best_waves) continue;
-    ctx->waves_per_cu = w;
-    double t = 0.0;
-    if (int rc = time_it(&t)) return restore(rc);
-    if (t < best) { best = t; best_waves = w; }
+  {
+    double lb = 1e30;
+    const int cur = ctx->waves_per_cu;
+    for (int w : {cur, 8, 12, 16, 20}) {
+      if (w == cur && lb < 1e30) continue;
+      ctx->waves_per_cu = w;
+      double t = 0.0;
+      if (int rc = time_it(&t)) return restore(rc);
+      if (t < lb) { lb = t; best_waves = w; }
+    }
+    ctx->waves_per_cu = best_waves;
+    last = lb;
   }
-  ctx->waves_per_cu = best_waves;
+  const double best = std::min(initial, last);
   PM_HIP(hipStreamSynchronize(s));
   out->leaf_size = best_leaf;
   out->waves_per_cu = best_waves;

@@ -646,6 +646,22 @@ int64_t batch_pixels_of(const pmx_render_params* p) {
 namespace pm {
 namespace {
 
+// Grows buf to at least `need` bytes (by half again at least), keeping its
+// first `keep` bytes.  The copy runs on the null stream, which the render
+// uses throughout, and completes before the old storage is freed.
+int grow_keep(DevBuf& buf, size_t need, size_t keep) {
+  if (need <= buf.bytes) return PM_OK;
+  DevBuf nb;
+  if (int rc = nb.reserve(std::max(need, buf.bytes + buf.bytes / 2))) return rc;
+  if (keep > 0) {
+    PM_HIP(hipMemcpyAsync(nb.ptr, buf.ptr, keep, hipMemcpyDeviceToDevice, nullptr));
+    PM_HIP(hipStreamSynchronize(nullptr));
+  }
+  std::swap(buf.ptr, nb.ptr);
+  std::swap(buf.bytes, nb.bytes);
+  return PM_OK;  // nb frees the old storage
+}
+
 int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* global_map,
               pm_photonmap* caustics_map, const pmx_render_params* p, const float* in_rays,
               float* out_mean, float* out_var, pmx_render_stats* stats) {
@@ -681,20 +697,29 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
   if (!rc) rc = seg[0].reserve(static_cast<size_t>(bsamples) * kSegWords * sizeof(uint32_t));
   if (rc) return rc;
   // A round of n_in segments stages at most 2 n_in segments and n_in queries
-  // per map (fixed slots), then compacts them.
+  // per map (fixed slots), then compacts them: the segments into the next
+  // round's list, the queries and their weights onto the end of the batch's
+  // list of each map (used[map] entries so far).  Each map's list is
+  // gathered once, when the batch's rounds are done: one sort and one
+  // persistent launch per map and batch instead of one per round (the late
+  // rounds hold few queries, and a launch of few groups is mostly its slowest
+  // waves).  The accumulators are fixed point, so the order in which the
+  // terms land does not matter.
+  size_t used[2] = {0, 0};
   const auto reserve_round = [&](size_t n_in, DevBuf& next) -> int {
     int e = stage.reserve(2 * n_in * kSegWords * sizeof(uint32_t));
     if (!e) e = next.reserve(2 * n_in * kSegWords * sizeof(uint32_t));
     if (!e) e = qg_s.reserve(n_in * sizeof(pm_query));
     if (!e) e = wg_s.reserve(n_in * sizeof(Weighted));
-    if (!e) e = qg.reserve(n_in * sizeof(pm_query));
-    if (!e) e = wg.reserve(n_in * sizeof(Weighted));
+    // the batch's query lists grow by at most n_in entries each
+    if (!e) e = grow_keep(qg, (used[0] + n_in) * sizeof(pm_query), used[0] * sizeof(pm_query));
+    if (!e) e = grow_keep(wg, (used[0] + n_in) * sizeof(Weighted), used[0] * sizeof(Weighted));
     if (!e && R.caustics) e = qc_s.reserve(n_in * sizeof(pm_query));
     if (!e && R.caustics) e = wc_s.reserve(n_in * sizeof(Weighted));
-    if (!e && R.caustics) e = qc.reserve(n_in * sizeof(pm_query));
-    if (!e && R.caustics) e = wc.reserve(n_in * sizeof(Weighted));
-    if (!e) e = rgb.reserve(n_in * 3 * sizeof(float));
-    if (!e) e = r2.reserve(n_in * sizeof(float));
+    if (!e && R.caustics)
+      e = grow_keep(qc, (used[1] + n_in) * sizeof(pm_query), used[1] * sizeof(pm_query));
+    if (!e && R.caustics)
+      e = grow_keep(wc, (used[1] + n_in) * sizeof(Weighted), used[1] * sizeof(Weighted));
     if (!e) e = flags.reserve(n_in * sizeof(uint32_t));
     if (!e) e = c_seg.reserve(n_in * sizeof(uint32_t));
     if (!e) e = c_qg.reserve(n_in * sizeof(uint32_t));
@@ -752,34 +777,43 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
                          c32);
       hipLaunchKernelGGL(compact_kernel, grid, dim3(kThreads), 0, nullptr, flags.as<uint32_t>(),
                          n_cur, c_seg.as<uint32_t>(), c_qg.as<uint32_t>(), c_qc.as<uint32_t>(),
-                         stg, nxt, qg_s.as<pm_query>(), wg_s.as<Weighted>(), qg.as<pm_query>(),
-                         wg.as<Weighted>(), qc_s.as<pm_query>(), wc_s.as<Weighted>(),
-                         qc.as<pm_query>(), wc.as<Weighted>());
+                         stg, nxt, qg_s.as<pm_query>(), wg_s.as<Weighted>(),
+                         qg.as<pm_query>() + used[0], wg.as<Weighted>() + used[0],
+                         qc_s.as<pm_query>(), wc_s.as<Weighted>(),
+                         R.caustics ? qc.as<pm_query>() + used[1] : nullptr,
+                         R.caustics ? wc.as<Weighted>() + used[1] : nullptr);
       PM_HIP(hipGetLastError());
       uint32_t cnt[3];
       PM_HIP(hipMemcpy(cnt, c32, sizeof(cnt), hipMemcpyDeviceToHost));
       ++st.iterations;
-      // The lists are in segment order, which follows from the camera order
-      // through every compaction, so the estimates do not depend on scheduling.
-      for (int map_i = 0; map_i < 2; ++map_i) {
-        const uint32_t nq = cnt[1 + map_i];
-        if (nq == 0) continue;
-        pm_photonmap* map = map_i == 0 ? global_map : caustics_map;
-        const double n_emit = map_i == 0 ? p->n_emit_global : p->n_emit_caustics;
-        const pm_query* q = map_i == 0 ? qg.as<pm_query>() : qc.as<pm_query>();
-        const Weighted* wq = map_i == 0 ? wg.as<Weighted>() : wc.as<Weighted>();
-        if (int e = pm_gather_radiance(map, nq, q, s->mats.as<pm_material>(), s->n_mat, p->k,
-                                       n_emit, rgb.as<float>(), r2.as<float>(), nullptr,
-                                       PM_MEM_DEVICE, nullptr))
-          return e;
-        hipLaunchKernelGGL(scatter_kernel, dim3(grid_of(nq)), dim3(kThreads), 0, nullptr, wq,
-                           rgb.as<float>(), nq, acc.as<unsigned long long>(),
-                           bad.as<uint32_t>());
-        PM_HIP(hipGetLastError());
-        (map_i == 0 ? st.gathers_global : st.gathers_caustics) += nq;
-      }
+      used[0] += cnt[1];
+      used[1] += cnt[2];
       cur_i ^= 1;
       n_cur = cnt[0];
+    }
+    // The lists are in round order and, within a round, in segment order,
+    // which follows from the camera order through every compaction, so the
+    // estimates do not depend on scheduling.
+    for (int map_i = 0; map_i < 2; ++map_i) {
+      const size_t nq = used[map_i];
+      used[map_i] = 0;
+      if (nq == 0) continue;
+      if (nq > 0xffffffffull) return fail(PM_ERR_UNSUPPORTED, "render batch gathers too large");
+      pm_photonmap* map = map_i == 0 ? global_map : caustics_map;
+      const double n_emit = map_i == 0 ? p->n_emit_global : p->n_emit_caustics;
+      const pm_query* q = map_i == 0 ? qg.as<pm_query>() : qc.as<pm_query>();
+      const Weighted* wq = map_i == 0 ? wg.as<Weighted>() : wc.as<Weighted>();
+      if (int e = rgb.reserve(nq * 3 * sizeof(float))) return e;
+      if (int e = r2.reserve(nq * sizeof(float))) return e;
+      if (int e = pm_gather_radiance(map, static_cast<int64_t>(nq), q, s->mats.as<pm_material>(),
+                                     s->n_mat, p->k, n_emit, rgb.as<float>(), r2.as<float>(),
+                                     nullptr, PM_MEM_DEVICE, nullptr))
+        return e;
+      hipLaunchKernelGGL(scatter_kernel, dim3(grid_of(nq)), dim3(kThreads), 0, nullptr, wq,
+                         rgb.as<float>(), static_cast<uint32_t>(nq), acc.as<unsigned long long>(),
+                         bad.as<uint32_t>());
+      PM_HIP(hipGetLastError());
+      (map_i == 0 ? st.gathers_global : st.gathers_caustics) += static_cast<int64_t>(nq);
     }
     hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(np)), dim3(kThreads), 0, nullptr,
                        acc.as<unsigned long long>(), bad.as<uint32_t>(), p->spp,

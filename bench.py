@@ -294,7 +294,7 @@ def plain_config_bench(ctx, with_cpu, budget_s=10.0):
     with sc.NativeScene(s, ctx) as ns:
         gmap, n_stored = sc.trace_photon_map(ns, n_paths, seed=0)
         gmap.build()
-        sc.render_photon_mapping(ns, gmap, n_paths, 32, 32, 2, k=k)   # warm
+        sc.render_photon_mapping(ns, gmap, n_paths, w, h, spp, k=k, seed=0)  # warm, same size
         torch.cuda.synchronize()
         img, _, st = sc.render_photon_mapping(ns, gmap, n_paths, w, h, spp, k=k, seed=0)
         torch.cuda.synchronize()
@@ -431,7 +431,7 @@ def water_diff_bench(ctx, dev, k=100, reps=3, frame=True):
     if frame:
         kw = dict(k=k, caustics_map=cmap, n_emit_caustics=float(n_cpaths),
                   final_gathering_depth=1, seed=0)
-        sc.render_photon_mapping(ns, pmap, float(n_emit), 64, 64, 2, **kw)  # warm
+        sc.render_photon_mapping(ns, pmap, float(n_emit), 512, 512, 64, **kw)  # warm, same size
         torch.cuda.synchronize()
         img, _, st = sc.render_photon_mapping(ns, pmap, float(n_emit), 512, 512, 64, **kw)
         torch.cuda.synchronize()
@@ -474,7 +474,8 @@ def caustics_frame_bench(ctx, k=100):
         out["maps_trace_build_ms"] = 1e3 * (time.perf_counter() - t0)
         out["global_photons"], out["caustic_photons"] = int(n_g), int(n_c)
         kw = dict(k=k, caustics_map=cmap, n_emit_caustics=1e7, final_gathering_depth=1, seed=0)
-        sc.render_photon_mapping(ns, gmap, 1e6, 64, 64, 2, **kw)  # warm
+        # one warm frame at the same size (workspaces sized), then the timed one
+        sc.render_photon_mapping(ns, gmap, 1e6, 512, 512, 100, **kw)
         torch.cuda.synchronize()
         img, _, st = sc.render_photon_mapping(ns, gmap, 1e6, 512, 512, 100, **kw)
         torch.cuda.synchronize()
@@ -627,7 +628,7 @@ def render_bench(ns, pmap, n_emit, width, height, spp, k, seed=0):
     empty.build()
     out = {}
     kw = dict(k=k, caustics_map=empty, n_emit_caustics=1.0, final_gathering_depth=1, seed=seed)
-    sc.render_photon_mapping(ns, pmap, n_emit, 64, 64, 4, **kw)     # warm
+    sc.render_photon_mapping(ns, pmap, n_emit, width, height, spp, **kw)  # warm, same size
     torch.cuda.synchronize()
     img, _, st = sc.render_photon_mapping(ns, pmap, n_emit, width, height, spp, **kw)
     torch.cuda.synchronize()
@@ -638,7 +639,7 @@ def render_bench(ns, pmap, n_emit, width, height, spp, k, seed=0):
         "rounds": st["iterations"], "batches": st["batches"],
         "dropped_samples": st["dropped_samples"],
         "image_mean": [float(x) for x in img.double().mean((0, 1)).cpu()]}
-    sc.render_path_tracing(ns, 64, 64, 4, seed=seed)
+    sc.render_path_tracing(ns, width, height, spp, seed=seed)  # warm, same size
     torch.cuda.synchronize()
     img, _, st = sc.render_path_tracing(ns, width, height, spp, seed=seed)
     torch.cuda.synchronize()

@@ -418,56 +418,145 @@ __global__ __launch_bounds__(kThreads) void shade_kernel(RenderDev R, Segs in, u
   wave_count_add<16>(O.rays, n_rays);
 }
 
-// The three per-segment counts of a shade round (segments, global queries,
-// caustics queries), to be prefix-summed.
-__global__ __launch_bounds__(kThreads) void flag_counts_kernel(const uint32_t* __restrict__ flags,
-                                                               uint32_t n,
-                                                               uint32_t* __restrict__ c_seg,
-                                                               uint32_t* __restrict__ c_qg,
-                                                               uint32_t* __restrict__ c_qc) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t f = flags[i];
-  c_seg[i] = f & kSegCount;
-  c_qg[i] = (f & kFlagGlobal) ? 1u : 0u;
-  c_qc[i] = (f & kFlagCaustics) ? 1u : 0u;
+// ---- list compaction: one scan over the flag words for all three lists ----
+// A chunk of kListChunk flags is taken by one block of kListThreads threads,
+// wave w holding the consecutive elements [w * 1024, (w + 1) * 1024), 64 per
+// round.  An element's three counts (segments written, a global query, a
+// caustics query) are summed packed: 10-bit fields within a round of a wave,
+// 21-bit fields within a chunk (a chunk's totals stay below 2^13).
+constexpr int kListThreads = 256;
+constexpr int kListItems = 16;
+constexpr uint32_t kListChunk = kListThreads * kListItems;
+constexpr int kListWaves = kListThreads / 64;
+
+__device__ __forceinline__ uint32_t list_counts10(uint32_t f) {
+  return (f & kSegCount) | (((f >> 2) & 1u) << 10) | (((f >> 3) & 1u) << 20);
+}
+__device__ __forceinline__ uint64_t widen21(uint32_t v) {
+  return static_cast<uint64_t>(v & 1023u) | (static_cast<uint64_t>((v >> 10) & 1023u) << 21) |
+         (static_cast<uint64_t>((v >> 20) & 1023u) << 42);
+}
+__device__ __forceinline__ uint32_t field21(uint64_t v, int c) {
+  return static_cast<uint32_t>((v >> (21 * c)) & ((1u << 21) - 1u));
+}
+// inclusive prefix over the wave's lanes
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = static_cast<int>(__lane_id());
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  return v;
 }
 
-// The lists' lengths: the last exclusive prefix plus the last count.
-__global__ void list_totals_kernel(const uint32_t* __restrict__ flags, uint32_t n,
-                                   const uint32_t* __restrict__ p_seg,
-                                   const uint32_t* __restrict__ p_qg,
-                                   const uint32_t* __restrict__ p_qc, uint32_t* __restrict__ out) {
-  if (threadIdx.x != 0 || n == 0) return;
-  const uint32_t f = flags[n - 1];
-  out[0] = p_seg[n - 1] + (f & kSegCount);
-  out[1] = p_qg[n - 1] + ((f & kFlagGlobal) ? 1u : 0u);
-  out[2] = p_qc[n - 1] + ((f & kFlagCaustics) ? 1u : 0u);
+// Totals of each chunk's three counts: part[3 * chunk + c].
+__global__ __launch_bounds__(kListThreads) void list_reduce_kernel(
+    const uint32_t* __restrict__ flags, uint32_t n, uint32_t* __restrict__ part) {
+  __shared__ uint64_t wt[kListWaves];
+  const int lane = static_cast<int>(__lane_id());
+  const int w = threadIdx.x >> 6;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kListChunk + w * 64 * kListItems;
+  uint64_t sum = 0;
+#pragma unroll
+  for (int r = 0; r < kListItems; ++r) {
+    const uint64_t e = base + r * 64 + lane;
+    if (e < n) sum += widen21(list_counts10(flags[e]));
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, 64);
+  if (lane == 0) wt[w] = sum;
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    uint64_t t = 0;
+    for (int i = 0; i < kListWaves; ++i) t += wt[i];
+    part[3 * blockIdx.x + threadIdx.x] = field21(t, static_cast<int>(threadIdx.x));
+  }
 }
 
-// Moves the staged outputs of a shade round to their compacted positions.
-__global__ __launch_bounds__(kThreads) void compact_kernel(
-    const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ p_seg,
-    const uint32_t* __restrict__ p_qg, const uint32_t* __restrict__ p_qc, Segs stage, Segs next,
-    const pm_query* __restrict__ qg_s, const Weighted* __restrict__ wg_s,
+// Exclusive prefix of the chunk totals of each count, in place (one block),
+// and the three list lengths into totals[0..2].
+__global__ __launch_bounds__(kListThreads) void list_top_kernel(uint32_t* __restrict__ part,
+                                                                uint32_t n_chunks,
+                                                                uint32_t* __restrict__ totals) {
+  __shared__ uint32_t wsum[kListWaves];
+  const int lane = static_cast<int>(__lane_id());
+  const int w = threadIdx.x >> 6;
+  for (int c = 0; c < 3; ++c) {
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < n_chunks; b += kListThreads) {
+      const uint32_t j = b + threadIdx.x;
+      const uint32_t v = j < n_chunks ? part[3 * j + c] : 0u;
+      const uint32_t incl = wave_incl_scan(v);
+      if (lane == 63) wsum[w] = incl;
+      __syncthreads();
+      uint32_t before = 0, all = 0;
+      for (int i = 0; i < kListWaves; ++i) {
+        if (i < w) before += wsum[i];
+        all += wsum[i];
+      }
+      if (j < n_chunks) part[3 * j + c] = carry + before + incl - v;
+      carry += all;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[c] = carry;
+  }
+}
+
+// Moves the staged outputs of a shade round to their compacted positions:
+// a chunk's element takes its list positions from the chunk's base (the
+// scanned totals) plus the counts before it in the chunk.
+__global__ __launch_bounds__(kListThreads) void compact_kernel(
+    const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ part,
+    Segs stage, Segs next, const pm_query* __restrict__ qg_s, const Weighted* __restrict__ wg_s,
     pm_query* __restrict__ qg, Weighted* __restrict__ wg, const pm_query* __restrict__ qc_s,
     const Weighted* __restrict__ wc_s, pm_query* __restrict__ qc, Weighted* __restrict__ wc) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t f = flags[i];
-  const uint32_t ns = f & kSegCount;
-  for (uint32_t j = 0; j < ns; ++j) {
-    const Segment sg = seg_load(stage, 2 * i + j);
-    seg_store(next, p_seg[i] + j, ldv(sg.o), ldv(sg.d), ldv(sg.w), sg.sample, sg.dim, sg.meta,
-              sg.stream);
+  __shared__ uint64_t wt[kListWaves];
+  const int lane = static_cast<int>(__lane_id());
+  const int w = threadIdx.x >> 6;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kListChunk + w * 64 * kListItems;
+  uint32_t f[kListItems];
+  uint64_t before[kListItems];  // counts of the wave's earlier elements, 21-bit fields
+  uint64_t run = 0;
+#pragma unroll
+  for (int r = 0; r < kListItems; ++r) {
+    const uint64_t e = base + r * 64 + lane;
+    f[r] = e < n ? flags[e] : 0u;
+    const uint32_t v = list_counts10(f[r]);
+    const uint32_t incl = wave_incl_scan(v);
+    before[r] = run + widen21(incl - v);
+    run += widen21(__shfl(incl, 63, 64));
   }
-  if (f & kFlagGlobal) {
-    qg[p_qg[i]] = qg_s[i];
-    wg[p_qg[i]] = wg_s[i];
-  }
-  if (f & kFlagCaustics) {
-    qc[p_qc[i]] = qc_s[i];
-    wc[p_qc[i]] = wc_s[i];
+  if (lane == 0) wt[w] = run;
+  __syncthreads();
+  uint64_t wb = 0;
+  for (int i = 0; i < w; ++i) wb += wt[i];
+  const uint32_t b_seg = part[3 * blockIdx.x + 0] + field21(wb, 0);
+  const uint32_t b_qg = part[3 * blockIdx.x + 1] + field21(wb, 1);
+  const uint32_t b_qc = part[3 * blockIdx.x + 2] + field21(wb, 2);
+#pragma unroll
+  for (int r = 0; r < kListItems; ++r) {
+    const uint64_t e64 = base + r * 64 + lane;
+    if (e64 >= n) continue;
+    const uint32_t i = static_cast<uint32_t>(e64);
+    const uint32_t fl = f[r];
+    const uint32_t ns = fl & kSegCount;
+    const uint32_t at_seg = b_seg + field21(before[r], 0);
+    for (uint32_t j = 0; j < ns; ++j) {
+      const Segment sg = seg_load(stage, 2 * i + j);
+      seg_store(next, at_seg + j, ldv(sg.o), ldv(sg.d), ldv(sg.w), sg.sample, sg.dim, sg.meta,
+                sg.stream);
+    }
+    if (fl & kFlagGlobal) {
+      const uint32_t at = b_qg + field21(before[r], 1);
+      qg[at] = qg_s[i];
+      wg[at] = wg_s[i];
+    }
+    if (fl & kFlagCaustics) {
+      const uint32_t at = b_qc + field21(before[r], 2);
+      qc[at] = qc_s[i];
+      wc[at] = wc_s[i];
+    }
   }
 }
 
@@ -690,7 +779,7 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
   // Buffers grow to each round's needs: a segment spawns at most two
   // segments and one query per photon map.
   DevBuf seg[2], stage, qg_s, wg_s, qc_s, wc_s, qg, wg, qc, wc, rgb, r2, acc, bad, counters;
-  DevBuf flags, c_seg, c_qg, c_qc, partial;
+  DevBuf flags, part;
   int rc = acc.reserve(static_cast<size_t>(bsamples) * 3 * sizeof(unsigned long long));
   if (!rc) rc = bad.reserve(static_cast<size_t>(bsamples) * sizeof(uint32_t));
   if (!rc) rc = counters.reserve((8 + kRayShards + 8) * sizeof(unsigned long long));
@@ -721,10 +810,7 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
     if (!e && R.caustics)
       e = grow_keep(wc, (used[1] + n_in) * sizeof(Weighted), used[1] * sizeof(Weighted));
     if (!e) e = flags.reserve(n_in * sizeof(uint32_t));
-    if (!e) e = c_seg.reserve(n_in * sizeof(uint32_t));
-    if (!e) e = c_qg.reserve(n_in * sizeof(uint32_t));
-    if (!e) e = c_qc.reserve(n_in * sizeof(uint32_t));
-    if (!e) e = partial.reserve(static_cast<size_t>(scan_partial_words(n_in)) * sizeof(uint32_t));
+    if (!e) e = part.reserve(3 * ((n_in + kListChunk - 1) / kListChunk + 1) * sizeof(uint32_t));
     return e;
   };
   // counter words (uint32): 0-2 the lists' lengths after a round; uint64 at
@@ -765,18 +851,15 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
       const dim3 grid(grid_of(n_cur));
       hipLaunchKernelGGL(shade_kernel, grid, dim3(kThreads), s->stack_bytes(0), nullptr, R, cur,
                          n_cur, O);
-      hipLaunchKernelGGL(flag_counts_kernel, grid, dim3(kThreads), 0, nullptr,
-                         flags.as<uint32_t>(), n_cur, c_seg.as<uint32_t>(), c_qg.as<uint32_t>(),
-                         c_qc.as<uint32_t>());
-      PM_HIP(hipGetLastError());
-      for (DevBuf* c : {&c_seg, &c_qg, &c_qc})
-        if (int e = exclusive_scan_u32(partial.as<uint32_t>(), c->as<uint32_t>(), n_cur, nullptr))
-          return e;
-      hipLaunchKernelGGL(list_totals_kernel, dim3(1), dim3(64), 0, nullptr, flags.as<uint32_t>(),
-                         n_cur, c_seg.as<uint32_t>(), c_qg.as<uint32_t>(), c_qc.as<uint32_t>(),
-                         c32);
-      hipLaunchKernelGGL(compact_kernel, grid, dim3(kThreads), 0, nullptr, flags.as<uint32_t>(),
-                         n_cur, c_seg.as<uint32_t>(), c_qg.as<uint32_t>(), c_qc.as<uint32_t>(),
+      // the three lists' positions: chunk totals, their prefix (and the
+      // lists' lengths into c32[0..2]), then the compaction itself
+      const uint32_t n_chunks = static_cast<uint32_t>((n_cur + kListChunk - 1) / kListChunk);
+      hipLaunchKernelGGL(list_reduce_kernel, dim3(n_chunks), dim3(kListThreads), 0, nullptr,
+                         flags.as<uint32_t>(), n_cur, part.as<uint32_t>());
+      hipLaunchKernelGGL(list_top_kernel, dim3(1), dim3(kListThreads), 0, nullptr,
+                         part.as<uint32_t>(), n_chunks, c32);
+      hipLaunchKernelGGL(compact_kernel, dim3(n_chunks), dim3(kListThreads), 0, nullptr,
+                         flags.as<uint32_t>(), n_cur, part.as<uint32_t>(),
                          stg, nxt, qg_s.as<pm_query>(), wg_s.as<Weighted>(),
                          qg.as<pm_query>() + used[0], wg.as<Weighted>() + used[0],
                          qc_s.as<pm_query>(), wc_s.as<Weighted>(),

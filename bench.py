@@ -738,7 +738,8 @@ def main() -> int:
         dt = time.perf_counter() - t1
         tracing = {"ms": 1e3 * dt, "stored_photons_per_s": args.photons / dt,
                    "paths_per_s": n_emit / dt, "emitted_paths": int(n_emit),
-                   "note": "two passes (count, write) over the same counter-based paths; "
+                   "note": "counter-based paths traced once, each path's first 16 photons "
+                           "cached and then placed (paths that store more traced again); "
                            "BVH closest hit"}
     q = sc.camera_queries(ns, args.width, args.height, args.spp, depth=1, seed=0)
     torch.cuda.synchronize()

@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cstring>
 #include <new>
+#include <deque>
 #include <vector>
 
 #include "pm_scene_dev.hpp"
@@ -147,20 +148,26 @@ __global__ __launch_bounds__(kThreads) void surrogate_photons_kernel(
 
 // ---- photon tracing (include/integrator.h:393-563) -------------------------------
 
-// One photon path per lane.  COUNT: photons stored per path into counts;
+// A photon in the count pass's cache: position, wi, flux.
+constexpr int kCacheFloats = 9;
+
+// One photon path per lane.  COUNT: photons stored per path into counts, and
+// (with a cache) the path's first cache_cap photons into its cache slots;
 // otherwise the photons are written at offsets (original photon order =
-// (path, bounce) order), those at or past n_store dropped.  CAUSTICS: the
-// path ends at its first Lambert hit, stored only after specular hits.
+// (path, bounce) order), those at or past n_store dropped.  The paths are
+// path0 + i, or path_list[i] when a list is given.  CAUSTICS: the path ends at
+// its first Lambert hit, stored only after specular hits.
 template <bool COUNT, bool CAUSTICS>
 __global__ __launch_bounds__(kThreads) void trace_kernel(
     SceneDev S, int64_t path0, int64_t n_paths, uint64_t seed, int max_depth,
     uint32_t* __restrict__ counts, const int64_t* __restrict__ offsets, int64_t n_store,
-    float* __restrict__ pos, float* __restrict__ wi, float* __restrict__ flux) {
+    float* __restrict__ pos, float* __restrict__ wi, float* __restrict__ flux,
+    float* __restrict__ cache, int cache_cap, const int64_t* __restrict__ path_list) {
   extern __shared__ int stk_all[];  // SceneImpl::stack_bytes
   int* stk = stk_all + threadIdx.x;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= n_paths) return;
-  const uint64_t path = static_cast<uint64_t>(path0 + i);
+  const uint64_t path = static_cast<uint64_t>(path_list ? path_list[i] : path0 + i);
   uint32_t dim = 0;
   // light, point and direction (include/integrator.h:265-288, include/light.h:27-44)
   int li = static_cast<int>(rnd(seed, path, dim++) * S.n_lights);
@@ -203,6 +210,11 @@ __global__ __launch_bounds__(kThreads) void trace_kernel(
             wi[at] = wo.x; wi[n_store + at] = wo.y; wi[2 * n_store + at] = wo.z;
             flux[at] = tr_r; flux[n_store + at] = tr_g; flux[2 * n_store + at] = tr_b;
           }
+        } else if (cache && stored < static_cast<uint32_t>(cache_cap)) {
+          float* c = cache + (i * cache_cap + stored) * kCacheFloats;
+          c[0] = x.x; c[1] = x.y; c[2] = x.z;
+          c[3] = wo.x; c[4] = wo.y; c[5] = wo.z;
+          c[6] = tr_r; c[7] = tr_g; c[8] = tr_b;
         }
         ++stored;
       }
@@ -230,6 +242,27 @@ __global__ __launch_bounds__(kThreads) void trace_kernel(
   if (COUNT) counts[i] = stored;
 }
 
+// The count pass's cached photons to their places: thread t takes slot
+// t % cap of path t / cap of a chunk (paths [0, n) of it, their counts and
+// offsets), and writes it at offsets[path] + slot when the path stored that
+// many and the place is below n_store.
+__global__ __launch_bounds__(kThreads) void cache_scatter_kernel(
+    const float* __restrict__ cache, int cap, int64_t n, const uint32_t* __restrict__ counts,
+    const int64_t* __restrict__ offsets, int64_t n_store, float* __restrict__ pos,
+    float* __restrict__ wi, float* __restrict__ flux) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t pl = t / cap;
+  const int j = static_cast<int>(t - pl * cap);
+  if (pl >= n) return;
+  if (static_cast<uint32_t>(j) >= counts[pl]) return;
+  const int64_t at = offsets[pl] + j;
+  if (at >= n_store) return;
+  const float* c = cache + t * kCacheFloats;
+  pos[at] = c[0]; pos[n_store + at] = c[1]; pos[2 * n_store + at] = c[2];
+  wi[at] = c[3]; wi[n_store + at] = c[4]; wi[2 * n_store + at] = c[5];
+  flux[at] = c[6]; flux[n_store + at] = c[7]; flux[2 * n_store + at] = c[8];
+}
+
 }  // namespace
 
 // ---- host side --------------------------------------------------------------------
@@ -243,45 +276,86 @@ uint64_t kind_seed(uint64_t seed, int kind) {
 }
 
 int launch_trace_count(const SceneImpl* s, int kind, int64_t path0, int64_t n, uint64_t seed,
-                       int max_depth, uint32_t* counts) {
+                       int max_depth, uint32_t* counts, float* cache, int cap) {
   const dim3 grid(static_cast<unsigned>((n + kThreads - 1) / kThreads)), block(kThreads);
   const SceneDev S = s->dev(0);
   if (kind == PMX_PHOTONS_CAUSTICS)
     hipLaunchKernelGGL((trace_kernel<true, true>), grid, block, s->stack_bytes(0), nullptr, S, path0, n, seed,
-                       max_depth, counts, nullptr, int64_t{0}, nullptr, nullptr, nullptr);
+                       max_depth, counts, nullptr, int64_t{0}, nullptr, nullptr, nullptr, cache,
+                       cap, nullptr);
   else
     hipLaunchKernelGGL((trace_kernel<true, false>), grid, block, s->stack_bytes(0), nullptr, S, path0, n, seed,
-                       max_depth, counts, nullptr, int64_t{0}, nullptr, nullptr, nullptr);
+                       max_depth, counts, nullptr, int64_t{0}, nullptr, nullptr, nullptr, cache,
+                       cap, nullptr);
   PM_HIP(hipGetLastError());
   return PM_OK;
 }
 
+// The write pass over paths [0, n_paths), or over the n_paths paths of
+// path_list (offsets then holds their offsets, in list order).
 int launch_trace_write(const SceneImpl* s, int kind, int64_t n_paths, uint64_t seed,
                        int max_depth, const int64_t* offsets, int64_t n_store, float* pos,
-                       float* wi, float* flux) {
+                       float* wi, float* flux, const int64_t* path_list = nullptr) {
   const dim3 grid(static_cast<unsigned>((n_paths + kThreads - 1) / kThreads)), block(kThreads);
   const SceneDev S = s->dev(0);
   if (kind == PMX_PHOTONS_CAUSTICS)
     hipLaunchKernelGGL((trace_kernel<false, true>), grid, block, s->stack_bytes(0), nullptr, S, int64_t{0},
-                       n_paths, seed, max_depth, nullptr, offsets, n_store, pos, wi, flux);
+                       n_paths, seed, max_depth, nullptr, offsets, n_store, pos, wi, flux, nullptr,
+                       0, path_list);
   else
     hipLaunchKernelGGL((trace_kernel<false, false>), grid, block, s->stack_bytes(0), nullptr, S, int64_t{0},
-                       n_paths, seed, max_depth, nullptr, offsets, n_store, pos, wi, flux);
+                       n_paths, seed, max_depth, nullptr, offsets, n_store, pos, wi, flux, nullptr,
+                       0, path_list);
   PM_HIP(hipGetLastError());
   return PM_OK;
 }
 
 constexpr int64_t kMinChunk = int64_t{1} << 16;
 constexpr int64_t kMaxChunk = int64_t{1} << 22;
+// Photons a path keeps in the count pass's cache: a caustic path stores at
+// most one; a global path in the box scenes stores about four on average.
+int cache_cap_of(int kind) { return kind == PMX_PHOTONS_CAUSTICS ? 1 : 16; }
+// Most bytes of cached photons one tracing call keeps before it falls back to
+// tracing every path twice.
+constexpr int64_t kCacheBytesMax = int64_t{2} << 30;
+
+// A chunk of the count pass whose counts and first photons stay on the
+// device, in the scene's trace_counts and trace_cache from these offsets.
+struct CountChunk {
+  int64_t path0 = 0;      // first path of the chunk
+  int64_t n = 0;          // paths of the chunk that are emitted
+  int64_t count_at = 0;   // uint32 words into trace_counts
+  int64_t cache_at = 0;   // floats into trace_cache
+};
+
+// Grows buf to at least `need` bytes (by half again at least), keeping its
+// first `keep` bytes.
+int grow_keep(DevBuf& buf, size_t need, size_t keep) {
+  if (need <= buf.bytes) return PM_OK;
+  DevBuf nb;
+  if (int rc = nb.reserve(std::max(need, buf.bytes + buf.bytes / 2))) return rc;
+  if (keep > 0) PM_HIP(hipMemcpy(nb.ptr, buf.ptr, keep, hipMemcpyDeviceToDevice));
+  std::swap(buf.ptr, nb.ptr);
+  std::swap(buf.bytes, nb.bytes);
+  return PM_OK;  // nb frees the old storage
+}
 
 // Count pass over paths [0, ...): appends per-path exclusive offsets until
 // either `stop_store` photons are stored (stop_store >= 0) or `stop_paths`
 // paths are counted (stop_paths >= 0).  Returns the photons stored in *acc.
+// With chunks, the pass also keeps every chunk's counts and each path's first
+// cache_cap_of(kind) photons on the device, and lists in *overflow the paths
+// that stored more; when that cache would pass kCacheBytesMax it is dropped
+// (chunks left empty, *cached false) and the caller traces every path again.
 int count_paths(const SceneImpl* s, int kind, uint64_t seed, int max_depth, int64_t stop_store,
-                int64_t stop_paths, std::vector<int64_t>& offs, int64_t* acc_out) {
-  DevBuf dcount;
-  if (int rc = dcount.reserve(kMaxChunk * sizeof(uint32_t))) return rc;
-  std::vector<uint32_t> cnt(kMaxChunk);
+                int64_t stop_paths, std::vector<int64_t>& offs, int64_t* acc_out,
+                std::deque<CountChunk>* chunks = nullptr,
+                std::vector<int64_t>* overflow = nullptr, bool* cached = nullptr) {
+  DevBuf dcount;              // counts of a chunk that is not cached
+  std::vector<uint32_t> cnt;  // grown to the largest chunk
+  const int cap = cache_cap_of(kind);
+  bool caching = chunks != nullptr && overflow != nullptr;
+  int64_t counts_used = 0, cache_used = 0;  // words / floats of the scene's buffers in use
   int64_t acc = 0, paths = 0;
   int64_t chunk = kMinChunk;
   for (;;) {
@@ -294,16 +368,48 @@ int count_paths(const SceneImpl* s, int kind, uint64_t seed, int max_depth, int6
     if (stop_store >= 0 && paths >= (int64_t{1} << 26) &&
         static_cast<double>(acc) * 1e4 < static_cast<double>(stop_store))
       return fail(PM_ERR_INVALID_ARG, "photon tracing stores too few photons to reach the count");
-    if (int rc = launch_trace_count(s, kind, paths, chunk, seed, max_depth,
-                                    dcount.as<uint32_t>()))
+    uint32_t* dc = nullptr;
+    float* dcache = nullptr;
+    CountChunk* cc = nullptr;
+    if (caching) {
+      const int64_t cf = chunk * cap * kCacheFloats;  // floats of this chunk's cache
+      if ((cache_used + cf) * static_cast<int64_t>(sizeof(float)) > kCacheBytesMax) {
+        caching = false;
+        chunks->clear();
+        overflow->clear();
+      } else {
+        if (int rc = grow_keep(s->trace_counts, (counts_used + chunk) * sizeof(uint32_t),
+                               counts_used * sizeof(uint32_t)))
+          return rc;
+        if (int rc = grow_keep(s->trace_cache, (cache_used + cf) * sizeof(float),
+                               cache_used * sizeof(float)))
+          return rc;
+        cc = &chunks->emplace_back();
+        cc->count_at = counts_used;
+        cc->cache_at = cache_used;
+        dc = s->trace_counts.as<uint32_t>() + counts_used;
+        dcache = s->trace_cache.as<float>() + cache_used;
+        counts_used += chunk;
+        cache_used += cf;
+      }
+    }
+    if (!dc) {
+      if (int rc = dcount.reserve(chunk * sizeof(uint32_t))) return rc;
+      dc = dcount.as<uint32_t>();
+    }
+    if (int rc = launch_trace_count(s, kind, paths, chunk, seed, max_depth, dc, dcache, cap))
       return rc;
-    PM_HIP(hipMemcpy(cnt.data(), dcount.ptr, chunk * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (cnt.size() < static_cast<size_t>(chunk)) cnt.resize(static_cast<size_t>(chunk));
+    PM_HIP(hipMemcpy(cnt.data(), dc, chunk * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (cc) cc->path0 = paths;
     for (int64_t j = 0; j < chunk; ++j) {
       if (stop_store >= 0 && acc >= stop_store) break;
       offs.push_back(acc);
       acc += cnt[j];
+      if (cc && cnt[j] > static_cast<uint32_t>(cap)) overflow->push_back(paths);
       ++paths;
     }
+    if (cc) cc->n = paths - cc->path0;
     if (stop_store >= 0 && acc < stop_store) {
       // size the next chunk from the photons per path seen so far
       const double per_path = std::max(static_cast<double>(acc) / static_cast<double>(paths), 1e-3);
@@ -312,6 +418,45 @@ int count_paths(const SceneImpl* s, int kind, uint64_t seed, int max_depth, int6
     }
   }
   *acc_out = acc;
+  if (cached) *cached = caching;
+  return PM_OK;
+}
+
+// The write pass after a count pass: the cached photons of every chunk to
+// their places, then the paths that stored more than the cache holds traced
+// again (all their photons written, the cached ones to the same values); or,
+// without a cache, every path traced again.
+int write_paths(const SceneImpl* s, int kind, uint64_t seed, int max_depth,
+                const std::vector<int64_t>& offs, const int64_t* doff, bool cached,
+                const std::deque<CountChunk>& chunks, const std::vector<int64_t>& overflow,
+                int64_t n_store, float* pos, float* wi, float* flux) {
+  const int64_t paths = static_cast<int64_t>(offs.size());
+  if (!cached)
+    return launch_trace_write(s, kind, paths, seed, max_depth, doff, n_store, pos, wi, flux);
+  const int cap = cache_cap_of(kind);
+  for (const CountChunk& cc : chunks) {
+    if (cc.n <= 0) continue;
+    const int64_t threads = cc.n * cap;
+    hipLaunchKernelGGL(cache_scatter_kernel, dim3(static_cast<unsigned>((threads + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, nullptr, s->trace_cache.as<float>() + cc.cache_at, cap,
+                       cc.n, s->trace_counts.as<uint32_t>() + cc.count_at, doff + cc.path0,
+                       n_store, pos, wi, flux);
+    PM_HIP(hipGetLastError());
+  }
+  if (!overflow.empty()) {
+    const size_t no = overflow.size();
+    std::vector<int64_t> ooff(no);
+    for (size_t i = 0; i < no; ++i) ooff[i] = offs[static_cast<size_t>(overflow[i])];
+    DevBuf dlist;
+    if (int rc = dlist.reserve(2 * no * sizeof(int64_t))) return rc;
+    int64_t* dl = dlist.as<int64_t>();
+    PM_HIP(hipMemcpy(dl, overflow.data(), no * sizeof(int64_t), hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(dl + no, ooff.data(), no * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (int rc = launch_trace_write(s, kind, static_cast<int64_t>(no), seed, max_depth, dl + no,
+                                    n_store, pos, wi, flux, dl))
+      return rc;
+    PM_HIP(hipDeviceSynchronize());  // dlist is freed on return
+  }
   return PM_OK;
 }
 
@@ -489,19 +634,25 @@ int pmx_trace_photons(const pmx_scene* scene, int kind, int64_t n_store, int max
   if (s->n_lights == 0) return fail(PM_ERR_INVALID_ARG, "the scene has no light triangle");
   PM_HIP(hipSetDevice(s->ctx->device));
   const uint64_t kseed = kind_seed(seed, kind);
-  // pass 1: photons stored per path until n_store are reached
+  // pass 1: photons stored per path until n_store are reached (the first
+  // photons of each path kept in a cache)
   std::vector<int64_t> offs;
+  std::deque<CountChunk> chunks;
+  std::vector<int64_t> over;
+  bool cached = false;
   int64_t acc = 0;
-  if (int rc = count_paths(s, kind, kseed, max_depth, n_store, -1, offs, &acc)) return rc;
+  if (int rc = count_paths(s, kind, kseed, max_depth, n_store, -1, offs, &acc, &chunks, &over,
+                           &cached))
+    return rc;
   // the path holding photon n_store - 1 is the last one emitted
   const int64_t paths = static_cast<int64_t>(offs.size());
   *n_emit = paths;
-  // pass 2: the same paths again, writing their photons in (path, bounce) order
+  // pass 2: the photons in (path, bounce) order
   DevBuf doff;
   if (int rc = doff.reserve(offs.size() * sizeof(int64_t))) return rc;
   PM_HIP(hipMemcpy(doff.ptr, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-  if (int rc = launch_trace_write(s, kind, paths, kseed, max_depth, doff.as<int64_t>(), n_store,
-                                  pos_soa, wi_soa, flux_soa))
+  if (int rc = write_paths(s, kind, kseed, max_depth, offs, doff.as<int64_t>(), cached, chunks,
+                           over, n_store, pos_soa, wi_soa, flux_soa))
     return rc;
   PM_HIP(hipDeviceSynchronize());
   return PM_OK;
@@ -519,9 +670,14 @@ int pmx_trace_photon_map(const pmx_scene* scene, int kind, int64_t n_paths, int 
   PM_HIP(hipSetDevice(s->ctx->device));
   const uint64_t kseed = kind_seed(seed, kind);
   std::vector<int64_t> offs;
+  std::deque<CountChunk> chunks;
+  std::vector<int64_t> over;
+  bool cached = false;
   int64_t total = 0;
   if (n_paths > 0)
-    if (int rc = count_paths(s, kind, kseed, max_depth, -1, n_paths, offs, &total)) return rc;
+    if (int rc = count_paths(s, kind, kseed, max_depth, -1, n_paths, offs, &total, &chunks, &over,
+                             &cached))
+      return rc;
   if (total > (int64_t{1} << 31) - 1) return fail(PM_ERR_INVALID_ARG, "more than 2^31 photons stored");
   *n_stored = total;
   DevBuf doff, dpos, dwi, dflux;
@@ -532,8 +688,8 @@ int pmx_trace_photon_map(const pmx_scene* scene, int kind, int64_t n_paths, int 
     if (int rc = dwi.reserve(bytes)) return rc;
     if (int rc = dflux.reserve(bytes)) return rc;
     PM_HIP(hipMemcpy(doff.ptr, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-    if (int rc = launch_trace_write(s, kind, n_paths, kseed, max_depth, doff.as<int64_t>(), total,
-                                    dpos.as<float>(), dwi.as<float>(), dflux.as<float>()))
+    if (int rc = write_paths(s, kind, kseed, max_depth, offs, doff.as<int64_t>(), cached, chunks,
+                             over, total, dpos.as<float>(), dwi.as<float>(), dflux.as<float>()))
       return rc;
     PM_HIP(hipDeviceSynchronize());
   }

@@ -753,12 +753,20 @@ def main() -> int:
         ctx.set_option(_lib.PM_OPT_RUN_SPLIT, int(round(args.split * 1000)))
     if args.waves > 0:
         ctx.set_option(_lib.PM_OPT_WAVES_PER_CU, args.waves)
+    # the map build: the process's first (the context's workspaces allocated
+    # on the way), then a second map over the same photons, as a renderer that
+    # traces a new map every frame builds it
+    ctx.set_option(_lib.PM_OPT_TIMING, 1)
+    first = pm.PhotonMap(ctx)
+    first.set_photons_soa(pos, wi, flux)
+    first.build(args.leaf)
+    build_first_ms = ctx.stats()["build_ms"]
     pmap = pm.PhotonMap(ctx)
     pmap.set_photons_soa(pos, wi, flux)
-    ctx.set_option(_lib.PM_OPT_TIMING, 1)
     pmap.build(args.leaf)
     build_ms = ctx.stats()["build_ms"]
     ctx.set_option(_lib.PM_OPT_TIMING, 0)
+    del first
 
     rgb = torch.empty((m, 3), dtype=torch.float32, device=dev)
     r2 = torch.empty((m,), dtype=torch.float32, device=dev)
@@ -925,7 +933,7 @@ def main() -> int:
         },
         "roofline": roofline(k, m, gather_ms, pmc),
         "cpu_baseline": cpu,
-        "detail": {"sort_ms": sort_ms, "build_ms": build_ms, **work, **(adj or {}),
+        "detail": {"sort_ms": sort_ms, "build_ms": build_ms, "build_first_ms": build_first_ms, **work, **(adj or {}),
                    "batched": batched, "autotune": tuned, "tracing": tracing, "rays": rays,
                    "render": render},
     }

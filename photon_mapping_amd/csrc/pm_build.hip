@@ -123,17 +123,31 @@ __global__ __launch_bounds__(kThreads) void bbox_partial_kernel(
   }
 }
 
-__global__ void bbox_final_kernel(const float* __restrict__ partial, int nb,
-                                  float* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+// The per-block boxes reduced by one block: each thread a strided share, then
+// a tree over the threads.
+__global__ __launch_bounds__(kThreads) void bbox_final_kernel(const float* __restrict__ partial,
+                                                              int nb, float* __restrict__ out) {
+  __shared__ float sr[6][kThreads];
+  const int t = threadIdx.x;
   float r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < nb; ++b) {
+  for (int b = t; b < nb; b += kThreads) {
     for (int c = 0; c < 3; ++c) {
       r[c] = fminf(r[c], partial[b * 6 + c]);
       r[3 + c] = fmaxf(r[3 + c], partial[b * 6 + 3 + c]);
     }
   }
-  for (int c = 0; c < 6; ++c) out[c] = r[c];
+  for (int c = 0; c < 6; ++c) sr[c][t] = r[c];
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      for (int c = 0; c < 3; ++c) {
+        sr[c][t] = fminf(sr[c][t], sr[c][t + o]);
+        sr[3 + c][t] = fmaxf(sr[3 + c][t], sr[3 + c][t + o]);
+      }
+    }
+    __syncthreads();
+  }
+  if (t < 6) out[t] = sr[t][0];
 }
 
 struct Quant {
@@ -150,18 +164,28 @@ __global__ __launch_bounds__(kThreads) void photon_codes_kernel(
   vals[i] = static_cast<uint32_t>(i);
 }
 
-__global__ __launch_bounds__(kThreads) void reorder_kernel(
-    const float* __restrict__ pos, const float* __restrict__ wi,
-    const float* __restrict__ flux, int64_t n, const uint32_t* __restrict__ order,
-    float4* __restrict__ pos4, float4* __restrict__ wif4, float2* __restrict__ flux2,
-    int32_t* __restrict__ inv_perm) {
+// inv_perm[order[s]] = s: each photon's slot.
+__global__ __launch_bounds__(kThreads) void inv_perm_kernel(const uint32_t* __restrict__ order,
+                                                            int64_t n,
+                                                            int32_t* __restrict__ inv_perm) {
   const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (s >= n) return;
-  const uint32_t i = order[s];
-  pos4[s] = make_float4(pos[i], pos[n + i], pos[2 * n + i], __uint_as_float(i));
+  inv_perm[order[s]] = static_cast<int32_t>(s);
+}
+
+// The photons into slot order, taken in original order: the nine SoA reads
+// are coalesced and each photon's three records are written at its slot
+// (three scattered stores instead of nine scattered loads).
+__global__ __launch_bounds__(kThreads) void reorder_kernel(
+    const float* __restrict__ pos, const float* __restrict__ wi,
+    const float* __restrict__ flux, int64_t n, const int32_t* __restrict__ inv_perm,
+    float4* __restrict__ pos4, float4* __restrict__ wif4, float2* __restrict__ flux2) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int32_t s = inv_perm[i];
+  pos4[s] = make_float4(pos[i], pos[n + i], pos[2 * n + i], __uint_as_float(static_cast<uint32_t>(i)));
   wif4[s] = make_float4(wi[i], wi[n + i], wi[2 * n + i], flux[i]);
   flux2[s] = make_float2(flux[n + i], flux[2 * n + i]);
-  inv_perm[i] = static_cast<int32_t>(s);
 }
 
 __global__ __launch_bounds__(kThreads) void update_flux_kernel(
@@ -565,7 +589,7 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
   if (int rc = launch_fill_u32(bad, 0u, 1, s)) return rc;
   hipLaunchKernelGGL(bbox_partial_kernel, dim3(nb), dim3(kThreads), 0, s, pos, n, partial,
                      bad);
-  hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(64), 0, s, partial, nb, box_out);
+  hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kThreads), 0, s, partial, nb, box_out);
   PM_HIP(hipGetLastError());
   float hb[7];
   PM_HIP(hipMemcpyAsync(hb, box_out, 7 * sizeof(float), hipMemcpyDeviceToHost, s));
@@ -607,10 +631,12 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
   if (int rc = map->flux2.reserve(n * sizeof(float2))) return rc;
   if (int rc = map->inv_perm.reserve(n * sizeof(int32_t))) return rc;
   if (int rc = map->codes.reserve(n * sizeof(uint32_t))) return rc;
+  hipLaunchKernelGGL(inv_perm_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, order,
+                     n, map->inv_perm.as<int32_t>());
   hipLaunchKernelGGL(reorder_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, pos,
-                     map->wi_soa.as<float>(), map->flux_soa.as<float>(), n, order,
-                     map->pos4.as<float4>(), map->wif4.as<float4>(),
-                     map->flux2.as<float2>(), map->inv_perm.as<int32_t>());
+                     map->wi_soa.as<float>(), map->flux_soa.as<float>(), n,
+                     map->inv_perm.as<int32_t>(), map->pos4.as<float4>(),
+                     map->wif4.as<float4>(), map->flux2.as<float2>());
   PM_HIP(hipMemcpyAsync(map->codes.ptr, sorted_codes, n * sizeof(uint32_t),
                         hipMemcpyDeviceToDevice, s));
 

@@ -332,6 +332,21 @@ def plain_config_bench(ctx, with_cpu, budget_s=10.0):
     return out
 
 
+def gpu_call_ms(fn, stream) -> float:
+    """GPU time of one library call: HIP events on the call's stream around
+    it, so every kernel and copy it enqueues, and the gaps between them,
+    count."""
+    import torch
+
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    fn()
+    e1.record(stream)
+    e1.synchronize()
+    return float(e0.elapsed_time(e1))
+
+
 def water_diff_bench(ctx, dev, k=100, reps=3, frame=True):
     """The differentiation configuration on the water scene (131,072-triangle
     height field, glass IOR 1.33, walls cut into 64 albedo patches, 66
@@ -341,7 +356,8 @@ def water_diff_bench(ctx, dev, k=100, reps=3, frame=True):
     final-gathering hits (times the primary-hit albedo) and the caustics map
     at the primary diffuse hits (include/integrator.h:155-175, 239-244, 314);
     the adjoint runs both, with the forward keys reused and with the kNN
-    recomputed.  HIP-event times of whole calls, like the main measurement.
+    recomputed; the adjoint with the forward keys follows its forward call and
+    takes that call's query order.  HIP-event times of whole calls.
     With frame, also the rendered photon-mapping frame of the scene (both
     maps, final gathering, k = 100)."""
     import numpy as np
@@ -403,22 +419,18 @@ def water_diff_bench(ctx, dev, k=100, reps=3, frame=True):
                   sptr)
 
     def call_ms(fn):
-        fn()
-        st = ctx.stats()
-        return st["sort_ms"] + st["gather_ms"] + st["post_ms"]
+        return gpu_call_ms(fn, stream)
 
     for b in bufs.values():
         fwd(b)
         adj(b, True)
     torch.cuda.synchronize()
-    ctx.set_option(_lib.PM_OPT_TIMING, 1)
     times = {f"{n}_{w}": [] for n in bufs for w in ("forward", "reuse", "recompute")}
     for _ in range(reps):
         for name, b in bufs.items():
             times[f"{name}_forward"].append(call_ms(lambda: fwd(b)))
             times[f"{name}_reuse"].append(call_ms(lambda: adj(b, True)))
             times[f"{name}_recompute"].append(call_ms(lambda: adj(b, False)))
-    ctx.set_option(_lib.PM_OPT_TIMING, 0)
     t = {kk: float(np.mean(v)) for kk, v in times.items()}
     out.update({f"{kk}_ms": v for kk, v in t.items()})
     f = t["global_forward"] + t["caustics_forward"]
@@ -833,26 +845,31 @@ def main() -> int:
     sort_ms = float(np.mean(sms))
 
     # ---- adjoint: reuse the forward keys, and recompute the sets ------------------
+    # Whole calls timed by HIP events on their stream: each round is a forward
+    # call (fresh keys), the adjoint with those keys (which takes the forward
+    # call's query order), then the adjoint that recomputes the sets.
     adj = None
     if not args.no_adjoint:
         g = torch.randn((m, 3), dtype=torch.float32, device=dev,
                         generator=torch.Generator(device=dev).manual_seed(0))
         dflux = torch.zeros((3, args.photons), dtype=torch.float32, device=dev)
         dalb = torch.zeros((mats.shape[0], 3), dtype=torch.float32, device=dev)
-        step()  # fresh keys of this forward pass
-        adj = {}
-        ctx.set_option(_lib.PM_OPT_TIMING, 1)
-        for name, kk in (("reuse", key), ("recompute", None)):
-            times = []
-            for _ in range(max(2, args.steps // 2)):
-                _lib.call("pm_gather_adjoint", pmap.handle, m, _lib.ptr(q), _lib.ptr(mats),
-                          mats.shape[0], k, n_emit, _lib.ptr(g), _lib.ptr(kk), None,
-                          _lib.ptr(dflux), _lib.ptr(dalb), _lib.PM_MEM_DEVICE, sptr)
-                st_a = ctx.stats()
-                times.append(st_a["sort_ms"] + st_a["gather_ms"] + st_a["post_ms"])
-            adj[f"adjoint_{name}_ms"] = float(np.mean(times))
-        ctx.set_option(_lib.PM_OPT_TIMING, 0)
-        fwd_ms = sort_ms + gather_ms
+
+        def adjoint(kk):
+            _lib.call("pm_gather_adjoint", pmap.handle, m, _lib.ptr(q), _lib.ptr(mats),
+                      mats.shape[0], k, n_emit, _lib.ptr(g), _lib.ptr(kk), None,
+                      _lib.ptr(dflux), _lib.ptr(dalb), _lib.PM_MEM_DEVICE, sptr)
+
+        step()
+        adjoint(key)
+        adjoint(None)
+        times = {"forward": [], "reuse": [], "recompute": []}
+        for _ in range(max(2, args.steps // 2)):
+            times["forward"].append(gpu_call_ms(step, stream))
+            times["reuse"].append(gpu_call_ms(lambda: adjoint(key), stream))
+            times["recompute"].append(gpu_call_ms(lambda: adjoint(None), stream))
+        adj = {f"adjoint_{n}_ms": float(np.mean(times[n])) for n in ("reuse", "recompute")}
+        fwd_ms = float(np.mean(times["forward"]))
         adj["forward_ms"] = fwd_ms
         adj["fwd_plus_adj_over_fwd_reuse"] = (fwd_ms + adj["adjoint_reuse_ms"]) / fwd_ms
         adj["fwd_plus_adj_over_fwd_recompute"] = (fwd_ms + adj["adjoint_recompute_ms"]) / fwd_ms

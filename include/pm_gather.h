@@ -216,7 +216,12 @@ int pm_gather_radiance_noalloc(pm_photonmap* map, int64_t m, const pm_query* q,
  * fg_mat (may be NULL): per-query index of the primary-hit material of a
  * final-gathering bounce; when given, the forward value is rho_fg * L, so
  * the incoming gradient is scaled by rho_fg and dl_dalbedo[fg_mat] receives
- * dl_drgb * L (include/integrator.h:227-243). */
+ * dl_drgb * L (include/integrator.h:227-243).
+ * With keys and PM_MEM_DEVICE, a call on the same query pointer, m, map and
+ * stream as the context's last single-instance gather (no other gather,
+ * kNN or map build of the context in between) takes that call's query order
+ * instead of sorting the queries again.  The results do not depend on the
+ * order; only the speed does. */
 int pm_gather_adjoint(pm_photonmap* map, int64_t m, const pm_query* q,
                       const pm_material* mats, int n_mats, int k,
                       double n_emit, const float* dl_drgb, const uint64_t* key,

@@ -606,7 +606,9 @@ int build_photon_map(PhotonMapImpl* map, hipStream_t s) {
     map->qscale[c] = ext > 0.0f ? static_cast<float>(1u << kGridBits) / ext : 1.0f;
   }
 
-  // 2-3. Hilbert codes, stable sort
+  // 2-3. Hilbert codes, stable sort (over the context's query-sort buffers:
+  // the query order they held is gone)
+  ctx->last_order = Context::QueryOrder{};
   if (int rc = ctx->ws.keys.reserve(n * sizeof(uint32_t))) return rc;
   if (int rc = ctx->ws.vals.reserve(n * sizeof(uint32_t))) return rc;
   if (int rc = ctx->ws.keys2.reserve(n * sizeof(uint32_t))) return rc;

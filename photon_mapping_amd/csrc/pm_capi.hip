@@ -200,9 +200,12 @@ struct CallTimer {
 // query/output part filled, sizes in d.m) are laid out in query and group
 // order, staged, the queries of all instances sorted together along the curve, and
 // one persistent launch covers every group.  knn_out_idx (single kNN
-// instance only) receives the sorted rows.
+// instance only) receives the sorted rows.  order_q and order_map (a single
+// instance's device queries and its map, else null) name the query order the
+// context keeps for an adjoint that reuses the forward keys.
 int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream_t s,
-                  int32_t* knn_out_idx) {
+                  int32_t* knn_out_idx, const void* order_q = nullptr,
+                  const void* order_map = nullptr) {
   const int n_inst = static_cast<int>(ds.size());
   std::vector<int64_t> qp(n_inst + 1, 0), gp(n_inst + 1, 0);
   for (int i = 0; i < n_inst; ++i) {
@@ -238,9 +241,20 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
     need_anchors = need_anchors ||
                    (!(mode == kModeAdjoint && d.key_in != nullptr) && d.all_members == 0);
   const uint32_t* perm = nullptr;
-  if (int rc = prepare_queries(&ctx->ws, n_inst, dd, dqp, dgp, total_q, total_groups, s, &perm,
-                               need_anchors))
-    return rc;
+  const Context::QueryOrder& last = ctx->last_order;
+  const bool take_order = n_inst == 1 && mode == kModeAdjoint && ds[0].key_in != nullptr &&
+                          order_q != nullptr && last.perm != nullptr && last.q == order_q &&
+                          last.m == total_q && last.map == order_map && last.s == s;
+  if (take_order) {
+    perm = last.perm;
+  } else {
+    ctx->last_order = Context::QueryOrder{};
+    if (int rc = prepare_queries(&ctx->ws, n_inst, dd, dqp, dgp, total_q, total_groups, s,
+                                 &perm, need_anchors))
+      return rc;
+    if (n_inst == 1 && order_q != nullptr)
+      ctx->last_order = Context::QueryOrder{order_q, total_q, order_map, s, perm};
+  }
   tm.mark(1);
   GatherLaunch L{};
   L.descs = dd;
@@ -293,9 +307,9 @@ int run_instances(Context* ctx, std::vector<GatherDesc>& ds, int mode, hipStream
 }
 
 int run_single(PhotonMapImpl* map, GatherDesc d, int mode, hipStream_t s,
-               int32_t* knn_out_idx) {
+               int32_t* knn_out_idx, const void* order_q = nullptr) {
   std::vector<GatherDesc> ds(1, d);
-  return run_instances(map->ctx, ds, mode, s, knn_out_idx);
+  return run_instances(map->ctx, ds, mode, s, knn_out_idx, order_q, map);
 }
 
 int check_map(pm_photonmap* h) {
@@ -824,7 +838,10 @@ int pm_gather_radiance(pm_photonmap* h, int64_t m, const pm_query* q, const pm_m
     d.k_eff = k_eff;
     d.n_emit = static_cast<float>(n_emit);
     d.all_members = k_eff >= n ? 1 : 0;
-    if (int rc = run_single(map, d, kModeRadiance, s, nullptr)) return rc;
+    // (device queries: the adjoint with these keys takes this call's order)
+    if (int rc = run_single(map, d, kModeRadiance, s, nullptr,
+                            mem == PM_MEM_DEVICE ? static_cast<const void*>(q) : nullptr))
+      return rc;
   }
   if (mem == PM_MEM_HOST) {
     PM_HIP(hipMemcpyAsync(out_rgb, drgb, 3 * m * sizeof(float), hipMemcpyDeviceToHost, s));
@@ -1018,7 +1035,9 @@ int pm_gather_adjoint(pm_photonmap* h, int64_t m, const pm_query* q, const pm_ma
   d.k_eff = k_eff;
   d.n_emit = static_cast<float>(n_emit);
   d.all_members = k_eff >= n ? 1 : 0;
-  if (int rc = run_single(map, d, kModeAdjoint, s, nullptr)) return rc;
+  if (int rc = run_single(map, d, kModeAdjoint, s, nullptr,
+                          mem == PM_MEM_DEVICE ? static_cast<const void*>(q) : nullptr))
+    return rc;
   if (int rc = launch_slot_grad_add(acc, map->inv_perm.as<int32_t>(), n, dflux, s)) return rc;
   if (mem == PM_MEM_HOST) {
     PM_HIP(hipMemcpyAsync(dl_dflux, dflux, 3 * n * sizeof(float), hipMemcpyDeviceToHost, s));

@@ -585,10 +585,10 @@ struct SharedNormal {
 
 // Iterates the candidate photons of the current list: body(cnt, amb) is
 // called once per staged chunk, after sh.pos (and with kStageFull also
-// sh.b.wif/flux, with kStageEff the effective flux in sh.b.wif) hold cnt
-// candidates; amb has bit j set for each staged photon j whose sign
-// kStageEff left open.  The next chunk's loads are in flight while body
-// runs.
+// sh.b.wif/flux, with kStageEff the effective flux in sh.b.wif, its w one
+// when the sign lets the photon through) hold cnt candidates; amb has bit j
+// set for each staged photon j whose sign kStageEff left open.  The next
+// chunk's loads are in flight while body runs.
 //
 // kStageEff decides each photon's Lambert sign once for all lanes: with d0 =
 // dot(wi, n0) in the reference's FP32 order, |d0| > |wi|_1 * m implies that
@@ -632,7 +632,8 @@ __device__ __forceinline__ void for_candidates_st(DescRef D, GatherShared& sh, i
       const float mw = (fabsf(w.x) + fabsf(w.y) + fabsf(w.z)) * sn.m + 1e-30f;
       const bool amb = !(fabsf(d0) > mw);
       const bool inc = !amb && d0 > 0.0f;
-      sh.b.wif[lane] = make_float4(inc ? w.w : 0.0f, inc ? f.x : 0.0f, inc ? f.y : 0.0f, 0.0f);
+      sh.b.wif[lane] = make_float4(inc ? w.w : 0.0f, inc ? f.x : 0.0f, inc ? f.y : 0.0f,
+                                   inc ? 1.0f : 0.0f);
       amb_mask = __ballot(amb && o + static_cast<uint32_t>(lane) < total);
     }
     if (kWF) sh.b.sb.chunk_slot[lane] = static_cast<uint32_t>(slot);  // (aliases the bins)
@@ -1759,7 +1760,13 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         const float c_g = ((rho_g / kPiF) * ge_g) / denom;
         const float c_b = ((rho_b / kPiF) * ge_b) / denom;
 
-        // (1) per query: the flux sum S of the estimate, for dL/drho
+        // One sweep over the run's candidates gives (1) each lane's flux sum
+        // S of its estimate, for dL/drho, and (2) per staged chunk, which
+        // contributing queries hold each of its photons: lane q's flags over
+        // the chunk (the same FP32 key and sign tests as the estimate) go to
+        // LDS as row q, and lane j adds up the coefficients of the queries
+        // whose row has bit j, in lane order, then makes one atomic per
+        // channel for photon j of the chunk.
         float s_r = 0.f, s_g = 0.f, s_b = 0.f;
         const uint32_t t_hi = static_cast<uint32_t>(tau >> 32);
         const uint32_t t_lo = static_cast<uint32_t>(tau & 0xffffffffull);
@@ -1767,27 +1774,43 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         // lanes share their normal, as in the forward selection pass)
         SharedNormal sn;
         const bool shared = shared_normal(contributes, nx, ny, nz, &sn);
-        auto pass1 = [&](auto shared_tag) {
+        // the rows and the coefficients live where the stash was: the stash
+        // is no longer read, and the chunk staging does not touch it
+        float4* qcoef = reinterpret_cast<float4*>(sh.b.sb.key);
+        uint32_t* rows32 = reinterpret_cast<uint32_t*>(qcoef + kWave);  // row q: words 2q, 2q + 1
+        static_assert(kWave * (sizeof(float4) + 2 * sizeof(uint32_t)) <= sizeof(sh.b.sb.key),
+                      "query coefficients and rows fit the stash");
+        qcoef[lane] = make_float4(c_r, c_g, c_b, 0.0f);
+        __syncthreads();
+        float* __restrict__ dflux = D.dl_dflux;
+        const int64_t n = D.n;
+        const uint32_t half = lane >= 32 ? 1u : 0u;
+        auto sweep = [&](auto shared_tag) {
           constexpr bool kSh = decltype(shared_tag)::value;
           constexpr int kSt = kSh ? kStageEff : kStageFull;
           for_candidates_st<kSt>(D, sh, lane, nr, total, sn, [&](uint32_t cnt, uint64_t amb) {
-            consume_st<kSt>(sh, cnt, [&](const Cand& c) {
+            uint64_t m = consume_st<kSt>(sh, cnt, [&](const Cand& c) {
               const float d2 = d2_ref(qx, qy, qz, c.p.x, c.p.y, c.p.z);
               const uint32_t db = __float_as_uint(d2);
               const bool nb = db < t_hi || (db == t_hi && __float_as_uint(c.p.w) <= t_lo);
+              bool on;
               if constexpr (kSh) {
+                // c.w: the flux when the shared sign lets it through (w set
+                // to one), else zeros
+                on = contributes && nb && c.w.w != 0.0f;
                 const float sel = (contributes && nb) ? 1.0f : 0.0f;
                 s_r = fmaf(sel, c.w.x, s_r);
                 s_g = fmaf(sel, c.w.y, s_g);
                 s_b = fmaf(sel, c.w.z, s_b);
               } else {
                 const float cos_i = dot_ref(c.w.x, c.w.y, c.w.z, nx, ny, nz);
-                const float sel = (contributes && nb && !(cos_i < 0.0f)) ? 1.0f : 0.0f;
+                on = contributes && nb && !(cos_i < 0.0f);
+                const float sel = on ? 1.0f : 0.0f;
                 s_r = fmaf(sel, c.w.w, s_r);
                 s_g = fmaf(sel, c.f.x, s_g);
                 s_b = fmaf(sel, c.f.y, s_b);
               }
-              return false;
+              return on;
             });
             while (kSh && amb != 0ull) {
               const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(amb));
@@ -1803,13 +1826,39 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
                   s_r = s_r + w.w;
                   s_g = s_g + f.x;
                   s_b = s_b + f.y;
+                  m |= 1ull << j;
                 }
+              }
+            }
+            rows32[2 * lane] = static_cast<uint32_t>(m);
+            rows32[2 * lane + 1] = static_cast<uint32_t>(m >> 32);
+            __syncthreads();
+            // queries whose set holds some photon of the chunk (only
+            // contributing lanes flag any)
+            const uint64_t held = __ballot(m != 0ull);
+            if (held != 0ull) {
+              float a_r = 0.f, a_g = 0.f, a_b = 0.f;
+              for (uint64_t hq = held; hq != 0ull; hq &= hq - 1ull) {
+                const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(hq));
+                // the half of row q that holds this lane's bit
+                const uint32_t word = rows32[2 * q + half];
+                const float4 C = qcoef[q];
+                const float sel = static_cast<float>((word >> (lane & 31)) & 1u);
+                a_r = fmaf(sel, C.x, a_r);
+                a_g = fmaf(sel, C.y, a_g);
+                a_b = fmaf(sel, C.z, a_b);
+              }
+              if (static_cast<uint32_t>(lane) < cnt && (a_r != 0.0f || a_g != 0.0f || a_b != 0.0f)) {
+                const uint32_t slot = sh.b.sb.chunk_slot[lane];
+                atomicAdd(&dflux[slot], a_r);
+                atomicAdd(&dflux[n + slot], a_g);
+                atomicAdd(&dflux[2 * n + slot], a_b);
               }
             }
           });
         };
-        if (shared) pass1(std::true_type{});
-        else pass1(std::false_type{});
+        if (shared) sweep(std::true_type{});
+        else sweep(std::false_type{});
         float dr = 0.f, dg = 0.f, db = 0.f, fr = 0.f, fgv = 0.f, fb = 0.f;
         if (contributes) {
           const float inv = 1.0f / (kPiF * denom);
@@ -1824,93 +1873,6 @@ __device__ void process_group(DescRef D, int64_t lg, const uint32_t* __restrict_
         }
         add_albedo_grad(drho, D.dl_dalbedo, contributes ? mat : -1, dr, dg, db, lane);
         add_albedo_grad(drho, D.dl_dalbedo, contributes ? fgm : -1, fr, fgv, fb, lane);
-
-        // (2) per photon, one lane each: sum of the coefficients of the
-        // queries whose set holds it (the same FP32 key test as above), then
-        // one atomic per channel.  The contributing queries are read from
-        // LDS (the lane histograms are free here).
-        float4* qa = reinterpret_cast<float4*>(sh.hist);
-        float4* qb = qa + kWave;
-        float4* qc = qb + kWave;
-        static_assert(3 * kWave * 4 <= (kBins + 2) * kWave, "query staging fits the bins");
-        // The contributing queries are packed in lane order and padded to a
-        // multiple of four with entries whose coefficients are zero (adding
-        // sel * 0 leaves a sum unchanged), so the loop below reads four
-        // queries per step.
-        const uint64_t cm = __ballot(contributes);
-        const uint32_t ncm = static_cast<uint32_t>(__popcll(cm));
-        const uint32_t npad = (ncm + 3u) & ~3u;
-        {
-          uint32_t j = kWave;
-          if (contributes) j = lanes_below(cm, lane);
-          else if (ncm + lanes_below(~cm, lane) < npad) j = ncm + lanes_below(~cm, lane);
-          if (j < static_cast<uint32_t>(kWave)) {
-            qa[j] = contributes ? make_float4(qx, qy, qz, __uint_as_float(t_hi))
-                                : make_float4(0.f, 0.f, 0.f, 0.f);
-            qb[j] = contributes ? make_float4(nx, ny, nz, __uint_as_float(t_lo))
-                                : make_float4(0.f, 0.f, 0.f, 0.f);
-            qc[j] = contributes ? make_float4(c_r, c_g, c_b, 0.0f)
-                                : make_float4(0.f, 0.f, 0.f, 0.f);
-          }
-        }
-        __syncthreads();
-        if (cm != 0ull) {
-          const float4* __restrict__ P = D.pos4;
-          const float4* __restrict__ W = D.wif4;
-          float* __restrict__ dflux = D.dl_dflux;
-          const int64_t n = D.n;
-          uint32_t r = 0;
-          for (uint32_t base = 0; base < total; base += kWave) {
-            const uint32_t t = base + static_cast<uint32_t>(lane);
-            const bool ok = t < total;
-            float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p;
-            int32_t slot = 0;
-            if (ok) {
-              slot = list_slot(D, sh, nr, t, r);
-              p = P[slot];
-              w = W[slot];
-            }
-            const uint32_t idx = __float_as_uint(p.w);
-            // the photon's sign under the shared normal, when it settles it
-            bool per_query = true, inc = false;
-            if (shared) {
-              const float d0 = dot_ref(w.x, w.y, w.z, sn.x, sn.y, sn.z);
-              const float mw = (fabsf(w.x) + fabsf(w.y) + fabsf(w.z)) * sn.m + 1e-30f;
-              per_query = !(fabsf(d0) > mw);
-              inc = !per_query && d0 > 0.0f;
-            }
-            float a_r = 0.f, a_g = 0.f, a_b = 0.f;
-            auto queries = [&](auto test_tag) {
-              constexpr bool kTest = decltype(test_tag)::value;
-              for (uint32_t i0 = 0; i0 < npad; i0 += 4) {
-#pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) {
-                  const float4 A = qa[i0 + u];
-                  const float4 B = qb[i0 + u];
-                  const float4 C = qc[i0 + u];
-                  const float d2 = d2_ref(A.x, A.y, A.z, p.x, p.y, p.z);
-                  const uint32_t db = __float_as_uint(d2);
-                  const uint32_t th = __float_as_uint(A.w);
-                  const bool nb = db < th || (db == th && idx <= __float_as_uint(B.w));
-                  bool on = nb && inc;
-                  if (kTest && per_query)
-                    on = nb && !(dot_ref(w.x, w.y, w.z, B.x, B.y, B.z) < 0.0f);
-                  const float sel = on ? 1.0f : 0.0f;
-                  a_r = fmaf(sel, C.x, a_r);
-                  a_g = fmaf(sel, C.y, a_g);
-                  a_b = fmaf(sel, C.z, a_b);
-                }
-              }
-            };
-            if (__ballot(ok && per_query) != 0ull) queries(std::true_type{});
-            else if (__ballot(ok && inc) != 0ull) queries(std::false_type{});
-            if (ok && (a_r != 0.0f || a_g != 0.0f || a_b != 0.0f)) {
-              atomicAdd(&dflux[slot], a_r);
-              atomicAdd(&dflux[n + slot], a_g);
-              atomicAdd(&dflux[2 * n + slot], a_b);
-            }
-          }
-        }
         __syncthreads();
       }
       pt.mark(6, lane);

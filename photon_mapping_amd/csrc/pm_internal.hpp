@@ -236,6 +236,20 @@ struct Context {
   DevBuf ws_stage_in, ws_stage_out, ws_stage_aux;
   DevBuf ws_tmp;
   DevBuf ws_grad;  // adjoint: per-photon gradients in slot order (3 n floats)
+  // The query order of this context's last single-instance gather on device
+  // queries (its permutation, in ws.vals or ws.vals2, lasts until the next
+  // sort or map build of the context, which clear this record).  An adjoint
+  // with the forward keys on the same queries, map and stream takes it
+  // instead of sorting the queries again; its results do not depend on the
+  // order, only its speed does.
+  struct QueryOrder {
+    const void* q = nullptr;
+    int64_t m = 0;
+    const void* map = nullptr;
+    hipStream_t s = nullptr;
+    const uint32_t* perm = nullptr;
+  };
+  QueryOrder last_order;
   // descriptor staging: pinned host arena mirrored into a device arena;
   // `stage_done` is recorded after each upload so the host side is only
   // rewritten once the previous copy has completed.

@@ -68,6 +68,45 @@ def test_adjoint_reused_keys_on_fresh_map(gpu, oracle, rng):
     np.testing.assert_allclose(drho, er, rtol=1e-4, atol=1e-6 * np.abs(er).max())
 
 
+def test_adjoint_takes_the_forward_order(gpu, oracle, rng):
+    """With keys on device queries, the adjoint takes the query order of the
+    context's last gather on the same query pointer, map and stream instead
+    of sorting again.  Its results do not depend on that order: the queries
+    under the pointer are replaced (shuffled) after the forward call that set
+    the order, their keys come from a second context, and the gradients
+    still match the oracle's; a call that sorted afresh agrees too."""
+    import torch
+
+    pos, wi, flux, q1, mats, pmap = _setup(gpu, rng)
+    k, n_emit = 64, 3e4
+    dev = torch.device("cuda:0")
+    g = torch.from_numpy(rng.normal(size=(len(q1), 3)).astype(np.float32)).to(dev)
+    q2 = q1[rng.permutation(len(q1))]
+    print("adjoint, forward order taken over replaced queries", flush=True)
+    tq = torch.from_numpy(q1).to(dev)
+    pmap.gather_radiance(tq, mats, k, n_emit)  # sets the order of q1
+    other = gpu.Context()
+    omap = gpu.PhotonMap(other)
+    try:
+        omap.set_photons_soa(pos, wi, flux)
+        omap.build()
+        _, _, key2 = omap.gather_radiance(torch.from_numpy(q2).to(dev), mats, k, n_emit)
+        torch.cuda.synchronize()
+        tq.copy_(torch.from_numpy(q2).to(dev))  # same pointer, other queries
+        taken = [t.cpu().numpy() for t in pmap.gather_adjoint(tq, mats, k, n_emit, g, key=key2)]
+        # a gather of other queries in between: the next call sorts afresh
+        pmap.gather_radiance(torch.from_numpy(q1).to(dev), mats, k, n_emit)
+        fresh = [t.cpu().numpy() for t in pmap.gather_adjoint(tq, mats, k, n_emit, g, key=key2)]
+    finally:
+        del omap  # (its map goes before its context)
+        other.close()
+    bidx, br2 = oracle.brute_knn(pos, q2[:, :3], k)
+    ef, er = oracle.adjoint_fp64(wi, flux, q2, mats, bidx, br2, n_emit, g.cpu().numpy())
+    for dflux, drho in (taken, fresh):
+        np.testing.assert_allclose(dflux, ef, rtol=1e-4, atol=1e-5 * np.abs(ef).max())
+        np.testing.assert_allclose(drho, er, rtol=1e-4, atol=1e-6 * np.abs(er).max())
+
+
 def test_adjoint_directional_finite_difference(gpu, rng):
     """<dL/dflux, v> against (L(flux + h v) - L(flux - h v)) / 2h on the GPU
     forward pass, and the same for the albedos (target: 1e-3 relative)."""

@@ -635,11 +635,16 @@ __global__ __launch_bounds__(kThreads) void resolve_kernel(const unsigned long l
                                                            float* __restrict__ mean,
                                                            float* __restrict__ var,
                                                            unsigned long long* dropped) {
-  const uint32_t p = blockIdx.x * kThreads + threadIdx.x;
-  if (p >= n_pix) return;
+  // One wave per pixel: lane l takes samples l, l + 64, ... of it, so a
+  // wave's loads cover consecutive samples, and the sums are then reduced
+  // over the wave in a fixed order.
+  constexpr int kW = 64;
+  const int lane = static_cast<int>(threadIdx.x) & (kW - 1);
+  const uint32_t p = blockIdx.x * (kThreads / kW) + threadIdx.x / kW;
+  if (p >= n_pix) return;  // (the whole wave)
   double s[3] = {0.0, 0.0, 0.0}, q[3] = {0.0, 0.0, 0.0};
   uint32_t nd = 0;
-  for (int k = 0; k < spp; ++k) {
+  for (int k = lane; k < spp; k += kW) {
     const size_t si = static_cast<size_t>(p) * spp + k;
     if (bad[si]) {
       ++nd;
@@ -651,6 +656,14 @@ __global__ __launch_bounds__(kThreads) void resolve_kernel(const unsigned long l
       q[c] += x * x;
     }
   }
+  for (int o = kW / 2; o >= 1; o >>= 1) {
+    for (int c = 0; c < 3; ++c) {
+      s[c] += __shfl_xor(s[c], o, kW);
+      q[c] += __shfl_xor(q[c], o, kW);
+    }
+    nd += __shfl_xor(nd, o, kW);
+  }
+  if (lane != 0) return;
   const int64_t at = (pix0 + p) * 3;
   for (int c = 0; c < 3; ++c) {
     const double m = s[c] / spp;
@@ -678,6 +691,7 @@ int zero_words(void* p, size_t bytes) {
 }
 
 unsigned grid_of(uint64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
+// (resolve_kernel takes one wave per pixel: grid_of(64 * pixels))
 
 int check_params(const pmx_scene* scene, const pmx_camera* cam, const pmx_render_params* p,
                  float* out_mean) {
@@ -898,7 +912,8 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
       PM_HIP(hipGetLastError());
       (map_i == 0 ? st.gathers_global : st.gathers_caustics) += static_cast<int64_t>(nq);
     }
-    hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(np)), dim3(kThreads), 0, nullptr,
+    hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(static_cast<uint64_t>(np) * 64)),
+                       dim3(kThreads), 0, nullptr,
                        acc.as<unsigned long long>(), bad.as<uint32_t>(), p->spp,
                        static_cast<uint32_t>(np), pix0, img.d_mean, img.d_var, dropped);
     PM_HIP(hipGetLastError());
@@ -951,7 +966,8 @@ int render_pt(const pmx_scene* scene, const pmx_camera* cam, const pmx_render_pa
                        nullptr, R, *cam,
                        pix0 * p->spp, ns, acc.as<unsigned long long>(), bad.as<uint32_t>(), rays);
     PM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(np)), dim3(kThreads), 0, nullptr,
+    hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(static_cast<uint64_t>(np) * 64)),
+                       dim3(kThreads), 0, nullptr,
                        acc.as<unsigned long long>(), bad.as<uint32_t>(), p->spp,
                        static_cast<uint32_t>(np), pix0, img.d_mean, img.d_var, dropped);
     PM_HIP(hipGetLastError());

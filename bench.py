@@ -751,7 +751,7 @@ def main() -> int:
         tracing = {"ms": 1e3 * dt, "stored_photons_per_s": args.photons / dt,
                    "paths_per_s": n_emit / dt, "emitted_paths": int(n_emit),
                    "note": "counter-based paths traced once, each path's first 16 photons "
-                           "cached and then placed (paths that store more traced again); "
+                           "cached, the photons past them recorded, both then placed; "
                            "BVH closest hit"}
     q = sc.camera_queries(ns, args.width, args.height, args.spp, depth=1, seed=0)
     torch.cuda.synchronize()

@@ -112,6 +112,11 @@ typedef struct pm_stats {
 #define PM_OPT_XCD_AWARE 5      /* 0/1: per-XCD work queues (default 1)     */
 #define PM_OPT_COUNT_STATS 6    /* 0/1: count fallbacks/refinements         */
 #define PM_OPT_RUN_SPLIT 7      /* query-run split factor x1000 (def. 900)  */
+#define PM_OPT_TRACE_RECORDS 8  /* photon tracing: most photons past a path's
+                                 * cache kept from the count pass, 0..2^30
+                                 * (default 262144); when more are made, or
+                                 * with 0, the paths that overflowed their
+                                 * cache are traced again                   */
 
 /* ---- errors / version ----------------------------------------------------- */
 /* Thread-local text of the last failing call ("" when none). */

@@ -38,6 +38,7 @@ PM_OPT_LEAF_SIZE = 4
 PM_OPT_XCD_AWARE = 5
 PM_OPT_COUNT_STATS = 6
 PM_OPT_RUN_SPLIT = 7
+PM_OPT_TRACE_RECORDS = 8
 
 PM_NOALLOC_MAX_QUERIES = 1 << 20
 PM_NOALLOC_MAX_ANCHORS = 1 << 18

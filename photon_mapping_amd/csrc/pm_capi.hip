@@ -569,6 +569,11 @@ int pm_context_set_option(pm_context* h, int option, int64_t value) {
       if (value < 1 || value > 1000000) return fail(PM_ERR_INVALID_ARG, "run split in 1..1000000");
       ctx->run_split = static_cast<float>(value) / 1000.0f;
       break;
+    case PM_OPT_TRACE_RECORDS:
+      if (value < 0 || value > (int64_t{1} << 30))
+        return fail(PM_ERR_INVALID_ARG, "trace records in 0..2^30");
+      ctx->trace_records = value;
+      break;
     default: return fail(PM_ERR_INVALID_ARG, "unknown option");
   }
   return PM_OK;

@@ -226,6 +226,7 @@ struct Context {
   int leaf_size = 8;
   int xcd_aware = 1;
   float run_split = 0.9f;
+  int64_t trace_records = int64_t{1} << 18;  // photon tracing's overflow records
   uint64_t batch_epoch = 0;  // pins the anchor sets of the batch being prepared
   int timing = 0;
   int count_stats = 0;

@@ -150,6 +150,9 @@ __global__ __launch_bounds__(kThreads) void surrogate_photons_kernel(
 
 // A photon in the count pass's cache: position, wi, flux.
 constexpr int kCacheFloats = 9;
+// an overflow record: the photon's 9 floats, its place in its path, and the
+// path (two words)
+constexpr int kOvfFloats = 12;
 
 // One photon path per lane.  COUNT: photons stored per path into counts, and
 // (with a cache) the path's first cache_cap photons into its cache slots;
@@ -162,7 +165,8 @@ __global__ __launch_bounds__(kThreads) void trace_kernel(
     SceneDev S, int64_t path0, int64_t n_paths, uint64_t seed, int max_depth,
     uint32_t* __restrict__ counts, const int64_t* __restrict__ offsets, int64_t n_store,
     float* __restrict__ pos, float* __restrict__ wi, float* __restrict__ flux,
-    float* __restrict__ cache, int cache_cap, const int64_t* __restrict__ path_list) {
+    float* __restrict__ cache, int cache_cap, const int64_t* __restrict__ path_list,
+    float* __restrict__ ovf, uint32_t* __restrict__ ovf_n, uint32_t ovf_cap) {
   extern __shared__ int stk_all[];  // SceneImpl::stack_bytes
   int* stk = stk_all + threadIdx.x;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
@@ -215,6 +219,19 @@ __global__ __launch_bounds__(kThreads) void trace_kernel(
           c[0] = x.x; c[1] = x.y; c[2] = x.z;
           c[3] = wo.x; c[4] = wo.y; c[5] = wo.z;
           c[6] = tr_r; c[7] = tr_g; c[8] = tr_b;
+        } else if (ovf) {
+          // past the cache: a record of the photon, its path and its place
+          // in the path, appended to the overflow area
+          const uint32_t r = atomicAdd(ovf_n, 1u);
+          if (r < ovf_cap) {
+            float* c = ovf + static_cast<int64_t>(r) * kOvfFloats;
+            c[0] = x.x; c[1] = x.y; c[2] = x.z;
+            c[3] = wo.x; c[4] = wo.y; c[5] = wo.z;
+            c[6] = tr_r; c[7] = tr_g; c[8] = tr_b;
+            c[9] = __uint_as_float(stored);
+            c[10] = __uint_as_float(static_cast<uint32_t>(path));
+            c[11] = __uint_as_float(static_cast<uint32_t>(path >> 32));
+          }
         }
         ++stored;
       }
@@ -263,6 +280,26 @@ __global__ __launch_bounds__(kThreads) void cache_scatter_kernel(
   flux[at] = c[6]; flux[n_store + at] = c[7]; flux[2 * n_store + at] = c[8];
 }
 
+// The overflow records to their places: record r holds photon `place` of
+// its path; it goes to offsets[path] + place when the path is one of the
+// n_paths emitted and the place is below n_store.
+__global__ __launch_bounds__(kThreads) void ovf_scatter_kernel(
+    const float* __restrict__ ovf, int64_t n_rec, const int64_t* __restrict__ offsets,
+    int64_t n_paths, int64_t n_store, float* __restrict__ pos, float* __restrict__ wi,
+    float* __restrict__ flux) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (r >= n_rec) return;
+  const float* c = ovf + r * kOvfFloats;
+  const uint64_t path = static_cast<uint64_t>(__float_as_uint(c[10])) |
+                        (static_cast<uint64_t>(__float_as_uint(c[11])) << 32);
+  if (path >= static_cast<uint64_t>(n_paths)) return;
+  const int64_t at = offsets[path] + static_cast<int64_t>(__float_as_uint(c[9]));
+  if (at >= n_store) return;
+  pos[at] = c[0]; pos[n_store + at] = c[1]; pos[2 * n_store + at] = c[2];
+  wi[at] = c[3]; wi[n_store + at] = c[4]; wi[2 * n_store + at] = c[5];
+  flux[at] = c[6]; flux[n_store + at] = c[7]; flux[2 * n_store + at] = c[8];
+}
+
 }  // namespace
 
 // ---- host side --------------------------------------------------------------------
@@ -276,17 +313,18 @@ uint64_t kind_seed(uint64_t seed, int kind) {
 }
 
 int launch_trace_count(const SceneImpl* s, int kind, int64_t path0, int64_t n, uint64_t seed,
-                       int max_depth, uint32_t* counts, float* cache, int cap) {
+                       int max_depth, uint32_t* counts, float* cache, int cap, float* ovf,
+                       uint32_t* ovf_n, uint32_t ovf_cap) {
   const dim3 grid(static_cast<unsigned>((n + kThreads - 1) / kThreads)), block(kThreads);
   const SceneDev S = s->dev(0);
   if (kind == PMX_PHOTONS_CAUSTICS)
     hipLaunchKernelGGL((trace_kernel<true, true>), grid, block, s->stack_bytes(0), nullptr, S, path0, n, seed,
                        max_depth, counts, nullptr, int64_t{0}, nullptr, nullptr, nullptr, cache,
-                       cap, nullptr);
+                       cap, nullptr, ovf, ovf_n, ovf_cap);
   else
     hipLaunchKernelGGL((trace_kernel<true, false>), grid, block, s->stack_bytes(0), nullptr, S, path0, n, seed,
                        max_depth, counts, nullptr, int64_t{0}, nullptr, nullptr, nullptr, cache,
-                       cap, nullptr);
+                       cap, nullptr, ovf, ovf_n, ovf_cap);
   PM_HIP(hipGetLastError());
   return PM_OK;
 }
@@ -301,11 +339,11 @@ int launch_trace_write(const SceneImpl* s, int kind, int64_t n_paths, uint64_t s
   if (kind == PMX_PHOTONS_CAUSTICS)
     hipLaunchKernelGGL((trace_kernel<false, true>), grid, block, s->stack_bytes(0), nullptr, S, int64_t{0},
                        n_paths, seed, max_depth, nullptr, offsets, n_store, pos, wi, flux, nullptr,
-                       0, path_list);
+                       0, path_list, nullptr, nullptr, 0u);
   else
     hipLaunchKernelGGL((trace_kernel<false, false>), grid, block, s->stack_bytes(0), nullptr, S, int64_t{0},
                        n_paths, seed, max_depth, nullptr, offsets, n_store, pos, wi, flux, nullptr,
-                       0, path_list);
+                       0, path_list, nullptr, nullptr, 0u);
   PM_HIP(hipGetLastError());
   return PM_OK;
 }
@@ -350,11 +388,28 @@ int grow_keep(DevBuf& buf, size_t need, size_t keep) {
 int count_paths(const SceneImpl* s, int kind, uint64_t seed, int max_depth, int64_t stop_store,
                 int64_t stop_paths, std::vector<int64_t>& offs, int64_t* acc_out,
                 std::deque<CountChunk>* chunks = nullptr,
-                std::vector<int64_t>* overflow = nullptr, bool* cached = nullptr) {
+                std::vector<int64_t>* overflow = nullptr, bool* cached = nullptr,
+                int64_t* ovf_recs = nullptr) {
   DevBuf dcount;              // counts of a chunk that is not cached
   std::vector<uint32_t> cnt;  // grown to the largest chunk
   const int cap = cache_cap_of(kind);
   bool caching = chunks != nullptr && overflow != nullptr;
+  // the overflow area: a counter word, then the records (a caustic path
+  // never stores past its cache of one)
+  if (ovf_recs) *ovf_recs = -1;
+  float* ovf = nullptr;
+  uint32_t* ovf_n = nullptr;
+  // (the context's PM_OPT_TRACE_RECORDS records at most; when more are made
+  // the paths that overflowed their cache are traced again instead)
+  const uint32_t ovf_cap = static_cast<uint32_t>(s->ctx->trace_records);
+  if (caching && ovf_recs && kind != PMX_PHOTONS_CAUSTICS && ovf_cap > 0) {
+    if (int rc = s->trace_ovf.reserve(256 + static_cast<size_t>(ovf_cap) * kOvfFloats *
+                                                sizeof(float)))
+      return rc;
+    ovf_n = s->trace_ovf.as<uint32_t>();
+    ovf = reinterpret_cast<float*>(static_cast<char*>(s->trace_ovf.ptr) + 256);
+    if (int rc = launch_fill_u32(ovf_n, 0u, 1, nullptr)) return rc;
+  }
   int64_t counts_used = 0, cache_used = 0;  // words / floats of the scene's buffers in use
   int64_t acc = 0, paths = 0;
   int64_t chunk = kMinChunk;
@@ -397,7 +452,8 @@ int count_paths(const SceneImpl* s, int kind, uint64_t seed, int max_depth, int6
       if (int rc = dcount.reserve(chunk * sizeof(uint32_t))) return rc;
       dc = dcount.as<uint32_t>();
     }
-    if (int rc = launch_trace_count(s, kind, paths, chunk, seed, max_depth, dc, dcache, cap))
+    if (int rc = launch_trace_count(s, kind, paths, chunk, seed, max_depth, dc, dcache, cap,
+                                    dcache ? ovf : nullptr, ovf_n, ovf_cap))
       return rc;
     if (cnt.size() < static_cast<size_t>(chunk)) cnt.resize(static_cast<size_t>(chunk));
     PM_HIP(hipMemcpy(cnt.data(), dc, chunk * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -419,17 +475,27 @@ int count_paths(const SceneImpl* s, int kind, uint64_t seed, int max_depth, int6
   }
   *acc_out = acc;
   if (cached) *cached = caching;
+  if (caching && ovf_n) {
+    // every photon past a cache has its record: no path is traced again
+    uint32_t n_rec = 0;
+    PM_HIP(hipMemcpy(&n_rec, ovf_n, sizeof(n_rec), hipMemcpyDeviceToHost));
+    if (n_rec <= ovf_cap) {
+      *ovf_recs = n_rec;
+      overflow->clear();
+    }
+  }
   return PM_OK;
 }
 
 // The write pass after a count pass: the cached photons of every chunk to
-// their places, then the paths that stored more than the cache holds traced
-// again (all their photons written, the cached ones to the same values); or,
-// without a cache, every path traced again.
+// their places, then the count pass's overflow records (ovf_recs >= 0), or
+// the paths that stored more than the cache holds traced again (all their
+// photons written, the cached ones to the same values); or, without a
+// cache, every path traced again.
 int write_paths(const SceneImpl* s, int kind, uint64_t seed, int max_depth,
                 const std::vector<int64_t>& offs, const int64_t* doff, bool cached,
                 const std::deque<CountChunk>& chunks, const std::vector<int64_t>& overflow,
-                int64_t n_store, float* pos, float* wi, float* flux) {
+                int64_t n_store, float* pos, float* wi, float* flux, int64_t ovf_recs) {
   const int64_t paths = static_cast<int64_t>(offs.size());
   if (!cached)
     return launch_trace_write(s, kind, paths, seed, max_depth, doff, n_store, pos, wi, flux);
@@ -441,6 +507,15 @@ int write_paths(const SceneImpl* s, int kind, uint64_t seed, int max_depth,
                        dim3(kThreads), 0, nullptr, s->trace_cache.as<float>() + cc.cache_at, cap,
                        cc.n, s->trace_counts.as<uint32_t>() + cc.count_at, doff + cc.path0,
                        n_store, pos, wi, flux);
+    PM_HIP(hipGetLastError());
+  }
+  if (ovf_recs > 0) {
+    hipLaunchKernelGGL(ovf_scatter_kernel,
+                       dim3(static_cast<unsigned>((ovf_recs + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, nullptr,
+                       reinterpret_cast<const float*>(static_cast<const char*>(s->trace_ovf.ptr) +
+                                                      256),
+                       ovf_recs, doff, paths, n_store, pos, wi, flux);
     PM_HIP(hipGetLastError());
   }
   if (!overflow.empty()) {
@@ -641,8 +716,9 @@ int pmx_trace_photons(const pmx_scene* scene, int kind, int64_t n_store, int max
   std::vector<int64_t> over;
   bool cached = false;
   int64_t acc = 0;
+  int64_t ovf_recs = -1;
   if (int rc = count_paths(s, kind, kseed, max_depth, n_store, -1, offs, &acc, &chunks, &over,
-                           &cached))
+                           &cached, &ovf_recs))
     return rc;
   // the path holding photon n_store - 1 is the last one emitted
   const int64_t paths = static_cast<int64_t>(offs.size());
@@ -652,7 +728,7 @@ int pmx_trace_photons(const pmx_scene* scene, int kind, int64_t n_store, int max
   if (int rc = doff.reserve(offs.size() * sizeof(int64_t))) return rc;
   PM_HIP(hipMemcpy(doff.ptr, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   if (int rc = write_paths(s, kind, kseed, max_depth, offs, doff.as<int64_t>(), cached, chunks,
-                           over, n_store, pos_soa, wi_soa, flux_soa))
+                           over, n_store, pos_soa, wi_soa, flux_soa, ovf_recs))
     return rc;
   PM_HIP(hipDeviceSynchronize());
   return PM_OK;
@@ -674,9 +750,10 @@ int pmx_trace_photon_map(const pmx_scene* scene, int kind, int64_t n_paths, int 
   std::vector<int64_t> over;
   bool cached = false;
   int64_t total = 0;
+  int64_t ovf_recs = -1;
   if (n_paths > 0)
     if (int rc = count_paths(s, kind, kseed, max_depth, -1, n_paths, offs, &total, &chunks, &over,
-                             &cached))
+                             &cached, &ovf_recs))
       return rc;
   if (total > (int64_t{1} << 31) - 1) return fail(PM_ERR_INVALID_ARG, "more than 2^31 photons stored");
   *n_stored = total;
@@ -689,7 +766,8 @@ int pmx_trace_photon_map(const pmx_scene* scene, int kind, int64_t n_paths, int 
     if (int rc = dflux.reserve(bytes)) return rc;
     PM_HIP(hipMemcpy(doff.ptr, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     if (int rc = write_paths(s, kind, kseed, max_depth, offs, doff.as<int64_t>(), cached, chunks,
-                             over, total, dpos.as<float>(), dwi.as<float>(), dflux.as<float>()))
+                             over, total, dpos.as<float>(), dwi.as<float>(), dflux.as<float>(),
+                             ovf_recs))
       return rc;
     PM_HIP(hipDeviceSynchronize());
   }

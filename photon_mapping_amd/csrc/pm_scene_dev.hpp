@@ -324,10 +324,11 @@ struct SceneImpl {
   double sah_cost = 0.0, light_area = 0.0;
   bool has_ior = false;
   DevBuf tris, tri4, nodes, mats, ior, lights, le;
-  // Photon tracing's count pass keeps each path's count and first photons
-  // here (grown as needed, reused by later calls on the scene; calls on one
-  // context are externally synchronised).
-  mutable DevBuf trace_counts, trace_cache;
+  // Photon tracing's count pass keeps each path's count and first photons,
+  // and records of the photons past them, here (grown as needed, reused by
+  // later calls on the scene; calls on one context are externally
+  // synchronised).
+  mutable DevBuf trace_counts, trace_cache, trace_ovf;
 
   // LDS of a traversal stack for a workgroup of ray kernels: a ray pushes at
   // most one entry per internal level (bvh_depth + 1 of them; the builder

@@ -118,11 +118,11 @@ def test_trace_closed_box_energy_and_count(gpu, rho):
 def test_trace_photons_match_the_photon_map(gpu, rho):
     """pmx_trace_photons stops at n stored photons and pmx_trace_photon_map
     emits a given number of paths; they count their paths in chunks of other
-    sizes.  The count pass caches each path's first 16 photons, which are
-    then moved to their places, and a path that stored more is traced again.
+    sizes.  The count pass caches each path's first 16 photons and keeps a
+    record of each photon past them; both are then moved to their places.
     Photon i must be the same photon, bit for bit, either way.  In the closed
     box at albedo 0.5 most paths store under 16 photons (cached); at 0.9 every
-    path stores more than 18 before the roulette starts (traced again)."""
+    path stores more than 18 before the roulette starts (records)."""
     from photon_mapping_amd import scene as sc
 
     s = _box(rho)
@@ -138,3 +138,31 @@ def test_trace_photons_match_the_photon_map(gpu, rho):
         got = np.array([p.position, p.wi, p.throughput], np.float32)
         want = np.stack([pos[i], wi[i], flux[i]]).astype(np.float32)
         assert (got.view(np.uint32) == want.view(np.uint32)).all(), (i, got, want)
+
+
+@pytest.mark.parametrize("rho", [0.5, 0.9])
+def test_trace_overflow_records_match_tracing_again(gpu, rho):
+    """Past a path's cache of 16 photons, the count pass keeps a record of
+    each photon (its path and its place in the path) and places it from
+    there.  With no records allowed, or too few for the photons made, the
+    paths that overflowed their cache are traced again instead.  The photon
+    arrays must be the same bit for bit every way."""
+    from photon_mapping_amd import _lib
+    from photon_mapping_amd import scene as sc
+
+    ctx = gpu.Context.default()
+    ns = sc.NativeScene(_box(rho), ctx)
+    n = 120_000
+    outs = []
+    try:
+        for recs in (1 << 18, 0, 64):
+            print(f"trace records {recs}, rho={rho}", flush=True)
+            ctx.set_option(_lib.PM_OPT_TRACE_RECORDS, recs)
+            pos, wi, flux, n_emit = sc.trace_photons(ns, n, seed=7)
+            outs.append((n_emit, [x.cpu().numpy() for x in (pos, wi, flux)]))
+    finally:
+        ctx.set_option(_lib.PM_OPT_TRACE_RECORDS, 1 << 18)
+    for n_emit, arrs in outs[1:]:
+        assert n_emit == outs[0][0]
+        for a, b in zip(arrs, outs[0][1]):
+            assert (a.view(np.uint32) == b.view(np.uint32)).all()

@@ -634,17 +634,18 @@ __global__ __launch_bounds__(kThreads) void resolve_kernel(const unsigned long l
                                                            int spp, uint32_t n_pix, int64_t pix0,
                                                            float* __restrict__ mean,
                                                            float* __restrict__ var,
-                                                           unsigned long long* dropped) {
-  // One wave per pixel: lane l takes samples l, l + 64, ... of it, so a
-  // wave's loads cover consecutive samples, and the sums are then reduced
-  // over the wave in a fixed order.
-  constexpr int kW = 64;
-  const int lane = static_cast<int>(threadIdx.x) & (kW - 1);
-  const uint32_t p = blockIdx.x * (kThreads / kW) + threadIdx.x / kW;
-  if (p >= n_pix) return;  // (the whole wave)
+                                                           unsigned long long* dropped,
+                                                           int lanes) {
+  // `lanes` lanes per pixel (a power of two <= 64, the least >= spp when spp
+  // is below 64): lane l of a pixel takes its samples l, l + lanes, ..., so
+  // a wave's loads cover consecutive samples, and the sums are then reduced
+  // over the pixel's lanes in a fixed order.
+  const int lane = static_cast<int>(threadIdx.x) & (lanes - 1);
+  const uint32_t p = (blockIdx.x * kThreads + threadIdx.x) / static_cast<uint32_t>(lanes);
+  if (p >= n_pix) return;  // (all lanes of the pixel)
   double s[3] = {0.0, 0.0, 0.0}, q[3] = {0.0, 0.0, 0.0};
   uint32_t nd = 0;
-  for (int k = lane; k < spp; k += kW) {
+  for (int k = lane; k < spp; k += lanes) {
     const size_t si = static_cast<size_t>(p) * spp + k;
     if (bad[si]) {
       ++nd;
@@ -656,12 +657,14 @@ __global__ __launch_bounds__(kThreads) void resolve_kernel(const unsigned long l
       q[c] += x * x;
     }
   }
-  for (int o = kW / 2; o >= 1; o >>= 1) {
+  // (a pixel's lanes are aligned within the wave, so the xor offsets below
+  // stay inside them; the pixels of a wave end together)
+  for (int o = lanes / 2; o >= 1; o >>= 1) {
     for (int c = 0; c < 3; ++c) {
-      s[c] += __shfl_xor(s[c], o, kW);
-      q[c] += __shfl_xor(q[c], o, kW);
+      s[c] += __shfl_xor(s[c], o, 64);
+      q[c] += __shfl_xor(q[c], o, 64);
     }
-    nd += __shfl_xor(nd, o, kW);
+    nd += __shfl_xor(nd, o, 64);
   }
   if (lane != 0) return;
   const int64_t at = (pix0 + p) * 3;
@@ -691,7 +694,12 @@ int zero_words(void* p, size_t bytes) {
 }
 
 unsigned grid_of(uint64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
-// (resolve_kernel takes one wave per pixel: grid_of(64 * pixels))
+// Lanes resolve_kernel gives a pixel of spp samples.
+int resolve_lanes(int spp) {
+  int l = 1;
+  while (l < spp && l < 64) l <<= 1;
+  return l;
+}
 
 int check_params(const pmx_scene* scene, const pmx_camera* cam, const pmx_render_params* p,
                  float* out_mean) {
@@ -912,10 +920,11 @@ int render_pm(const pmx_scene* scene, const pmx_camera* cam, pm_photonmap* globa
       PM_HIP(hipGetLastError());
       (map_i == 0 ? st.gathers_global : st.gathers_caustics) += static_cast<int64_t>(nq);
     }
-    hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(static_cast<uint64_t>(np) * 64)),
-                       dim3(kThreads), 0, nullptr,
-                       acc.as<unsigned long long>(), bad.as<uint32_t>(), p->spp,
-                       static_cast<uint32_t>(np), pix0, img.d_mean, img.d_var, dropped);
+    hipLaunchKernelGGL(resolve_kernel,
+                       dim3(grid_of(static_cast<uint64_t>(np) * resolve_lanes(p->spp))),
+                       dim3(kThreads), 0, nullptr, acc.as<unsigned long long>(),
+                       bad.as<uint32_t>(), p->spp, static_cast<uint32_t>(np), pix0, img.d_mean,
+                       img.d_var, dropped, resolve_lanes(p->spp));
     PM_HIP(hipGetLastError());
   }
   unsigned long long tail[1 + kRayShards];
@@ -966,10 +975,11 @@ int render_pt(const pmx_scene* scene, const pmx_camera* cam, const pmx_render_pa
                        nullptr, R, *cam,
                        pix0 * p->spp, ns, acc.as<unsigned long long>(), bad.as<uint32_t>(), rays);
     PM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(resolve_kernel, dim3(grid_of(static_cast<uint64_t>(np) * 64)),
-                       dim3(kThreads), 0, nullptr,
-                       acc.as<unsigned long long>(), bad.as<uint32_t>(), p->spp,
-                       static_cast<uint32_t>(np), pix0, img.d_mean, img.d_var, dropped);
+    hipLaunchKernelGGL(resolve_kernel,
+                       dim3(grid_of(static_cast<uint64_t>(np) * resolve_lanes(p->spp))),
+                       dim3(kThreads), 0, nullptr, acc.as<unsigned long long>(),
+                       bad.as<uint32_t>(), p->spp, static_cast<uint32_t>(np), pix0, img.d_mean,
+                       img.d_var, dropped, resolve_lanes(p->spp));
     PM_HIP(hipGetLastError());
     ++st.batches;
     ++st.iterations;

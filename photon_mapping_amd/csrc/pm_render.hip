@@ -542,10 +542,19 @@ __global__ __launch_bounds__(kListThreads) void compact_kernel(
     const uint32_t fl = f[r];
     const uint32_t ns = fl & kSegCount;
     const uint32_t at_seg = b_seg + field21(before[r], 0);
-    for (uint32_t j = 0; j < ns; ++j) {
-      const Segment sg = seg_load(stage, 2 * i + j);
-      seg_store(next, at_seg + j, ldv(sg.o), ldv(sg.d), ldv(sg.w), sg.sample, sg.dim, sg.meta,
-                sg.stream);
+    {
+      // a segment's kSegWords words (SoA, stride apart) through restrict pointers,
+      // so the loads of later items need not wait for earlier stores
+      const uint32_t* __restrict__ src = stage.w;
+      uint32_t* __restrict__ dst = next.w;
+      const uint32_t sst = stage.stride, dstr = next.stride;
+      for (uint32_t j = 0; j < ns; ++j) {
+        uint32_t v[kSegWords];
+#pragma unroll
+        for (int k = 0; k < kSegWords; ++k) v[k] = src[2 * i + j + k * sst];
+#pragma unroll
+        for (int k = 0; k < kSegWords; ++k) dst[at_seg + j + k * dstr] = v[k];
+      }
     }
     if (fl & kFlagGlobal) {
       const uint32_t at = b_qg + field21(before[r], 1);

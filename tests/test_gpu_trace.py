@@ -166,3 +166,20 @@ def test_trace_overflow_records_match_tracing_again(gpu, rho):
         assert n_emit == outs[0][0]
         for a, b in zip(arrs, outs[0][1]):
             assert (a.view(np.uint32) == b.view(np.uint32)).all()
+
+
+def test_trace_records_option_range(gpu):
+    """PM_OPT_TRACE_RECORDS takes 0..2^30; other values are rejected with
+    PM_ERR_INVALID_ARG and leave the setting as it was."""
+    from photon_mapping_amd import _lib
+
+    ctx = gpu.Context()
+    try:
+        for bad in (-1, (1 << 30) + 1):
+            with pytest.raises(_lib.PmError) as e:
+                ctx.set_option(_lib.PM_OPT_TRACE_RECORDS, bad)
+            assert e.value.code == 1  # PM_ERR_INVALID_ARG
+        for ok in (0, 1 << 30, 1 << 18):
+            ctx.set_option(_lib.PM_OPT_TRACE_RECORDS, ok)
+    finally:
+        ctx.close()
